@@ -1,0 +1,360 @@
+/*
+ * mrmd_hip.h - C ABI of the MI355X (gfx950) implementation of MRMD's per-timestep hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns an int
+ * status (MRMD_OK == 0, negative on error; the message is in mrmd_ctx_last_error()).
+ * No exception crosses the ABI.  Each context owns one HIP stream; calls are asynchronous
+ * on it unless they return a host scalar (energies, ghost counts, displacements), in which
+ * case they synchronise - the same fence points as the reference, whose every action ends
+ * in Kokkos::fence().
+ *
+ * Reference citations (XzzX/mrmd v0.2.0, paths relative to its source tree) name the
+ * interface each entry point replaces.
+ */
+#ifndef MRMD_HIP_H
+#define MRMD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+#define MRMD_OK 0
+#define MRMD_ERR_ARG (-1)      /* invalid argument or shape                          */
+#define MRMD_ERR_HIP (-2)      /* HIP runtime error                                  */
+#define MRMD_ERR_STATE (-3)    /* call order violated (e.g. update without samples)  */
+#define MRMD_ERR_NOMEM (-4)    /* device allocation failed                           */
+#define MRMD_ERR_UNSUP (-5)    /* configuration the kernels do not support           */
+
+/* ---- opaque handles ------------------------------------------------------------------ */
+typedef struct mrmd_ctx mrmd_ctx;
+typedef struct mrmd_atoms mrmd_atoms;
+typedef struct mrmd_molecules mrmd_molecules;
+typedef struct mrmd_ghost_layer mrmd_ghost_layer;
+typedef struct mrmd_cell_list mrmd_cell_list;
+typedef struct mrmd_verlet_list mrmd_verlet_list;
+typedef struct mrmd_lj mrmd_lj;
+typedef struct mrmd_lj_idealgas mrmd_lj_idealgas;
+typedef struct mrmd_thermo_force mrmd_thermo_force;
+typedef struct mrmd_batch mrmd_batch;
+
+/* ---- plain value types ----------------------------------------------------------------- */
+/* periodic box: data::Subdomain(minCorner, maxCorner, ghostLayerThickness)
+ * (mrmd/data/Subdomain.hpp:41-62); derived corners are computed exactly as there. */
+typedef struct mrmd_box {
+    double min[3];
+    double max[3];
+    double ghost[3];
+} mrmd_box;
+
+/* One-position predicate.  Lambdas cannot cross a C ABI; these descriptors cover the
+ * predicates the hot path uses (OnePositionPredicate / OneCoordinatePredicate,
+ * mrmd/datatypes.hpp:196-210).  c = coordinate `axis` of the position.
+ *   MRMD_PRED_ALWAYS          true
+ *   MRMD_PRED_NEVER           false
+ *   MRMD_PRED_OPEN_RANGE      a < c < b
+ *   MRMD_PRED_SYMMETRIC_SLAB  b - tol <= |c - a| <= c_ + tol  (util/IsInSymmetricSlab.hpp:34-50;
+ *                             center a, slabMin b, slabMax c, tolerance tol)
+ *   MRMD_PRED_SLAB_AT/HY/CG   Slab::isIn{AT,HY,CG}Region with center a, atomistic half
+ *                             diameter b, hybrid diameter c (weighting_function/Slab.hpp:44-71)
+ *   MRMD_PRED_GREATER         c > a
+ *   MRMD_PRED_LESS            c < a
+ */
+enum {
+    MRMD_PRED_ALWAYS = 0,
+    MRMD_PRED_NEVER = 1,
+    MRMD_PRED_OPEN_RANGE = 2,
+    MRMD_PRED_SYMMETRIC_SLAB = 3,
+    MRMD_PRED_SLAB_AT = 4,
+    MRMD_PRED_SLAB_HY = 5,
+    MRMD_PRED_SLAB_CG = 6,
+    MRMD_PRED_GREATER = 7,
+    MRMD_PRED_LESS = 8
+};
+typedef struct mrmd_pred1 {
+    int32_t kind;
+    int32_t axis;
+    double a;
+    double b;
+    double c;
+    double tol;
+} mrmd_pred1;
+
+/* Two-position predicate op(first(pos_i), second(pos_j)) (TwoPositionsPredicate,
+ * mrmd/datatypes.hpp:212-222; the forms used in
+ * examples/04_LennardJones_IdealGas_LocalCap/04_LennardJones_IdealGas_LocalCap.cpp:206-227). */
+enum { MRMD_PAIR_ALWAYS = 0, MRMD_PAIR_AND = 1, MRMD_PAIR_OR = 2 };
+typedef struct mrmd_pred2 {
+    int32_t op;
+    int32_t pad;
+    mrmd_pred1 first;
+    mrmd_pred1 second;
+} mrmd_pred2;
+
+/* weighting_function::Slab(center, atomisticRegionDiameter, hybridRegionDiameter, nu,
+ * interfaceType) (mrmd/weighting_function/Slab.hpp:189-200); interface 0 SMOOTH, 1 ABRUPT */
+typedef struct mrmd_slab {
+    double center[3];
+    double atomistic_diameter;
+    double hybrid_diameter;
+    int64_t nu;
+    int32_t interface_type;
+    int32_t pad;
+} mrmd_slab;
+
+/* ---- field identifiers ------------------------------------------------------------------ */
+/* data::Atoms members (mrmd/data/Atoms.hpp:37-46).  Host-side layout of transfers:
+ * POS/VEL/FORCE n x 3 doubles (row-major), TYPE int64, MASS/CHARGE/RELMASS doubles. */
+enum {
+    MRMD_POS = 0,
+    MRMD_VEL = 1,
+    MRMD_FORCE = 2,
+    MRMD_TYPE = 3,
+    MRMD_MASS = 4,
+    MRMD_CHARGE = 5,
+    MRMD_RELMASS = 6
+};
+/* data::Molecules members (mrmd/data/Molecules.hpp:31-40): POS/FORCE/GRADLAMBDA n x 3
+ * doubles, LAMBDA/MODLAMBDA doubles, OFFSET/NUMATOMS int64. */
+enum {
+    MRMD_MOL_POS = 0,
+    MRMD_MOL_FORCE = 1,
+    MRMD_MOL_LAMBDA = 2,
+    MRMD_MOL_MODLAMBDA = 3,
+    MRMD_MOL_GRADLAMBDA = 4,
+    MRMD_MOL_OFFSET = 5,
+    MRMD_MOL_NUMATOMS = 6
+};
+
+/* =========================================================================================
+ * context  (replaces Kokkos::ScopeGuard / initialization.cpp:23-44 for this path)
+ * ========================================================================================= */
+int mrmd_ctx_create(int device, mrmd_ctx** out);
+int mrmd_ctx_destroy(mrmd_ctx* ctx);
+const char* mrmd_ctx_last_error(const mrmd_ctx* ctx);
+/* run on an external stream (e.g. torch.cuda.current_stream().cuda_stream); NULL = own */
+int mrmd_ctx_set_stream(mrmd_ctx* ctx, void* hip_stream);
+int mrmd_ctx_synchronize(mrmd_ctx* ctx);
+/* per-kernel HIP-event timing on the context stream (off by default) */
+int mrmd_ctx_set_profiling(mrmd_ctx* ctx, int enable);
+int mrmd_ctx_kernel_stats(mrmd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+int mrmd_ctx_reset_stats(mrmd_ctx* ctx);
+/* bytes of device memory currently held by this context's objects */
+int mrmd_ctx_device_bytes(mrmd_ctx* ctx, int64_t* bytes);
+
+/* =========================================================================================
+ * containers
+ * ========================================================================================= */
+/* data::Atoms(numAtoms) (Atoms.hpp:123-133): all fields zero, numLocal = numGhost = 0 */
+int mrmd_atoms_create(mrmd_ctx* ctx, int64_t size, mrmd_atoms** out);
+int mrmd_atoms_destroy(mrmd_atoms* atoms);
+/* Atoms::resize (Atoms.hpp:89-93): keeps the first min(old,new) records */
+int mrmd_atoms_resize(mrmd_atoms* atoms, int64_t size);
+int mrmd_atoms_get_counts(const mrmd_atoms* atoms, int64_t* nlocal, int64_t* nghost, int64_t* size);
+/* numLocalAtoms / numGhostAtoms (Atoms.hpp:120-121) */
+int mrmd_atoms_set_counts(mrmd_atoms* atoms, int64_t nlocal, int64_t nghost);
+/* data::deep_copy host <-> device (Atoms.hpp:150-159) for one field, records [offset, offset+n) */
+int mrmd_atoms_upload(mrmd_atoms* atoms, int field, const void* host, int64_t offset, int64_t n);
+int mrmd_atoms_download(const mrmd_atoms* atoms, int field, void* host, int64_t offset, int64_t n);
+/* Atoms::setForce / Cabana::deep_copy(slice, value) over all records (Atoms.hpp:72) */
+int mrmd_atoms_fill(mrmd_atoms* atoms, int field, double value);
+/* device pointer of a field; vector fields are component-major with the returned stride
+ * (x at ptr[i], y at ptr[stride+i], z at ptr[2*stride+i]); POS is 4 doubles per record
+ * (x, y, z, type) and returns stride 0.  Pointers change when the container grows. */
+int mrmd_atoms_device_ptr(mrmd_atoms* atoms, int field, void** ptr, int64_t* stride);
+
+int mrmd_molecules_create(mrmd_ctx* ctx, int64_t size, mrmd_molecules** out);
+int mrmd_molecules_destroy(mrmd_molecules* mols);
+int mrmd_molecules_resize(mrmd_molecules* mols, int64_t size);
+int mrmd_molecules_get_counts(const mrmd_molecules* mols, int64_t* nlocal, int64_t* nghost, int64_t* size);
+int mrmd_molecules_set_counts(mrmd_molecules* mols, int64_t nlocal, int64_t nghost);
+int mrmd_molecules_upload(mrmd_molecules* mols, int field, const void* host, int64_t offset, int64_t n);
+int mrmd_molecules_download(const mrmd_molecules* mols, int field, void* host, int64_t offset, int64_t n);
+/* Molecules::setForce (Molecules.hpp:60) */
+int mrmd_molecules_fill(mrmd_molecules* mols, int field, double value);
+/* data::createMoleculeForEachAtom (mrmd/data/MoleculesFromAtoms.cpp:19-39) */
+int mrmd_molecules_for_each_atom(mrmd_molecules* mols, const mrmd_atoms* atoms);
+
+/* =========================================================================================
+ * periodic halo  (communication::GhostLayer, mrmd/communication/GhostLayer.hpp:28-56)
+ * ========================================================================================= */
+int mrmd_ghost_layer_create(mrmd_ctx* ctx, mrmd_ghost_layer** out);
+int mrmd_ghost_layer_destroy(mrmd_ghost_layer* gl);
+/* GhostLayer::exchangeRealAtoms -> PeriodicMapping::mapIntoDomain (PeriodicMapping.cpp:30-58) */
+int mrmd_halo_wrap(mrmd_atoms* atoms, const mrmd_box* box);
+/* GhostLayer::createGhostAtoms -> GhostExchange::createGhostAtomsXYZ (GhostExchange.cpp:171-181):
+ * ordered ghost creation, X then Y then Z; returns the new ghost count */
+int mrmd_halo_create(mrmd_ghost_layer* gl, mrmd_atoms* atoms, const mrmd_box* box, int64_t* nghost);
+/* GhostExchange::createGhostAtoms for one axis (GhostExchange.cpp:59-169); reset != 0 first
+ * runs resetCorrespondingRealAtoms (GhostExchange.cpp:183-187) */
+int mrmd_halo_create_axis(mrmd_ghost_layer* gl, mrmd_atoms* atoms, const mrmd_box* box, int axis, int reset);
+/* GhostLayer::updateGhostAtoms -> UpdateGhostAtoms::updateOnlyPos (UpdateGhostAtoms.cpp:31-68) */
+int mrmd_halo_update(mrmd_ghost_layer* gl, mrmd_atoms* atoms, const mrmd_box* box);
+/* GhostLayer::contributeBackGhostToReal -> AccumulateForce::ghostToReal (AccumulateForce.cpp:25-47) */
+int mrmd_halo_fold_forces(mrmd_ghost_layer* gl, mrmd_atoms* atoms);
+/* correspondingRealAtom view (GhostExchange.hpp:39-40) */
+int mrmd_halo_get_corresponding(const mrmd_ghost_layer* gl, int64_t* host, int64_t n);
+int mrmd_halo_set_corresponding(mrmd_ghost_layer* gl, const int64_t* host, int64_t n);
+/* MultiResGhostLayer::exchangeRealAtoms -> realAtomsExchange (MultiResRealAtomsExchange.cpp:23-73) */
+int mrmd_multires_wrap(mrmd_molecules* mols, mrmd_atoms* atoms, const mrmd_box* box);
+/* MultiResGhostLayer::createGhostAtoms -> MultiResPeriodicGhostExchange::createGhostAtomsXYZ
+ * (MultiResPeriodicGhostExchange.cpp:250-265) */
+int mrmd_multires_halo_create(mrmd_ghost_layer* gl, mrmd_molecules* mols, mrmd_atoms* atoms, const mrmd_box* box);
+/* MultiResPeriodicGhostExchange::createGhostAtoms, one axis (MultiResPeriodicGhostExchange.cpp:60-248) */
+int mrmd_multires_halo_create_axis(mrmd_ghost_layer* gl, mrmd_molecules* mols, mrmd_atoms* atoms,
+                                   const mrmd_box* box, int axis, int reset);
+
+/* =========================================================================================
+ * neighbour search  (Cabana::LinkedCellList and Cabana::VerletList<HalfNeighborTag,
+ * VerletLayout2D, TeamOpTag>, typedefs mrmd/datatypes.hpp:179-194)
+ * ========================================================================================= */
+int mrmd_cell_list_create(mrmd_ctx* ctx, mrmd_cell_list** out);
+int mrmd_cell_list_destroy(mrmd_cell_list* cl);
+/* LinkedCellList(pos, begin, end, delta[3], min[3], max[3]) (tests/NVT/NVT.cpp:138-143):
+ * in-cell order is ascending particle index */
+int mrmd_cell_list_build(mrmd_cell_list* cl, const mrmd_atoms* atoms, int64_t begin, int64_t end,
+                         const double* delta, const double* grid_min, const double* grid_max);
+int mrmd_cell_list_dims(const mrmd_cell_list* cl, int32_t* dims);
+/* binning data: cell of particle begin+k, counts and offsets per cell, permutation
+ * (sorted slot k -> particle index); any pointer may be NULL */
+int mrmd_cell_list_download(const mrmd_cell_list* cl, int32_t* cell_of, int32_t* counts,
+                            int64_t* offsets, int64_t* permute);
+/* Atoms::permute(LinkedCellList&) -> Cabana::permute (Atoms.hpp:95) */
+int mrmd_atoms_permute(mrmd_atoms* atoms, const mrmd_cell_list* cl);
+
+int mrmd_verlet_create(mrmd_ctx* ctx, mrmd_verlet_list** out);
+int mrmd_verlet_destroy(mrmd_verlet_list* vl);
+/* HalfVerletList::build(pos, begin, end, neighborhoodRadius, cellRatio, gridMin, gridMax,
+ * maxNeigh) (examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:156-163): binning of every
+ * record (local and ghost), 27-cell stencil with Cabana's min-distance prune, d^2 <= r^2.
+ * The half list (Cabana's position-lexicographic discriminator) is exact; the device keeps
+ * both directions so force kernels can run without atomics. */
+int mrmd_verlet_build(mrmd_verlet_list* vl, const mrmd_atoms* atoms, int64_t begin, int64_t end,
+                      double r_n, double cell_ratio, const double* grid_min, const double* grid_max,
+                      int64_t max_neigh);
+/* the same over molecule centres (LJ_IdealGas molecule list, LJ_IdealGas.test.cpp:108-120) */
+int mrmd_verlet_build_molecules(mrmd_verlet_list* vl, const mrmd_molecules* mols, int64_t begin,
+                                int64_t end, double r_n, double cell_ratio, const double* grid_min,
+                                const double* grid_max, int64_t max_neigh);
+/* HalfNeighborList::numNeighbor for every record (counts of the half list, zero outside
+ * [begin, end)); total = sum of counts (tests/LennardJones/LennardJones.cpp:72-80) */
+int mrmd_verlet_half_counts(mrmd_verlet_list* vl, int32_t* counts, int64_t n, int64_t* total);
+/* HalfNeighborList::getNeighbor rows [0, n) of the half list into host[row * row_stride + k] */
+int mrmd_verlet_half_neighbors(mrmd_verlet_list* vl, int32_t* host, int64_t n, int64_t row_stride);
+/* both-direction rows (record begin+r in row r), same layout; *max_count is the widest row */
+int mrmd_verlet_full_neighbors(mrmd_verlet_list* vl, int32_t* host, int32_t* counts, int64_t n,
+                               int64_t row_stride);
+int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* max_full, int64_t* total_full,
+                     int32_t* dims);
+
+/* =========================================================================================
+ * Lennard-Jones  (action::LennardJones, mrmd/action/LennardJones.hpp:98-206)
+ * ========================================================================================= */
+/* CappedLennardJonesPotential(cap[], rc[], sigma[], eps[], numTypes, isShifted)
+ * (LennardJones.cpp:81-116, 61-79).  type_stride selects the pair index
+ * type_i * type_stride + type_j: LennardJones hard-codes 1 (LennardJones.cpp:52),
+ * LJ_IdealGas uses numTypes (LJ_IdealGas.cpp:139). */
+int mrmd_lj_create(mrmd_ctx* ctx, const double* cap, const double* rc, const double* sigma,
+                   const double* epsilon, int64_t num_types, int shifted, int64_t type_stride, mrmd_lj** out);
+int mrmd_lj_destroy(mrmd_lj* lj);
+/* precomputed table, num_types^2 x 10 doubles in PrecomputedValues order (LennardJones.hpp:27-39) */
+int mrmd_lj_table(const mrmd_lj* lj, double* host);
+/* computeForceAndEnergy(distSqr, typeIdx) (LennardJones.hpp:52-78) on n host values, on the device */
+int mrmd_lj_eval(mrmd_lj* lj, const double* dist_sqr, const int64_t* type_idx, int64_t n,
+                 double* force_factor, double* energy);
+/* LennardJones::apply / apply_if (LennardJones.hpp:136-206): adds pair forces to local atoms.
+ * pred == NULL is apply().  Energy and virial stay on the device until requested. */
+int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* atoms, mrmd_verlet_list* vl, const mrmd_pred2* pred);
+/* getEnergy / getVirial (LennardJones.cpp:35-36); synchronises */
+int mrmd_lj_energy_virial(mrmd_lj* lj, double* energy, double* virial);
+/* Adjoint of the force map F(x; eps, sigma) (no reference counterpart; for force matching):
+ * given upstream g = dL/dF for local atoms (n x 3), returns dL/dx (n x 3, local atoms) and
+ * dL/d(eps_ab), dL/d(sigma_ab) per type pair (num_types^2 each, symmetric: entry (a,b) =
+ * entry (b,a) = derivative w.r.t. the shared pair parameter).  Pair terms are recomputed,
+ * never stored.  gl (the ghost layer that created the ghosts) is required when ghosts
+ * exist: a ghost partner takes the upstream gradient of its real atom.  device != 0: dLdF
+ * and dLdx are device pointers (the parameter gradients are always host arrays). */
+int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* atoms, mrmd_verlet_list* vl, const mrmd_ghost_layer* gl,
+                    const double* dLdF, double* dLdx, double* dLdeps, double* dLdsigma, int device);
+/* reparameterise an existing table in place (same cap/rc/shift flags) */
+int mrmd_lj_set_params(mrmd_lj* lj, const double* sigma, const double* epsilon);
+
+/* =========================================================================================
+ * integrators  (action::VelocityVerlet, VelocityVerletLangevinThermostat)
+ * ========================================================================================= */
+/* VelocityVerlet::preForceIntegrate (VelocityVerlet.cpp:26-67); max_disp may be NULL (the
+ * value then stays on the device and accumulates for mrmd_rebuild_check) */
+int mrmd_vv_pre(mrmd_atoms* atoms, double dt, double* max_disp);
+/* VelocityVerlet::postForceIntegrate (VelocityVerlet.cpp:69-90) */
+int mrmd_vv_post(mrmd_atoms* atoms, double dt);
+/* VelocityVerletLangevinThermostat::preForceIntegrate_apply_if
+ * (VelocityVerletLangevinThermostat.hpp:63-136).  Noise: Philox4x32-10 keyed by seed,
+ * counter (call, atom index) - replaces Random_XorShift1024_Pool(1234). pred NULL = always. */
+int mrmd_langevin_pre(mrmd_atoms* atoms, double dt, double zeta, double temperature, uint64_t seed,
+                      uint64_t call, const mrmd_pred1* pred, double* max_disp);
+/* analysis::getKineticEnergy (mrmd/analysis/KineticEnergy.hpp:26-39); synchronises */
+int mrmd_kinetic_energy(const mrmd_atoms* atoms, double* energy);
+
+/* =========================================================================================
+ * AdResS  (weighting_function::Slab, UpdateMolecules, LJ_IdealGas,
+ *          ContributeMoleculeForceToAtoms, ThermodynamicForce)
+ * ========================================================================================= */
+/* Slab::operator() (Slab.hpp:161-187) on n host x values: out5[i] = lambda, modLambda, grad */
+int mrmd_slab_eval(mrmd_ctx* ctx, const mrmd_slab* slab, const double* x, int64_t n, double* out5);
+/* UpdateMolecules::update (mrmd/action/UpdateMolecules.hpp:24-70); slab NULL = COM only */
+int mrmd_molecules_update(mrmd_molecules* mols, const mrmd_atoms* atoms, const mrmd_slab* slab);
+/* ContributeMoleculeForceToAtoms::update (ContributeMoleculeForceToAtoms.cpp:23-48) */
+int mrmd_molecules_contribute_force(const mrmd_molecules* mols, mrmd_atoms* atoms);
+
+/* LJ_IdealGas(cap[], rc[], sigma[], eps[], numTypes, doShift) (LJ_IdealGas.cpp:271-292) */
+int mrmd_ljig_create(mrmd_ctx* ctx, const double* cap, const double* rc, const double* sigma,
+                     const double* epsilon, int64_t num_types, int shifted, mrmd_lj_idealgas** out);
+int mrmd_ljig_destroy(mrmd_lj_idealgas* s);
+/* setCompensationEnergySamplingInterval / setCompensationEnergyUpdateInterval (LJ_IdealGas.hpp:73-80) */
+int mrmd_ljig_set_intervals(mrmd_lj_idealgas* s, int64_t sampling, int64_t update);
+/* LJ_IdealGas::run (LJ_IdealGas.cpp:227-260); synchronises for the energy */
+int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* mols, mrmd_verlet_list* vl, mrmd_atoms* atoms,
+                  double* energy);
+/* which: 0 compensation energy, 1 counter, 2 mean (getMeanCompensationEnergy), 200 x numTypes */
+int mrmd_ljig_get_histogram(const mrmd_lj_idealgas* s, int which, double* host);
+int mrmd_ljig_set_histogram(mrmd_lj_idealgas* s, int which, const double* host);
+
+/* ThermodynamicForce(targetDensity[], subdomain, binWidth, modulation[], enforceSymmetry,
+ * usePeriodicity) (ThermodynamicForce.cpp:25-57) */
+int mrmd_tf_create(mrmd_ctx* ctx, const double* target_density, const double* modulation,
+                   int64_t num_types, const mrmd_box* box, double bin_width, int enforce_symmetry,
+                   int use_periodicity, mrmd_thermo_force** out);
+int mrmd_tf_destroy(mrmd_thermo_force* tf);
+int mrmd_tf_num_bins(const mrmd_thermo_force* tf, int64_t* nbins);
+/* ThermodynamicForce::sample (ThermodynamicForce.cpp:74-86) */
+int mrmd_tf_sample(mrmd_thermo_force* tf, const mrmd_atoms* atoms);
+/* update / update_if (ThermodynamicForce.hpp:124-152); pred over the bin centre, NULL = always */
+int mrmd_tf_update(mrmd_thermo_force* tf, double smoothing_sigma, double smoothing_range,
+                   const mrmd_pred1* pred);
+/* apply / apply_if (ThermodynamicForce.hpp:98-122) */
+int mrmd_tf_apply(const mrmd_thermo_force* tf, mrmd_atoms* atoms, const mrmd_pred1* pred);
+/* applyInterpolated_if (ThermodynamicForce.hpp:154-216) */
+int mrmd_tf_apply_interpolated(const mrmd_thermo_force* tf, mrmd_atoms* atoms, const mrmd_pred1* pred);
+/* which: 0 force (getForce), 1 density profile (getDensityProfile); nbins x numTypes */
+int mrmd_tf_get(const mrmd_thermo_force* tf, int which, double* host);
+/* setForce (ThermodynamicForce.hpp:55) */
+int mrmd_tf_set_force(mrmd_thermo_force* tf, const double* host);
+/* getNumberOfDensityProfileSamples (ThermodynamicForce.hpp:63) */
+int mrmd_tf_samples(const mrmd_thermo_force* tf, int64_t* samples);
+/* getMuLeft / getMuRight (ThermodynamicForce.cpp:98-130) */
+int mrmd_tf_mu(const mrmd_thermo_force* tf, double* left, double* right);
+
+/* MultiHistogram operations on host arrays [nbins x nhist], executed on the device
+ * (MultiHistogram.cpp:113-212, MultiHistogram.cpp:76-90) */
+int mrmd_hist_gradient(mrmd_ctx* ctx, double min, double max, int64_t nbins, int64_t nhist,
+                       const double* in, int periodic, double* out);
+int mrmd_hist_smoothen(mrmd_ctx* ctx, double min, double max, int64_t nbins, int64_t nhist,
+                       const double* in, double sigma, double range, int periodic, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRMD_HIP_H */

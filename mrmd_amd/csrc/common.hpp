@@ -1,0 +1,404 @@
+// Shared device/host infrastructure of the gfx950 library: context, errors, device
+// buffers, launch helpers, per-kernel event timing and wave-level primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mrmd_hip.h"
+
+namespace mrmd_hip
+{
+constexpr int WAVE = 64;    // CDNA wavefront width
+constexpr int BLOCK = 256;  // 4 waves per workgroup
+
+// error carried through the C ABI as (code, message)
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Error(code, buf);
+}
+
+#define HIPCHECK(call)                                                                    \
+    do                                                                                    \
+    {                                                                                     \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            ::mrmd_hip::fail(e_ == hipErrorOutOfMemory ? MRMD_ERR_NOMEM : MRMD_ERR_HIP,   \
+                             "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                             __FILE__, __LINE__);                                         \
+    } while (0)
+
+#define REQUIRE(cond, ...)                                      \
+    do                                                          \
+    {                                                           \
+        if (!(cond)) ::mrmd_hip::fail(MRMD_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+struct KStat
+{
+    int64_t launches = 0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+}  // namespace mrmd_hip
+
+struct mrmd_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::string err;
+    bool profiling = false;
+    std::map<std::string, mrmd_hip::KStat> stats;
+    std::vector<hipEvent_t> eventPool;
+    double* pinned = nullptr;  // small pinned host staging area (64 doubles)
+    int64_t bytes = 0;         // device bytes held by objects of this context
+    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};  // per-purpose device scratch
+    size_t scratchBytes[4] = {0, 0, 0, 0};
+};
+
+namespace mrmd_hip
+{
+// ---------------------------------------------------------------------------------------
+// device buffer with explicit growth; contents are preserved on grow when asked to
+template <class T>
+struct DBuf
+{
+    T* p = nullptr;
+    int64_t n = 0;
+    mrmd_ctx* ctx = nullptr;
+
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+
+    void release()
+    {
+        if (p)
+        {
+            (void)hipFree(p);
+            if (ctx) ctx->bytes -= n * int64_t(sizeof(T));
+        }
+        p = nullptr;
+        n = 0;
+    }
+    // ensure capacity >= m elements; keep the first `keep` elements
+    void reserve(mrmd_ctx* c, int64_t m, int64_t keep = 0)
+    {
+        if (m <= n) return;
+        ctx = c;
+        T* q = nullptr;
+        HIPCHECK(hipMalloc(&q, size_t(m) * sizeof(T)));
+        if (p && keep > 0)
+            HIPCHECK(hipMemcpyAsync(q, p, size_t(std::min(keep, n)) * sizeof(T), hipMemcpyDeviceToDevice,
+                                    c->stream));
+        if (p)
+        {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            (void)hipFree(p);
+            c->bytes -= n * int64_t(sizeof(T));
+        }
+        p = q;
+        n = m;
+        c->bytes += m * int64_t(sizeof(T));
+    }
+};
+
+// per-context device scratch slot (0: scan partial sums, 1: reduction partials,
+// 2: small device scalars); grows, never shrinks, freed with the context
+inline void* ctxScratch(mrmd_ctx* ctx, int slot, size_t bytes)
+{
+    if (bytes > ctx->scratchBytes[slot])
+    {
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->scratch[slot]) (void)hipFree(ctx->scratch[slot]);
+        ctx->scratch[slot] = nullptr;
+        size_t nb = bytes + bytes / 2 + 4096;
+        HIPCHECK(hipMalloc(&ctx->scratch[slot], nb));
+        ctx->scratchBytes[slot] = nb;
+    }
+    return ctx->scratch[slot];
+}
+
+inline int64_t growCap(int64_t need) { return need + need / 4 + 256; }
+
+inline unsigned gridFor(int64_t n, int block = BLOCK)
+{
+    int64_t g = (n + block - 1) / block;
+    return unsigned(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------------------
+// per-kernel event timing on the context stream
+struct KTimer
+{
+    mrmd_ctx* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(mrmd_ctx* c, const char* n) : ctx(c), name(n)
+    {
+        if (!ctx->profiling) return;
+        a = take();
+        b = take();
+        HIPCHECK(hipEventRecord(a, ctx->stream));
+    }
+    ~KTimer()
+    {
+        if (!ctx->profiling || !a) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->stats[name].pending.push_back({a, b});
+    }
+    hipEvent_t take()
+    {
+        hipEvent_t e;
+        if (!ctx->eventPool.empty())
+        {
+            e = ctx->eventPool.back();
+            ctx->eventPool.pop_back();
+        }
+        else
+            HIPCHECK(hipEventCreate(&e));
+        return e;
+    }
+};
+
+inline void flushStats(mrmd_ctx* ctx)
+{
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->stats)
+    {
+        for (auto& pr : kv.second.pending)
+        {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            kv.second.ms += ms;
+            kv.second.launches += 1;
+            ctx->eventPool.push_back(pr.first);
+            ctx->eventPool.push_back(pr.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+
+inline void sync(mrmd_ctx* ctx) { HIPCHECK(hipStreamSynchronize(ctx->stream)); }
+
+inline void checkLaunch() { HIPCHECK(hipGetLastError()); }
+
+// ---------------------------------------------------------------------------------------
+// predicates (descriptor form; semantics in mrmd_hip.h)
+__host__ __device__ inline bool evalPred1(const mrmd_pred1& p, double x, double y, double z)
+{
+    const double v = p.axis == 0 ? x : (p.axis == 1 ? y : z);
+    switch (p.kind)
+    {
+        case MRMD_PRED_ALWAYS:
+            return true;
+        case MRMD_PRED_NEVER:
+            return false;
+        case MRMD_PRED_OPEN_RANGE:
+            return v > p.a && v < p.b;
+        case MRMD_PRED_SYMMETRIC_SLAB:
+        {
+            double absDx = fabs(v - p.a);
+            return absDx >= p.b - p.tol && absDx <= p.c + p.tol;
+        }
+        case MRMD_PRED_SLAB_AT:
+            return fabs(v - p.a) < p.b;
+        case MRMD_PRED_SLAB_HY:
+        {
+            double absDx = fabs(v - p.a);
+            if (absDx < p.b) return false;
+            return absDx < p.b + p.c;
+        }
+        case MRMD_PRED_SLAB_CG:
+            return !(fabs(v - p.a) < p.b + p.c);
+        case MRMD_PRED_GREATER:
+            return v > p.a;
+        case MRMD_PRED_LESS:
+            return v < p.a;
+        default:
+            return false;
+    }
+}
+
+__host__ __device__ inline bool evalPred2(const mrmd_pred2& p,
+                                          double xi, double yi, double zi,
+                                          double xj, double yj, double zj)
+{
+    switch (p.op)
+    {
+        case MRMD_PAIR_ALWAYS:
+            return true;
+        case MRMD_PAIR_AND:
+            return evalPred1(p.first, xi, yi, zi) && evalPred1(p.second, xj, yj, zj);
+        case MRMD_PAIR_OR:
+            return evalPred1(p.first, xi, yi, zi) || evalPred1(p.second, xj, yj, zj);
+        default:
+            return false;
+    }
+}
+
+inline bool predIsSymmetric(const mrmd_pred2* p)
+{
+    if (!p || p->op == MRMD_PAIR_ALWAYS) return true;
+    const auto& a = p->first;
+    const auto& b = p->second;
+    return a.kind == b.kind && a.axis == b.axis && a.a == b.a && a.b == b.b && a.c == b.c && a.tol == b.tol;
+}
+
+inline void validatePred1(const mrmd_pred1* p)
+{
+    if (!p) return;
+    REQUIRE(p->kind >= 0 && p->kind <= MRMD_PRED_LESS, "unknown predicate kind %d", p->kind);
+    REQUIRE(p->axis >= 0 && p->axis <= 2, "predicate axis %d out of range", p->axis);
+}
+inline void validatePred2(const mrmd_pred2* p)
+{
+    if (!p) return;
+    REQUIRE(p->op >= 0 && p->op <= MRMD_PAIR_OR, "unknown pair predicate op %d", p->op);
+    validatePred1(&p->first);
+    validatePred1(&p->second);
+}
+
+// ---------------------------------------------------------------------------------------
+// wave / block reductions (wave64 shuffles, then LDS across the 4 waves of a block)
+__device__ inline double waveSum(double v)
+{
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+__device__ inline double waveMax(double v)
+{
+    for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+// block of BLOCK threads; result valid in thread 0
+__device__ inline double blockSum(double v, double* lds /*[BLOCK/WAVE]*/)
+{
+    v = waveSum(v);
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BLOCK / WAVE; ++i) r += lds[i];
+    __syncthreads();
+    return r;
+}
+__device__ inline double blockMax(double v, double* lds)
+{
+    v = waveMax(v);
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BLOCK / WAVE; ++i) r = fmax(r, lds[i]);
+    __syncthreads();
+    return r;
+}
+
+template <class T>
+__device__ inline T waveInclusiveScan(T v)
+{
+    const int lane = threadIdx.x % WAVE;
+    for (int off = 1; off < WAVE; off <<= 1)
+    {
+        T o = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over a BLOCK-thread workgroup; *total = block sum
+template <class T>
+__device__ inline T blockExclusiveScan(T v, T* lds /*[BLOCK/WAVE]*/, T* total)
+{
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    T inc = waveInclusiveScan(v);
+    if (lane == WAVE - 1) lds[w] = inc;
+    __syncthreads();
+    T waveOff = 0, tot = 0;
+    for (int i = 0; i < BLOCK / WAVE; ++i)
+    {
+        if (i < w) waveOff += lds[i];
+        tot += lds[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return waveOff + inc - v;
+}
+
+// rank of this lane among the lanes of its wave whose flag is set (wave ballot + popcount)
+__device__ inline int waveRank(unsigned long long mask)
+{
+    const unsigned lane = threadIdx.x % WAVE;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return __popcll(mask & below);
+}
+
+// non-negative doubles order like their bit patterns
+__device__ inline void atomicMaxNonNeg(double* addr, double v)
+{
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ inline void atomicAddD(double* addr, double v) { unsafeAtomicAdd(addr, v); }
+
+// XCD-aware block remap: gfx950 dispatches consecutive workgroups round-robin over 8 XCDs,
+// each with its own L2.  Remapping gives every XCD a contiguous range of workgroups, so
+// spatially neighbouring atoms (cell-sorted) share one L2.
+__device__ inline unsigned xcdBlock()
+{
+    const unsigned n = gridDim.x, b = blockIdx.x;
+    const unsigned x = b & 7u, k = b >> 3;
+    const unsigned per = n >> 3;        // full rounds
+    const unsigned rem = n & 7u;        // XCDs with one extra block
+    // XCD x owns per + (x < rem) blocks, starting after the blocks of XCDs < x
+    const unsigned start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+
+// exclusive scan of int64 counts on the device (small helper used by binning/compaction)
+void scanExclusiveI64(mrmd_ctx* ctx, const int32_t* in, int64_t* out, int64_t n, int64_t* totalDev);
+void scanExclusiveI32(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int32_t* totalDev);
+
+}  // namespace mrmd_hip
+
+// C-ABI wrapper: converts exceptions into status codes and records the message
+#define MRMD_API_BEGIN(ctxexpr) \
+    mrmd_ctx* api_ctx_ = (ctxexpr); \
+    try                             \
+    {
+#define MRMD_API_END                                   \
+    }                                                  \
+    catch (const ::mrmd_hip::Error& e)                 \
+    {                                                  \
+        if (api_ctx_) api_ctx_->err = e.what();        \
+        return e.code;                                 \
+    }                                                  \
+    catch (const std::exception& e)                    \
+    {                                                  \
+        if (api_ctx_) api_ctx_->err = e.what();        \
+        return MRMD_ERR_HIP;                           \
+    }                                                  \
+    return MRMD_OK;

@@ -1,0 +1,100 @@
+// Device layouts of data::Atoms and data::Molecules (mrmd/data/Atoms.hpp:33-177,
+// mrmd/data/Molecules.hpp:27-176), redesigned for HBM streaming and neighbour gathers:
+//   pos    : double4 {x, y, z, type} per record - one 32-byte aligned read per neighbour
+//            gather; the type travels with the position so force kernels need no
+//            second gather
+//   vel    : component-major SoA, vel[d * cap + i]
+//   force  : component-major SoA, force[d * cap + i]
+//   mass, charge, relMass : SoA
+// Records [0, nlocal) are local atoms, [nlocal, nlocal + nghost) ghosts (the reference's
+// "HaloAtoms" segment).
+#pragma once
+
+#include "common.hpp"
+
+namespace mrmd_hip
+{
+struct AtomsView
+{
+    double4* pos;
+    double* vel;
+    double* force;
+    double* mass;
+    double* charge;
+    double* relmass;
+    int64_t cap;
+};
+
+struct MolView
+{
+    double4* pos;  ///< {x, y, z, unused}
+    double* force;  ///< [3][cap]
+    double* lambda;
+    double* modLambda;
+    double* grad;  ///< [3][cap]
+    int64_t* offset;
+    int64_t* numAtoms;
+    int64_t cap;
+};
+}  // namespace mrmd_hip
+
+struct mrmd_atoms
+{
+    mrmd_ctx* ctx = nullptr;
+    int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
+    mrmd_hip::DBuf<double4> pos;
+    mrmd_hip::DBuf<double> vel, force, mass, charge, relmass;
+    mrmd_hip::DBuf<double> stage;  ///< transfer staging
+
+    mrmd_hip::AtomsView view() const
+    {
+        return {pos.p, vel.p, force.p, mass.p, charge.p, relmass.p, cap};
+    }
+    /// grow the capacity to at least `need` records, preserving [0, size)
+    void ensureCap(int64_t need);
+    /// set size (records), growing when needed; new records are zero
+    void setSize(int64_t n);
+};
+
+struct mrmd_molecules
+{
+    mrmd_ctx* ctx = nullptr;
+    int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
+    mrmd_hip::DBuf<double4> pos;
+    mrmd_hip::DBuf<double> force, lambda, modLambda, grad;
+    mrmd_hip::DBuf<int64_t> offset, numAtoms;
+    mrmd_hip::DBuf<double> stage;
+
+    mrmd_hip::MolView view() const
+    {
+        return {pos.p, force.p, lambda.p, modLambda.p, grad.p, offset.p, numAtoms.p, cap};
+    }
+    void ensureCap(int64_t need);
+    void setSize(int64_t n);
+};
+
+namespace mrmd_hip
+{
+// copy record src -> dst (all fields) with a shift of `shift` on axis `axis`
+__device__ inline void copyAtom(const AtomsView& a, int64_t dst, int64_t src, int axis, double shift)
+{
+    double4 p = a.pos[src];
+    if (axis == 0) p.x += shift;
+    if (axis == 1) p.y += shift;
+    if (axis == 2) p.z += shift;
+    a.pos[dst] = p;
+    for (int d = 0; d < 3; ++d)
+    {
+        a.vel[d * a.cap + dst] = a.vel[d * a.cap + src];
+        a.force[d * a.cap + dst] = a.force[d * a.cap + src];
+    }
+    a.mass[dst] = a.mass[src];
+    a.charge[dst] = a.charge[src];
+    a.relmass[dst] = a.relmass[src];
+}
+
+__device__ inline double coord(const double4& p, int axis)
+{
+    return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+}
+}  // namespace mrmd_hip

@@ -1,0 +1,634 @@
+// Periodic halo on one device: wrap, ordered ghost creation, ghost refresh and force fold,
+// for atoms (communication::GhostLayer) and molecules (communication::MultiResGhostLayer).
+//
+// Ghost creation keeps the reference's exact index order (GhostExchange.cpp:76-106, a
+// Kokkos parallel_scan): per axis, the selected records of [0, nlocal + nghost) are
+// appended in index order, first the block shifted by +L (x < minInner) then the block
+// shifted by -L (x >= maxInner).  The compaction is a 1024-record tile count, a device
+// scan over tiles, and a scatter pass ranking records inside the tile with wave ballots.
+#include "containers.hpp"
+
+using namespace mrmd_hip;
+
+struct mrmd_ghost_layer
+{
+    mrmd_ctx* ctx = nullptr;
+    DBuf<int32_t> corr;  ///< correspondingRealAtom, -1 for real atoms
+    int64_t corrSize = 0;
+    DBuf<int32_t> tileCnt;   ///< 4 x tiles counts
+    DBuf<int32_t> tileOff;   ///< 4 x tiles exclusive offsets
+};
+
+namespace mrmd_hip
+{
+const int32_t* ghostCorrDevice(const mrmd_ghost_layer* gl) { return gl ? gl->corr.p : nullptr; }
+}  // namespace mrmd_hip
+
+namespace
+{
+constexpr int TILE = BLOCK * 4;
+
+struct SubD
+{
+    double min[3], max[3], minInner[3], maxInner[3], diam[3];
+};
+
+// derived corners exactly as data::Subdomain (Subdomain.hpp:48-61)
+SubD makeSub(const mrmd_box* b)
+{
+    SubD s;
+    for (int d = 0; d < 3; ++d)
+    {
+        s.min[d] = b->min[d];
+        s.max[d] = b->max[d];
+        s.minInner[d] = b->min[d] + b->ghost[d];
+        s.maxInner[d] = b->max[d] - b->ghost[d];
+        s.diam[d] = b->max[d] - b->min[d];
+    }
+    return s;
+}
+
+// PeriodicMapping.cpp:30-58
+__global__ void k_wrap(double4* pos, int64_t n, SubD s)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (s.max[d] <= x[d])
+        {
+            x[d] -= s.diam[d];
+            x[d] = fmax(x[d], s.min[d]);
+        }
+        if (x[d] < s.min[d])
+        {
+            x[d] += s.diam[d];
+            if (s.max[d] <= x[d]) x[d] = s.min[d];
+        }
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+}
+
+__global__ void k_fillI32(int32_t* p, int64_t n, int32_t v)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// per tile: number of records with x < minInner (P, copied +L) and x >= maxInner (N, -L)
+__global__ void k_ghostCount(const double4* pos, int64_t total, int axis, double lo, double hi,
+                             int32_t* cntP, int32_t* cntN)
+{
+    __shared__ int sP[BLOCK / WAVE], sN[BLOCK / WAVE];
+    const int64_t base = int64_t(blockIdx.x) * TILE;
+    int cP = 0, cN = 0;
+    for (int r = 0; r < 4; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        if (e < total)
+        {
+            const double x = coord(pos[e], axis);
+            cP += x < lo;
+            cN += x >= hi;
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+    {
+        cP += __shfl_xor(cP, off, WAVE);
+        cN += __shfl_xor(cN, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0)
+    {
+        sP[threadIdx.x / WAVE] = cP;
+        sN[threadIdx.x / WAVE] = cN;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int a = 0, b = 0;
+        for (int w = 0; w < BLOCK / WAVE; ++w)
+        {
+            a += sP[w];
+            b += sN[w];
+        }
+        cntP[blockIdx.x] = a;
+        cntN[blockIdx.x] = b;
+    }
+}
+
+// GhostExchange.cpp:119-165: copy selected records in index order
+__global__ void k_ghostScatter(AtomsView a, int32_t* corr, int64_t total, int axis, double lo, double hi,
+                               double L, const int32_t* offP, const int32_t* offN, const int32_t* totP)
+{
+    __shared__ int wP[BLOCK / WAVE], wN[BLOCK / WAVE];
+    const int64_t base = int64_t(blockIdx.x) * TILE;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int64_t nP = *totP;
+    int runP = 0, runN = 0;
+    for (int r = 0; r < 4; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        bool fP = false, fN = false;
+        if (e < total)
+        {
+            const double x = coord(a.pos[e], axis);
+            fP = x < lo;
+            fN = x >= hi;
+        }
+        const unsigned long long mP = __ballot(fP), mN = __ballot(fN);
+        if (lane == 0)
+        {
+            wP[w] = __popcll(mP);
+            wN[w] = __popcll(mN);
+        }
+        __syncthreads();
+        int oP = 0, oN = 0, tP = 0, tN = 0;
+        for (int i = 0; i < BLOCK / WAVE; ++i)
+        {
+            if (i < w)
+            {
+                oP += wP[i];
+                oN += wN[i];
+            }
+            tP += wP[i];
+            tN += wN[i];
+        }
+        __syncthreads();
+        if (fP)
+        {
+            const int64_t g = total + offP[blockIdx.x] + runP + oP + waveRank(mP);
+            copyAtom(a, g, e, axis, +L);
+            const int32_t c = corr[e];
+            corr[g] = c == -1 ? int32_t(e) : c;
+        }
+        if (fN)
+        {
+            const int64_t g = total + nP + offN[blockIdx.x] + runN + oN + waveRank(mN);
+            copyAtom(a, g, e, axis, -L);
+            const int32_t c = corr[e];
+            corr[g] = c == -1 ? int32_t(e) : c;
+        }
+        runP += tP;
+        runN += tN;
+    }
+}
+
+// UpdateGhostAtoms.cpp:31-68
+__global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, int64_t end, SubD s)
+{
+    const int64_t i = begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const int32_t real = corr[i];
+    double4 p = pos[i];
+    const double4 q = pos[real];
+    const double dx[3] = {p.x - q.x, p.y - q.y, p.z - q.z};
+    double x[3] = {q.x, q.y, q.z};
+    for (int d = 0; d < 3; ++d)
+        if (dx[d] > +(0.1 * s.diam[d])) x[d] += s.diam[d];
+    for (int d = 0; d < 3; ++d)
+        if (dx[d] < -(0.1 * s.diam[d])) x[d] -= s.diam[d];
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    pos[i] = p;
+}
+
+// AccumulateForce.cpp:25-47
+__global__ void k_ghostFold(double* force, int64_t cap, const int32_t* corr, int64_t begin, int64_t end)
+{
+    const int64_t i = begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const int32_t real = corr[i];
+    if (real == -1) return;
+    for (int d = 0; d < 3; ++d)
+    {
+        const double f = force[d * cap + i];
+        if (f != 0.0)
+        {
+            atomicAddD(&force[d * cap + real], f);
+            force[d * cap + i] = 0.0;
+        }
+    }
+}
+
+// MultiResRealAtomsExchange.cpp:23-73
+__global__ void k_molWrap(MolView m, AtomsView a, int64_t n, SubD s)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = m.pos[i];
+    double X[3] = {p.x, p.y, p.z};
+    const int64_t start = m.offset[i], end = start + m.numAtoms[i];
+    for (int d = 0; d < 3; ++d)
+    {
+        double shift = 0.0;
+        if (s.max[d] <= X[d])
+        {
+            X[d] -= s.diam[d];
+            shift = -s.diam[d];
+            for (int64_t k = start; k < end; ++k)
+            {
+                double4 q = a.pos[k];
+                if (d == 0) q.x -= s.diam[d];
+                if (d == 1) q.y -= s.diam[d];
+                if (d == 2) q.z -= s.diam[d];
+                a.pos[k] = q;
+            }
+        }
+        if (X[d] < s.min[d])
+        {
+            X[d] += s.diam[d];
+            for (int64_t k = start; k < end; ++k)
+            {
+                double4 q = a.pos[k];
+                if (d == 0) q.x += s.diam[d];
+                if (d == 1) q.y += s.diam[d];
+                if (d == 2) q.z += s.diam[d];
+                a.pos[k] = q;
+            }
+        }
+        (void)shift;
+    }
+    p.x = X[0];
+    p.y = X[1];
+    p.z = X[2];
+    m.pos[i] = p;
+}
+
+// per tile: molecules / atoms with centre >= maxInner (P) and < minInner (N)
+__global__ void k_molGhostCount(MolView m, int64_t total, int axis, double lo, double hi, int32_t* cnt,
+                                int64_t tiles)
+{
+    __shared__ int sh[4][BLOCK / WAVE];
+    const int64_t base = int64_t(blockIdx.x) * TILE;
+    int c[4] = {0, 0, 0, 0};  // molP, atomP, molN, atomN
+    for (int r = 0; r < 4; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        if (e < total)
+        {
+            const double x = coord(m.pos[e], axis);
+            const int na = int(m.numAtoms[e]);
+            if (x >= hi)
+            {
+                c[0] += 1;
+                c[1] += na;
+            }
+            if (x < lo)
+            {
+                c[2] += 1;
+                c[3] += na;
+            }
+        }
+    }
+    for (int k = 0; k < 4; ++k)
+    {
+        int v = c[k];
+        for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+        if (threadIdx.x % WAVE == 0) sh[k][threadIdx.x / WAVE] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4)
+    {
+        int v = 0;
+        for (int w = 0; w < BLOCK / WAVE; ++w) v += sh[threadIdx.x][w];
+        cnt[threadIdx.x * tiles + blockIdx.x] = v;
+    }
+}
+
+__device__ inline void copyMol(const MolView& m, int64_t dst, int64_t src, int axis, double shift)
+{
+    double4 p = m.pos[src];
+    if (axis == 0) p.x += shift;
+    if (axis == 1) p.y += shift;
+    if (axis == 2) p.z += shift;
+    m.pos[dst] = p;
+    for (int d = 0; d < 3; ++d)
+    {
+        m.force[d * m.cap + dst] = m.force[d * m.cap + src];
+        m.grad[d * m.cap + dst] = m.grad[d * m.cap + src];
+    }
+    m.lambda[dst] = m.lambda[src];
+    m.modLambda[dst] = m.modLambda[src];
+}
+
+// MultiResPeriodicGhostExchange.cpp:140-240
+__global__ void k_molGhostScatter(MolView m, AtomsView a, int32_t* corr, int64_t totM, int64_t totA, int axis,
+                                  double lo, double hi, double L, const int32_t* off, int64_t tiles,
+                                  const int32_t* tot)
+{
+    __shared__ int lds[BLOCK / WAVE];
+    const int64_t base = int64_t(blockIdx.x) * TILE;
+    const int64_t nPM = tot[0], nPA = tot[1];
+    int run[4] = {0, 0, 0, 0};
+    for (int r = 0; r < 4; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        bool fP = false, fN = false;
+        int na = 0;
+        if (e < totM)
+        {
+            const double x = coord(m.pos[e], axis);
+            fP = x >= hi;
+            fN = x < lo;
+            na = int(m.numAtoms[e]);
+        }
+        int t[4];
+        const int ex0 = blockExclusiveScan<int>(fP ? 1 : 0, lds, &t[0]);
+        const int ex1 = blockExclusiveScan<int>(fP ? na : 0, lds, &t[1]);
+        const int ex2 = blockExclusiveScan<int>(fN ? 1 : 0, lds, &t[2]);
+        const int ex3 = blockExclusiveScan<int>(fN ? na : 0, lds, &t[3]);
+        if (fP || fN)
+        {
+            const int64_t srcOff = m.offset[e];
+            int64_t dstM, dstA;
+            double shift;
+            if (fP)
+            {
+                dstM = totM + off[0 * tiles + blockIdx.x] + run[0] + ex0;
+                dstA = totA + off[1 * tiles + blockIdx.x] + run[1] + ex1;
+                shift = -L;
+            }
+            else
+            {
+                dstM = totM + nPM + off[2 * tiles + blockIdx.x] + run[2] + ex2;
+                dstA = totA + nPA + off[3 * tiles + blockIdx.x] + run[3] + ex3;
+                shift = +L;
+            }
+            // a molecule can satisfy both only if the box is thinner than two ghost layers;
+            // the reference then copies it into both blocks
+            for (int pass = 0; pass < (fP && fN ? 2 : 1); ++pass)
+            {
+                if (pass == 1)
+                {
+                    dstM = totM + nPM + off[2 * tiles + blockIdx.x] + run[2] + ex2;
+                    dstA = totA + nPA + off[3 * tiles + blockIdx.x] + run[3] + ex3;
+                    shift = +L;
+                }
+                copyMol(m, dstM, e, axis, shift);
+                m.offset[dstM] = dstA;
+                m.numAtoms[dstM] = na;
+                for (int k = 0; k < na; ++k)
+                {
+                    copyAtom(a, dstA + k, srcOff + k, axis, shift);
+                    const int32_t c = corr[srcOff + k];
+                    corr[dstA + k] = c == -1 ? int32_t(srcOff + k) : c;
+                }
+            }
+        }
+        for (int k = 0; k < 4; ++k) run[k] += t[k];
+    }
+}
+}  // namespace
+
+// =========================================================================================
+namespace
+{
+void resetCorr(mrmd_ghost_layer* gl, int64_t nlocal)
+{
+    gl->corr.reserve(gl->ctx, growCap(nlocal), 0);
+    if (nlocal > 0)
+        hipLaunchKernelGGL(k_fillI32, dim3(gridFor(nlocal)), dim3(BLOCK), 0, gl->ctx->stream, gl->corr.p, nlocal,
+                           -1);
+    gl->corrSize = nlocal;
+    checkLaunch();
+}
+
+void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int axis)
+{
+    auto* ctx = gl->ctx;
+    const SubD s = makeSub(box);
+    const int64_t total = a->nlocal + a->nghost;
+    if (total >= (int64_t(1) << 31) - 1) fail(MRMD_ERR_UNSUP, "more than 2^31 records");
+    const int64_t tiles = std::max<int64_t>(1, (total + TILE - 1) / TILE);
+    gl->tileCnt.reserve(ctx, 2 * tiles);
+    gl->tileOff.reserve(ctx, 2 * tiles);
+    int32_t* tot = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
+    hipLaunchKernelGGL(k_ghostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->pos.p, total, axis,
+                       s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, gl->tileCnt.p + tiles);
+    checkLaunch();
+    scanExclusiveI32(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, tot);
+    scanExclusiveI32(ctx, gl->tileCnt.p + tiles, gl->tileOff.p + tiles, tiles, tot + 1);
+    int32_t h[2];
+    HIPCHECK(hipMemcpyAsync(h, tot, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    const int64_t nP = h[0], nN = h[1];
+    const int64_t newTotal = total + nP + nN;
+    // atoms.resize(...) keeps existing records (GhostExchange.cpp:111-112)
+    a->ensureCap(newTotal);
+    a->size = newTotal;
+    if (gl->corr.n < newTotal) gl->corr.reserve(ctx, growCap(newTotal), gl->corrSize);
+    if (nP + nN > 0)
+    {
+        hipLaunchKernelGGL(k_ghostScatter, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->view(), gl->corr.p,
+                           total, axis, s.minInner[axis], s.maxInner[axis], s.diam[axis], gl->tileOff.p,
+                           gl->tileOff.p + tiles, tot);
+        checkLaunch();
+    }
+    gl->corrSize = newTotal;
+    a->nghost += nP + nN;
+}
+
+void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box, int axis)
+{
+    auto* ctx = gl->ctx;
+    const SubD s = makeSub(box);
+    const int64_t totM = m->nlocal + m->nghost;
+    const int64_t totA = a->nlocal + a->nghost;
+    const int64_t tiles = std::max<int64_t>(1, (totM + TILE - 1) / TILE);
+    gl->tileCnt.reserve(ctx, 4 * tiles);
+    gl->tileOff.reserve(ctx, 4 * tiles);
+    int32_t* tot = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
+    hipLaunchKernelGGL(k_molGhostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(), totM, axis,
+                       s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, tiles);
+    checkLaunch();
+    for (int k = 0; k < 4; ++k) scanExclusiveI32(ctx, gl->tileCnt.p + k * tiles, gl->tileOff.p + k * tiles, tiles, tot + k);
+    int32_t h[4];
+    HIPCHECK(hipMemcpyAsync(h, tot, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    const int64_t newM = totM + h[0] + h[2], newA = totA + h[1] + h[3];
+    m->ensureCap(newM);
+    m->size = newM;
+    a->ensureCap(newA);
+    a->size = newA;
+    if (gl->corr.n < newA) gl->corr.reserve(ctx, growCap(newA), gl->corrSize);
+    if (h[0] + h[2] > 0)
+    {
+        hipLaunchKernelGGL(k_molGhostScatter, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(), a->view(),
+                           gl->corr.p, totM, totA, axis, s.minInner[axis], s.maxInner[axis], s.diam[axis],
+                           gl->tileOff.p, tiles, tot);
+        checkLaunch();
+    }
+    gl->corrSize = newA;
+    m->nghost += h[0] + h[2];
+    a->nghost += h[1] + h[3];
+}
+
+void checkBox(const mrmd_box* box)
+{
+    REQUIRE(box, "null box");
+    for (int d = 0; d < 3; ++d)
+    {
+        REQUIRE(box->max[d] > box->min[d], "box max must exceed min");
+        REQUIRE(box->ghost[d] >= 0.0, "negative ghost layer");
+        REQUIRE(box->max[d] - box->min[d] > box->ghost[d], "ghost layer larger than subdomain");
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int mrmd_ghost_layer_create(mrmd_ctx* c, mrmd_ghost_layer** out)
+{
+    MRMD_API_BEGIN(c)
+    REQUIRE(out, "null output");
+    auto* g = new mrmd_ghost_layer();
+    g->ctx = c;
+    *out = g;
+    MRMD_API_END
+}
+
+int mrmd_ghost_layer_destroy(mrmd_ghost_layer* g)
+{
+    if (!g) return MRMD_OK;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    delete g;
+    return MRMD_OK;
+}
+
+int mrmd_halo_wrap(mrmd_atoms* a, const mrmd_box* box)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a, "null atoms");
+    checkBox(box);
+    if (a->nlocal == 0) return MRMD_OK;
+    KTimer t(a->ctx, "halo_wrap");
+    hipLaunchKernelGGL(k_wrap, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, a->ctx->stream, a->pos.p, a->nlocal,
+                       makeSub(box));
+    checkLaunch();
+    MRMD_API_END
+}
+
+int mrmd_halo_create(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int64_t* nghost)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && a && a->ctx == gl->ctx, "invalid arguments");
+    checkBox(box);
+    KTimer t(gl->ctx, "halo_create");
+    resetCorr(gl, a->nlocal);
+    a->nghost = 0;
+    for (int d = 0; d < 3; ++d) createAxis(gl, a, box, d);
+    if (nghost) *nghost = a->nghost;
+    MRMD_API_END
+}
+
+int mrmd_halo_create_axis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int axis, int reset)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && a && axis >= 0 && axis <= 2, "invalid arguments");
+    checkBox(box);
+    if (reset) resetCorr(gl, a->nlocal);
+    REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "ghost layer not initialised for these atoms");
+    createAxis(gl, a, box, axis);
+    MRMD_API_END
+}
+
+int mrmd_halo_update(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && a, "invalid arguments");
+    checkBox(box);
+    REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "correspondingRealAtom shorter than the atoms");
+    if (a->nghost == 0) return MRMD_OK;
+    KTimer t(gl->ctx, "halo_update");
+    hipLaunchKernelGGL(k_ghostUpdate, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
+                       gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box));
+    checkLaunch();
+    MRMD_API_END
+}
+
+int mrmd_halo_fold_forces(mrmd_ghost_layer* gl, mrmd_atoms* a)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && a, "invalid arguments");
+    REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "correspondingRealAtom shorter than the atoms");
+    if (a->nghost == 0) return MRMD_OK;
+    KTimer t(gl->ctx, "halo_fold");
+    hipLaunchKernelGGL(k_ghostFold, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->force.p, a->cap,
+                       gl->corr.p, a->nlocal, a->nlocal + a->nghost);
+    checkLaunch();
+    MRMD_API_END
+}
+
+int mrmd_halo_get_corresponding(const mrmd_ghost_layer* gl, int64_t* host, int64_t n)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && host && n >= 0 && n <= gl->corrSize, "invalid arguments (n=%lld, size=%lld)", (long long)n,
+            gl ? (long long)gl->corrSize : -1LL);
+    std::vector<int32_t> tmp(static_cast<size_t>(n));
+    if (n)
+    {
+        HIPCHECK(hipMemcpyAsync(tmp.data(), gl->corr.p, size_t(n) * 4, hipMemcpyDeviceToHost, gl->ctx->stream));
+        sync(gl->ctx);
+    }
+    for (int64_t i = 0; i < n; ++i) host[i] = tmp[size_t(i)];
+    MRMD_API_END
+}
+
+int mrmd_halo_set_corresponding(mrmd_ghost_layer* gl, const int64_t* host, int64_t n)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && host && n >= 0, "invalid arguments");
+    std::vector<int32_t> tmp(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) tmp[size_t(i)] = int32_t(host[i]);
+    gl->corr.reserve(gl->ctx, std::max<int64_t>(n, 1), 0);
+    if (n)
+    {
+        HIPCHECK(hipMemcpyAsync(gl->corr.p, tmp.data(), size_t(n) * 4, hipMemcpyHostToDevice, gl->ctx->stream));
+        sync(gl->ctx);
+    }
+    gl->corrSize = n;
+    MRMD_API_END
+}
+
+int mrmd_multires_wrap(mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box)
+{
+    MRMD_API_BEGIN(m ? m->ctx : nullptr)
+    REQUIRE(m && a, "invalid arguments");
+    checkBox(box);
+    if (m->nlocal == 0) return MRMD_OK;
+    hipLaunchKernelGGL(k_molWrap, dim3(gridFor(m->nlocal)), dim3(BLOCK), 0, m->ctx->stream, m->view(), a->view(),
+                       m->nlocal, makeSub(box));
+    checkLaunch();
+    MRMD_API_END
+}
+
+int mrmd_multires_halo_create(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && m && a, "invalid arguments");
+    checkBox(box);
+    m->nghost = 0;
+    resetCorr(gl, a->nlocal);
+    a->nghost = 0;
+    for (int d = 0; d < 3; ++d) molCreateAxis(gl, m, a, box, d);
+    MRMD_API_END
+}
+
+int mrmd_multires_halo_create_axis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box,
+                                   int axis, int reset)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && m && a && axis >= 0 && axis <= 2, "invalid arguments");
+    checkBox(box);
+    if (reset) resetCorr(gl, a->nlocal);
+    molCreateAxis(gl, m, a, box, axis);
+    MRMD_API_END
+}
+
+}  // extern "C"

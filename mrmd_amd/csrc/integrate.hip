@@ -1,0 +1,223 @@
+// Velocity-Verlet and Langevin (BAOAB-style) integration and the kinetic energy.
+// Streaming kernels: one lane per local atom, coalesced SoA reads/writes, the maximum
+// displacement reduced with wave shuffles and one 64-bit atomic max per workgroup.
+#include "containers.hpp"
+#include "detmath.hpp"
+
+using namespace mrmd_hip;
+
+namespace
+{
+// device scalars of an atoms container: [0] max |dx|^2 of the last pre-step
+struct DispScalars
+{
+    double maxSq;
+};
+
+__global__ void k_zero1(double* p) { *p = 0.0; }
+
+// VelocityVerlet::preForceIntegrate (VelocityVerlet.cpp:26-67; UpdateSteps.hpp:25-54)
+__global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double dt, double* maxSq)
+{
+    __shared__ double red[BLOCK / WAVE];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < n)
+    {
+        const double dtHalf = 0.5 * dt;
+        double4 p = a.pos[i];
+        const double m = a.mass[i];
+        double vx = a.vel[i], vy = a.vel[a.cap + i], vz = a.vel[2 * a.cap + i];
+        const double fx = a.force[i], fy = a.force[a.cap + i], fz = a.force[2 * a.cap + i];
+        const double dtfm = dtHalf / m;
+        vx += dtfm * fx;
+        vy += dtfm * fy;
+        vz += dtfm * fz;
+        const double ox = p.x, oy = p.y, oz = p.z;
+        p.x += dt * vx;
+        p.y += dt * vy;
+        p.z += dt * vz;
+        a.pos[i] = p;
+        a.vel[i] = vx;
+        a.vel[a.cap + i] = vy;
+        a.vel[2 * a.cap + i] = vz;
+        const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    const double m = blockMax(d2, red);
+    if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, m);
+}
+
+// VelocityVerlet::postForceIntegrate (VelocityVerlet.cpp:69-90)
+__global__ __launch_bounds__(BLOCK) void k_vvPost(AtomsView a, int64_t n, double dt)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dtfm = (0.5 * dt) / a.mass[i];
+    a.vel[i] += dtfm * a.force[i];
+    a.vel[a.cap + i] += dtfm * a.force[a.cap + i];
+    a.vel[2 * a.cap + i] += dtfm * a.force[2 * a.cap + i];
+}
+
+// VelocityVerletLangevinThermostat::preForceIntegrate_apply_if
+// (VelocityVerletLangevinThermostat.hpp:63-136; UpdateSteps.hpp:56-79)
+template <bool PRED>
+__global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
+                                                       uint64_t seed, uint64_t call, mrmd_pred1 pred, double* maxSq)
+{
+    __shared__ double red[BLOCK / WAVE];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < n)
+    {
+        const double dtHalf = 0.5 * dt;
+        double4 p = a.pos[i];
+        const double m = a.mass[i];
+        double vx = a.vel[i], vy = a.vel[a.cap + i], vz = a.vel[2 * a.cap + i];
+        const double dtfm = dtHalf / m;
+        vx += dtfm * a.force[i];
+        vy += dtfm * a.force[a.cap + i];
+        vz += dtfm * a.force[2 * a.cap + i];
+        const double ox = p.x, oy = p.y, oz = p.z;
+        p.x += dtHalf * vx;
+        p.y += dtHalf * vy;
+        p.z += dtHalf * vz;
+        if (!PRED || evalPred1(pred, p.x, p.y, p.z))
+        {
+            double xi[3];
+            normals3(seed, uint64_t(i), call, xi);
+            const double dtm = dt / m;
+            const double damping = detExp(-zeta * dtm);
+            const double sigma = sqrt(temperature / m * (1.0 - detExp(-2.0 * zeta * dtm)));
+            vx *= damping;
+            vy *= damping;
+            vz *= damping;
+            vx += sigma * xi[0];
+            vy += sigma * xi[1];
+            vz += sigma * xi[2];
+        }
+        p.x += dtHalf * vx;
+        p.y += dtHalf * vy;
+        p.z += dtHalf * vz;
+        a.pos[i] = p;
+        a.vel[i] = vx;
+        a.vel[a.cap + i] = vy;
+        a.vel[2 * a.cap + i] = vz;
+        const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    const double mx = blockMax(d2, red);
+    if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, mx);
+}
+
+// analysis::getKineticEnergy (KineticEnergy.hpp:26-39): per-workgroup partials
+__global__ __launch_bounds__(BLOCK) void k_kinetic(AtomsView a, int64_t n, double* partials)
+{
+    __shared__ double red[BLOCK / WAVE];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n)
+    {
+        const double vx = a.vel[i], vy = a.vel[a.cap + i], vz = a.vel[2 * a.cap + i];
+        s = a.mass[i] * (vx * vx + vy * vy + vz * vz);
+    }
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
+{
+    __shared__ double red[BLOCK / WAVE];
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k];
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+double* dispScalar(mrmd_atoms* a) { return static_cast<double*>(ctxScratch(a->ctx, 3, 64)); }
+
+void finishDisp(mrmd_atoms* a, double* maxDisp)
+{
+    if (!maxDisp) return;
+    HIPCHECK(hipMemcpyAsync(a->ctx->pinned, dispScalar(a), 8, hipMemcpyDeviceToHost, a->ctx->stream));
+    sync(a->ctx);
+    *maxDisp = sqrt(a->ctx->pinned[0]);
+}
+}  // namespace
+
+extern "C" {
+
+int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a, "null atoms");
+    auto* ctx = a->ctx;
+    double* ms = dispScalar(a);
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
+    if (a->nlocal > 0)
+    {
+        KTimer t(ctx, "vv_pre");
+        hipLaunchKernelGGL(k_vvPre, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, ms);
+    }
+    checkLaunch();
+    finishDisp(a, maxDisp);
+    MRMD_API_END
+}
+
+int mrmd_vv_post(mrmd_atoms* a, double dt)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a, "null atoms");
+    if (a->nlocal == 0) return MRMD_OK;
+    KTimer t(a->ctx, "vv_post");
+    hipLaunchKernelGGL(k_vvPost, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, a->ctx->stream, a->view(), a->nlocal, dt);
+    checkLaunch();
+    MRMD_API_END
+}
+
+int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature, uint64_t seed, uint64_t call,
+                      const mrmd_pred1* pred, double* maxDisp)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a, "null atoms");
+    validatePred1(pred);
+    auto* ctx = a->ctx;
+    double* ms = dispScalar(a);
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
+    if (a->nlocal > 0)
+    {
+        KTimer t(ctx, "langevin_pre");
+        const bool usePred = pred && pred->kind != MRMD_PRED_ALWAYS;
+        const mrmd_pred1 p = pred ? *pred : mrmd_pred1{};
+        if (usePred)
+            hipLaunchKernelGGL(k_langevinPre<true>, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                               a->nlocal, dt, zeta, temperature, seed, call, p, ms);
+        else
+            hipLaunchKernelGGL(k_langevinPre<false>, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                               a->nlocal, dt, zeta, temperature, seed, call, p, ms);
+    }
+    checkLaunch();
+    finishDisp(a, maxDisp);
+    MRMD_API_END
+}
+
+int mrmd_kinetic_energy(const mrmd_atoms* ac, double* energy)
+{
+    auto* a = const_cast<mrmd_atoms*>(ac);
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && energy, "invalid arguments");
+    auto* ctx = a->ctx;
+    const unsigned g = gridFor(a->nlocal);
+    double* part = static_cast<double*>(ctxScratch(ctx, 1, size_t(g + 1) * sizeof(double)));
+    if (a->nlocal > 0)
+        hipLaunchKernelGGL(k_kinetic, dim3(g), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, part);
+    hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, a->nlocal > 0 ? int64_t(g) : 0,
+                       part + g);
+    checkLaunch();
+    HIPCHECK(hipMemcpyAsync(ctx->pinned, part + g, 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    *energy = 0.5 * ctx->pinned[0];
+    MRMD_API_END
+}
+
+}  // extern "C"

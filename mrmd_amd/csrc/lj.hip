@@ -1,0 +1,667 @@
+// Lennard-Jones pair force (action::LennardJones), its precomputed table, and the adjoint.
+//
+// Force kernel design (gfx950, FP64): one lane per local atom over its full neighbour row
+// (both directions of every pair), so F_i is owned by one lane and no atomics are needed;
+// the pair work is done twice, which on a chip with 78.6 TFLOP/s FP64 costs less than the
+// L2 atomics of a half list.  Positions are gathered as one 32-byte {x,y,z,type} record;
+// rows are read column-major (coalesced across the wave).  Energy and virial are reduced
+// with wave shuffles, per-workgroup partials and one ordered final pass (deterministic).
+// The reference's half-list algorithm (atomic F_j, LennardJones.hpp:158-205) is kept for
+// asymmetric pair predicates and asymmetric parameter tables.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "nlist.hpp"
+
+using namespace mrmd_hip;
+
+namespace
+{
+// PrecomputedValues (LennardJones.hpp:27-39)
+struct Pre
+{
+    double ff1, ff2, ef1, ef2, rcSqr, capDist, capDistSqr, capCoeff, shift, eCap;
+};
+static_assert(sizeof(Pre) == 80, "table record");
+
+constexpr int MAX_TYPES2 = 64;
+
+// computeForceAndEnergy (LennardJones.hpp:52-78)
+__host__ __device__ inline void forceEnergy(const Pre& p, double distSqr, double& ff, double& e)
+{
+    if (distSqr >= p.capDistSqr)
+    {
+        const double frac2 = 1.0 / distSqr;
+        const double frac6 = frac2 * frac2 * frac2;
+        ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2;
+        e = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+        return;
+    }
+    const double dist = sqrt(distSqr);
+    ff = p.capCoeff / dist;
+    e = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
+}
+
+// CappedLennardJonesPotential ctor + init functor (LennardJones.cpp:81-116, 61-79)
+void buildTable(std::vector<Pre>& t, const double* cap, const double* rc, const double* sigma, const double* eps,
+                int64_t nt2, bool shifted)
+{
+    t.assign(size_t(nt2), Pre{});
+    for (int64_t k = 0; k < nt2; ++k)
+    {
+        auto& p = t[size_t(k)];
+        const double sig2 = sigma[k] * sigma[k];
+        const double sig6 = sig2 * sig2 * sig2;
+        p.ff1 = 48.0 * eps[k] * sig6 * sig6;
+        p.ff2 = 24.0 * eps[k] * sig6;
+        p.ef1 = 4.0 * eps[k] * sig6 * sig6;
+        p.ef2 = 4.0 * eps[k] * sig6;
+        p.rcSqr = rc[k] * rc[k];
+        p.capDist = cap[k];
+    }
+    for (int64_t k = 0; k < nt2; ++k)
+    {
+        auto& p = t[size_t(k)];
+        const double capDist = p.capDist;
+        p.capDist = 0.0;
+        p.capDistSqr = 0.0;
+        double ff, e;
+        forceEnergy(p, capDist * capDist, ff, e);
+        p.capCoeff = ff * capDist;
+        p.eCap = e;
+        p.capDist = capDist;
+        p.capDistSqr = capDist * capDist;
+        if (shifted)
+        {
+            forceEnergy(p, p.rcSqr, ff, e);
+            p.shift = e;
+        }
+    }
+}
+}  // namespace
+
+struct mrmd_lj
+{
+    mrmd_ctx* ctx = nullptr;
+    int64_t numTypes = 1, typeStride = 1;
+    bool shifted = false;
+    bool symmetric = true;  ///< table(ti*stride+tj) == table(tj*stride+ti) for all types
+    std::vector<double> cap, rc;
+    std::vector<Pre> table;
+    double rcSqr = 0.0;
+    DBuf<Pre> dtable;
+    DBuf<double> ev;  ///< energy, virial on the device
+    DBuf<double> partials;
+    bool evValid = true;
+    double hE = 0.0, hW = 0.0;
+
+    void upload();
+};
+
+void mrmd_lj::upload()
+{
+    dtable.reserve(ctx, int64_t(table.size()));
+    HIPCHECK(hipMemcpyAsync(dtable.p, table.data(), table.size() * sizeof(Pre), hipMemcpyHostToDevice, ctx->stream));
+    sync(ctx);
+    symmetric = true;
+    for (int64_t a = 0; a < numTypes; ++a)
+        for (int64_t b = 0; b < numTypes; ++b)
+        {
+            const int64_t i1 = a * typeStride + b, i2 = b * typeStride + a;
+            if (i1 >= int64_t(table.size()) || i2 >= int64_t(table.size())) continue;
+            if (std::memcmp(&table[size_t(i1)], &table[size_t(i2)], sizeof(Pre)) != 0) symmetric = false;
+        }
+}
+
+namespace
+{
+struct LJArgs
+{
+    const double4* pos;
+    double* force;
+    int64_t cap;
+    int64_t begin;
+    int64_t rows;
+    int64_t stride;
+    const Pre* table;
+    int nt2;
+    int typeStride;
+    double rcSqr;
+    mrmd_pred2 pred;
+    int usePred;
+};
+
+template <bool PRED>
+__global__ __launch_bounds__(BLOCK) void k_ljFull(LJArgs a, const uint32_t* __restrict__ nbr,
+                                                  const int32_t* __restrict__ cnt, double* partials)
+{
+    __shared__ Pre tab[MAX_TYPES2];
+    __shared__ double red[BLOCK / WAVE];
+    for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
+    __syncthreads();
+    const unsigned blk = xcdBlock();
+    const int64_t r = int64_t(blk) * blockDim.x + threadIdx.x;
+    double e = 0.0, w = 0.0;
+    if (r < a.rows)
+    {
+        const int64_t i = a.begin + r;
+        const double4 pi = a.pos[i];
+        const int ti = int(pi.w) * a.typeStride;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const int c = cnt[r];
+        const uint32_t* col = nbr + r;
+        for (int k = 0; k < c; ++k)
+        {
+            const uint32_t j = col[int64_t(k) * a.stride] & 0x7fffffffu;
+            const double4 pj = a.pos[j];
+            if (PRED && !evalPred2(a.pred, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z)) continue;
+            const double dx = pi.x - pj.x;
+            const double dy = pi.y - pj.y;
+            const double dz = pi.z - pj.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > a.rcSqr) continue;
+            const Pre& p = tab[ti + int(pj.w)];
+            double ff, en;
+            forceEnergy(p, d2, ff, en);
+            fx += dx * ff;
+            fy += dy * ff;
+            fz += dz * ff;
+            e += en;
+            w += ff * d2;
+        }
+        a.force[i] += fx;
+        a.force[a.cap + i] += fy;
+        a.force[2 * a.cap + i] += fz;
+    }
+    // every pair is visited from both ends: E = 1/2 sum, W = -1/2 sum(ff d2) / 2
+    e = blockSum(0.5 * e, red);
+    w = blockSum(-0.25 * w, red);
+    if (threadIdx.x == 0)
+    {
+        partials[2 * blk] = e;
+        partials[2 * blk + 1] = w;
+    }
+}
+
+// the reference algorithm: half rows, F_j by atomics (LennardJones.hpp:158-205)
+template <bool PRED>
+__global__ __launch_bounds__(BLOCK) void k_ljHalf(LJArgs a, const int32_t* __restrict__ nbr,
+                                                  const int32_t* __restrict__ cnt, double* partials)
+{
+    __shared__ Pre tab[MAX_TYPES2];
+    __shared__ double red[BLOCK / WAVE];
+    for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
+    __syncthreads();
+    const unsigned blk = xcdBlock();
+    const int64_t r = int64_t(blk) * blockDim.x + threadIdx.x;
+    double e = 0.0, w = 0.0;
+    if (r < a.rows)
+    {
+        const int64_t i = a.begin + r;
+        const double4 pi = a.pos[i];
+        const int ti = int(pi.w) * a.typeStride;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const int c = cnt[r];
+        for (int k = 0; k < c; ++k)
+        {
+            const int32_t j = nbr[int64_t(k) * a.stride + r];
+            const double4 pj = a.pos[j];
+            if (PRED && !evalPred2(a.pred, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z)) continue;
+            const double dx = pi.x - pj.x;
+            const double dy = pi.y - pj.y;
+            const double dz = pi.z - pj.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > a.rcSqr) continue;
+            const Pre& p = tab[ti + int(pj.w)];
+            double ff, en;
+            forceEnergy(p, d2, ff, en);
+            e += en;
+            w -= 0.5 * ff * d2;
+            fx += dx * ff;
+            fy += dy * ff;
+            fz += dz * ff;
+            atomicAddD(&a.force[j], -(dx * ff));
+            atomicAddD(&a.force[a.cap + j], -(dy * ff));
+            atomicAddD(&a.force[2 * a.cap + j], -(dz * ff));
+        }
+        atomicAddD(&a.force[i], fx);
+        atomicAddD(&a.force[a.cap + i], fy);
+        atomicAddD(&a.force[2 * a.cap + i], fz);
+    }
+    e = blockSum(e, red);
+    w = blockSum(w, red);
+    if (threadIdx.x == 0)
+    {
+        partials[2 * blk] = e;
+        partials[2 * blk + 1] = w;
+    }
+}
+
+// ordered sum of per-workgroup partials (one workgroup)
+__global__ void k_sumPartials(const double* partials, int64_t n, int width, double* out)
+{
+    __shared__ double red[BLOCK / WAVE];
+    for (int c = 0; c < width; ++c)
+    {
+        double s = 0.0;
+        for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k * width + c];
+        s = blockSum(s, red);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+}
+
+__global__ void k_ljEval(const Pre* table, const double* d2, const int64_t* idx, int64_t n, double* ff, double* e)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    forceEnergy(table[idx[k]], d2[k], ff[k], e[k]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Adjoint of F_i = sum_j s(r_ij^2) d_ij, d = x_i - x_j, s = r^-8 (ff1 r^-6 - ff2):
+//   dL/dx_i = sum_j J_ij (g_i - g_j),  J = s I + 2 s' d d^T,  s' = ds/d(r^2)
+//   dL/deps_ab   = sum_{pairs ab} (g_i - g_j) . d s/eps
+//   dL/dsigma_ab = sum_{pairs ab} (g_i - g_j) . d ds/dsigma
+// With full rows every pair is seen from both ends: dL/dx_i is complete per row, and the
+// parameter sums take half of each row's term.  Capped pairs use the capped expression.
+template <bool FEW>
+__global__ __launch_bounds__(BLOCK) void k_ljAdjoint(LJArgs a, const uint32_t* __restrict__ nbr,
+                                                     const int32_t* __restrict__ cnt, const double* __restrict__ g,
+                                                     int64_t gstride, const double* __restrict__ sigma,
+                                                     const double* __restrict__ eps, double* dLdx, int64_t xstride,
+                                                     double* pgrad /*[2*nt2] per block region*/)
+{
+    __shared__ Pre tab[MAX_TYPES2];
+    __shared__ double sg[MAX_TYPES2], se[MAX_TYPES2];
+    __shared__ double acc[2 * MAX_TYPES2];
+    for (int t = threadIdx.x; t < a.nt2; t += blockDim.x)
+    {
+        tab[t] = a.table[t];
+        sg[t] = sigma[t];
+        se[t] = eps[t];
+        acc[t] = 0.0;
+        acc[MAX_TYPES2 + t] = 0.0;
+    }
+    __syncthreads();
+    // FEW: up to 4 type pairs accumulate in registers, otherwise in LDS with atomics
+    double re[4] = {0.0, 0.0, 0.0, 0.0}, rs[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t r = int64_t(xcdBlock()) * blockDim.x + threadIdx.x;
+    if (r < a.rows)
+    {
+        const int64_t i = a.begin + r;
+        const double4 pi = a.pos[i];
+        const int ti = int(pi.w) * a.typeStride;
+        const double gix = g[r], giy = g[gstride + r], giz = g[2 * gstride + r];
+        double ox = 0.0, oy = 0.0, oz = 0.0;
+        const int c = cnt[r];
+        for (int k = 0; k < c; ++k)
+        {
+            const uint32_t j = nbr[int64_t(k) * a.stride + r] & 0x7fffffffu;
+            const double4 pj = a.pos[j];
+            const double dx = pi.x - pj.x;
+            const double dy = pi.y - pj.y;
+            const double dz = pi.z - pj.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > a.rcSqr) continue;
+            const int tIdx = ti + int(pj.w);
+            const Pre& p = tab[tIdx];
+            // upstream of the partner: local atoms carry their own row index, ghosts map to
+            // the real atom through the sign-free index in pj (handled by the caller's g)
+            const int64_t jr = int64_t(j) - a.begin;
+            const double gjx = g[jr], gjy = g[gstride + jr], gjz = g[2 * gstride + jr];
+            const double ux = gix - gjx, uy = giy - gjy, uz = giz - gjz;
+            const double du = dx * ux + dy * uy + dz * uz;
+            double s, sp, dsde, dsds;
+            if (d2 >= p.capDistSqr)
+            {
+                const double f2 = 1.0 / d2;
+                const double f6 = f2 * f2 * f2;
+                s = f6 * (p.ff1 * f6 - p.ff2) * f2;
+                // ds/d(r^2) = -(7 ff1 r^-16 - 4 ff2 r^-10)
+                sp = -(7.0 * p.ff1 * f6 * f6 * f2 * f2 - 4.0 * p.ff2 * f6 * f2 * f2);
+                dsde = s / se[tIdx];
+                dsds = f6 * (12.0 * p.ff1 * f6 - 6.0 * p.ff2) * f2 / sg[tIdx];
+            }
+            else
+            {
+                const double dist = sqrt(d2);
+                s = p.capCoeff / dist;
+                sp = -0.5 * p.capCoeff / (dist * d2);
+                // capCoeff = s(rcap) rcap is linear in eps; its sigma-derivative follows s
+                const double rc2 = p.capDistSqr;
+                const double f2 = 1.0 / rc2;
+                const double f6 = f2 * f2 * f2;
+                dsde = s / se[tIdx];
+                dsds = (f6 * (12.0 * p.ff1 * f6 - 6.0 * p.ff2) * f2 / sg[tIdx]) * p.capDist / dist;
+            }
+            const double two_sp_du = 2.0 * sp * du;
+            ox += s * ux + two_sp_du * dx;
+            oy += s * uy + two_sp_du * dy;
+            oz += s * uz + two_sp_du * dz;
+            const double h = 0.5 * du;
+            if (FEW)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == tIdx)
+                    {
+                        re[q] += h * dsde;
+                        rs[q] += h * dsds;
+                    }
+            }
+            else
+            {
+                atomicAdd(&acc[tIdx], h * dsde);
+                atomicAdd(&acc[MAX_TYPES2 + tIdx], h * dsds);
+            }
+        }
+        dLdx[r] = ox;
+        dLdx[xstride + r] = oy;
+        dLdx[2 * xstride + r] = oz;
+    }
+    if (FEW)
+    {
+        __shared__ double red[BLOCK / WAVE];
+        for (int q = 0; q < a.nt2; ++q)
+        {
+            const double se_ = blockSum(re[q], red);
+            const double ss_ = blockSum(rs[q], red);
+            if (threadIdx.x == 0)
+            {
+                atomicAddD(&pgrad[q], se_);
+                atomicAddD(&pgrad[a.nt2 + q], ss_);
+            }
+        }
+        return;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.nt2; t += blockDim.x)
+    {
+        atomicAddD(&pgrad[t], acc[t]);
+        atomicAddD(&pgrad[a.nt2 + t], acc[MAX_TYPES2 + t]);
+    }
+}
+
+// upstream gradient of ghosts = upstream of their real atom: expand g (local rows) to all
+// records [begin, nall) so the adjoint can read g_j for ghost partners
+__global__ void k_expandGhostUpstream(double* g, int64_t stride, const int32_t* corr, int64_t nlocal, int64_t nall)
+{
+    const int64_t i = nlocal + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nall) return;
+    const int32_t rr = corr[i];
+    for (int d = 0; d < 3; ++d) g[d * stride + i] = g[d * stride + rr];
+}
+
+__global__ void k_zeroD(double* p, int64_t n)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
+
+__global__ void k_rowMajorToSoA(const double* in, double* out, int64_t n, int64_t stride)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int d = 0; d < 3; ++d) out[d * stride + k] = in[3 * k + d];
+}
+__global__ void k_soaToRowMajor(const double* in, double* out, int64_t n, int64_t stride)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int d = 0; d < 3; ++d) out[3 * k + d] = in[d * stride + k];
+}
+}  // namespace
+
+
+extern "C" {
+
+int mrmd_lj_create(mrmd_ctx* c, const double* cap, const double* rc, const double* sigma, const double* eps,
+                   int64_t numTypes, int shifted, int64_t typeStride, mrmd_lj** out)
+{
+    MRMD_API_BEGIN(c)
+    REQUIRE(out && cap && rc && sigma && eps && numTypes >= 1, "invalid arguments");
+    REQUIRE(numTypes * numTypes <= MAX_TYPES2, "at most %d type pairs", MAX_TYPES2);
+    REQUIRE(typeStride >= 1, "type stride must be positive");
+    auto* lj = new mrmd_lj();
+    lj->ctx = c;
+    lj->numTypes = numTypes;
+    lj->typeStride = typeStride;
+    lj->shifted = shifted != 0;
+    const int64_t nt2 = numTypes * numTypes;
+    lj->cap.assign(cap, cap + nt2);
+    lj->rc.assign(rc, rc + nt2);
+    buildTable(lj->table, cap, rc, sigma, eps, nt2, shifted != 0);
+    double rcMax = rc[0];
+    for (int64_t k = 0; k < nt2; ++k) rcMax = std::max(rcMax, rc[k]);
+    lj->rcSqr = rcMax * rcMax;
+    try
+    {
+        lj->upload();
+        lj->ev.reserve(c, 2);
+        hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, c->stream, lj->ev.p, int64_t(2));
+        checkLaunch();
+    }
+    catch (...)
+    {
+        delete lj;
+        throw;
+    }
+    *out = lj;
+    MRMD_API_END
+}
+
+int mrmd_lj_destroy(mrmd_lj* lj)
+{
+    if (!lj) return MRMD_OK;
+    (void)hipStreamSynchronize(lj->ctx->stream);
+    delete lj;
+    return MRMD_OK;
+}
+
+int mrmd_lj_set_params(mrmd_lj* lj, const double* sigma, const double* eps)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj && sigma && eps, "invalid arguments");
+    buildTable(lj->table, lj->cap.data(), lj->rc.data(), sigma, eps, int64_t(lj->table.size()), lj->shifted);
+    lj->upload();
+    MRMD_API_END
+}
+
+int mrmd_lj_table(const mrmd_lj* lj, double* host)
+{
+    if (!lj || !host) return MRMD_ERR_ARG;
+    std::memcpy(host, lj->table.data(), lj->table.size() * sizeof(Pre));
+    return MRMD_OK;
+}
+
+int mrmd_lj_eval(mrmd_lj* lj, const double* d2, const int64_t* idx, int64_t n, double* ff, double* e)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj && d2 && idx && ff && e && n >= 0, "invalid arguments");
+    for (int64_t k = 0; k < n; ++k)
+        REQUIRE(idx[k] >= 0 && idx[k] < int64_t(lj->table.size()), "type index %lld out of range", (long long)idx[k]);
+    if (n == 0) return MRMD_OK;
+    auto* ctx = lj->ctx;
+    double *dd, *dff, *de;
+    int64_t* di;
+    HIPCHECK(hipMalloc(&dd, size_t(n) * 8));
+    HIPCHECK(hipMalloc(&dff, size_t(n) * 8));
+    HIPCHECK(hipMalloc(&de, size_t(n) * 8));
+    HIPCHECK(hipMalloc(&di, size_t(n) * 8));
+    HIPCHECK(hipMemcpyAsync(dd, d2, size_t(n) * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(di, idx, size_t(n) * 8, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_ljEval, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, lj->dtable.p, dd, di, n, dff, de);
+    checkLaunch();
+    HIPCHECK(hipMemcpyAsync(ff, dff, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(e, de, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    (void)hipFree(dd);
+    (void)hipFree(dff);
+    (void)hipFree(de);
+    (void)hipFree(di);
+    MRMD_API_END
+}
+
+int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_pred2* pred)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj && at && vl && at->ctx == lj->ctx && vl->ctx == lj->ctx, "invalid arguments");
+    validatePred2(pred);
+    REQUIRE(vl->begin == 0 && vl->end <= at->nlocal + at->nghost, "the list rows must start at record 0");
+    REQUIRE(vl->nrows <= at->size, "the list was built for more records than the atoms hold");
+    // the reference loops over local atoms with the list rows (LennardJones.hpp:146)
+    const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
+    auto* ctx = lj->ctx;
+    const unsigned grid = gridFor(rows);
+    double* partials = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid) * 2 * sizeof(double)));
+    LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, vl->stride, lj->dtable.p, int(lj->table.size()),
+             int(lj->typeStride), lj->rcSqr, pred ? *pred : mrmd_pred2{}, pred && pred->op != MRMD_PAIR_ALWAYS};
+    const bool full = lj->symmetric && predIsSymmetric(pred);
+    if (rows > 0)
+    {
+        if (full)
+        {
+            KTimer t(ctx, "lj_force");
+            if (a.usePred)
+                hipLaunchKernelGGL(k_ljFull<true>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p, vl->cnt.p,
+                                   partials);
+            else
+                hipLaunchKernelGGL(k_ljFull<false>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p, vl->cnt.p,
+                                   partials);
+        }
+        else
+        {
+            vl->ensureHalf();
+            KTimer t(ctx, "lj_force_half");
+            if (a.usePred)
+                hipLaunchKernelGGL(k_ljHalf<true>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->hnbr.p, vl->hcnt.p,
+                                   partials);
+            else
+                hipLaunchKernelGGL(k_ljHalf<false>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->hnbr.p, vl->hcnt.p,
+                                   partials);
+        }
+        hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, partials, int64_t(grid), 2, lj->ev.p);
+    }
+    else
+        hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, ctx->stream, lj->ev.p, int64_t(2));
+    checkLaunch();
+    lj->evValid = false;
+    MRMD_API_END
+}
+
+int mrmd_lj_energy_virial(mrmd_lj* lj, double* e, double* w)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj, "null lj");
+    if (!lj->evValid)
+    {
+        HIPCHECK(hipMemcpyAsync(lj->ctx->pinned, lj->ev.p, 16, hipMemcpyDeviceToHost, lj->ctx->stream));
+        sync(lj->ctx);
+        lj->hE = lj->ctx->pinned[0];
+        lj->hW = lj->ctx->pinned[1];
+        lj->evValid = true;
+    }
+    if (e) *e = lj->hE;
+    if (w) *w = lj->hW;
+    MRMD_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// adjoint entry: ghost partners take the upstream gradient of their real atom
+namespace mrmd_hip
+{
+const int32_t* ghostCorrDevice(const mrmd_ghost_layer* gl);
+}
+
+extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_ghost_layer* gl,
+                               const double* dLdF, double* dLdx, double* dLdeps, double* dLdsigma, int device)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj && at && vl && dLdF && dLdx && dLdeps && dLdsigma, "invalid arguments");
+    REQUIRE(lj->symmetric, "the adjoint needs a symmetric parameter table");
+    REQUIRE(vl->begin == 0, "the list rows must start at record 0");
+    auto* ctx = lj->ctx;
+    const int64_t nl = at->nlocal, nall = at->nlocal + at->nghost;
+    REQUIRE(at->nghost == 0 || gl, "ghost records present: the ghost layer is required");
+    const int64_t rows = std::min<int64_t>(nl, vl->end);
+    const int nt2 = int(lj->table.size());
+    // recover sigma / epsilon of each pair from the table: ef2 = 4 eps s^6, ef1 = 4 eps s^12
+    std::vector<double> sig(static_cast<size_t>(nt2)), ep(static_cast<size_t>(nt2));
+    for (int k = 0; k < nt2; ++k)
+    {
+        const auto& p = lj->table[size_t(k)];
+        const double s6 = p.ef1 / p.ef2;
+        sig[size_t(k)] = std::pow(s6, 1.0 / 6.0);
+        ep[size_t(k)] = p.ef2 * p.ef2 / (4.0 * p.ef1);
+    }
+    const int64_t gstride = std::max<int64_t>(nall, 1);
+    DBuf<double> g, out, pg, sd;
+    g.reserve(ctx, 3 * gstride);
+    out.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
+    pg.reserve(ctx, 2 * nt2);
+    sd.reserve(ctx, 2 * nt2);
+    HIPCHECK(hipMemcpyAsync(sd.p, sig.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(sd.p + nt2, ep.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
+    // upstream: row-major n x 3 -> component-major with stride gstride
+    {
+        DBuf<double> tmp;
+        tmp.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
+        HIPCHECK(hipMemcpyAsync(tmp.p, dLdF, size_t(3 * nl) * 8,
+                                device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+        if (nl) hipLaunchKernelGGL(k_rowMajorToSoA, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, tmp.p, g.p, nl, gstride);
+        if (nall > nl)
+            hipLaunchKernelGGL(k_expandGhostUpstream, dim3(gridFor(nall - nl)), dim3(BLOCK), 0, ctx->stream, g.p, gstride,
+                               mrmd_hip::ghostCorrDevice(gl), nl, nall);
+        sync(ctx);
+    }
+    hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(BLOCK), 0, ctx->stream, pg.p, int64_t(2 * nt2));
+    LJArgs a{at->pos.p, nullptr, at->cap, 0, rows, vl->stride, lj->dtable.p, nt2, int(lj->typeStride), lj->rcSqr,
+             mrmd_pred2{}, 0};
+    if (rows > 0)
+    {
+        KTimer t(ctx, "lj_adjoint");
+        if (nt2 <= 4)
+            hipLaunchKernelGGL(k_ljAdjoint<true>, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p,
+                               vl->cnt.p, g.p, gstride, sd.p, sd.p + nt2, out.p, std::max<int64_t>(nl, 1), pg.p);
+        else
+            hipLaunchKernelGGL(k_ljAdjoint<false>, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p,
+                               vl->cnt.p, g.p, gstride, sd.p, sd.p + nt2, out.p, std::max<int64_t>(nl, 1), pg.p);
+    }
+    checkLaunch();
+    // outputs
+    DBuf<double> rm;
+    rm.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
+    if (nl) hipLaunchKernelGGL(k_soaToRowMajor, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, out.p, rm.p, nl,
+                               std::max<int64_t>(nl, 1));
+    HIPCHECK(hipMemcpyAsync(dLdx, rm.p, size_t(3 * nl) * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            ctx->stream));
+    std::vector<double> hp(static_cast<size_t>(2 * nt2));
+    HIPCHECK(hipMemcpyAsync(hp.data(), pg.p, size_t(2 * nt2) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    // symmetrise: the parameter of pair {a,b} is shared by table entries (a,b) and (b,a)
+    const int64_t n = lj->numTypes, st = lj->typeStride;
+    std::vector<double> ge(size_t(nt2), 0.0), gs(size_t(nt2), 0.0);
+    for (int64_t x = 0; x < n; ++x)
+        for (int64_t y = 0; y < n; ++y)
+        {
+            const int64_t i1 = x * st + y, i2 = y * st + x;
+            if (i1 >= nt2 || i2 >= nt2) continue;
+            ge[size_t(i1)] = i1 == i2 ? hp[size_t(i1)] : hp[size_t(i1)] + hp[size_t(i2)];
+            gs[size_t(i1)] = i1 == i2 ? hp[size_t(nt2 + i1)] : hp[size_t(nt2 + i1)] + hp[size_t(nt2 + i2)];
+        }
+    if (device)
+    {
+        HIPCHECK(hipMemcpyAsync(dLdeps, ge.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(dLdsigma, gs.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
+        sync(ctx);
+    }
+    else
+    {
+        std::memcpy(dLdeps, ge.data(), size_t(nt2) * 8);
+        std::memcpy(dLdsigma, gs.data(), size_t(nt2) * 8);
+    }
+    MRMD_API_END
+}

@@ -1,0 +1,260 @@
+"""GPU parity of the plain LJ hot path against the CPU oracle (calls go through the C ABI).
+
+Tolerances: integer/index work (ghost order, cell assignment, neighbour sets) is bit-exact;
+element-wise integrators and the Philox noise are bit-exact given identical inputs; pair
+forces are compared with max_i |F_gpu - F_cpu| <= 1e-12 * max_i |F_cpu| (the device sums
+both directions of every pair in another order than the half-list CPU loop).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from mrmd_amd.lattice import fcc, maxwell_boltzmann
+
+pytestmark = pytest.mark.gpu
+
+L_ESPP = 33.8585
+F_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mrmd_amd
+    return mrmd_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(M):
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def _both(M, ctx, pos, **kw):
+    g = M.Atoms.from_arrays(ctx, pos, **kw)
+    c = O.Atoms.from_arrays(pos, **{k: v for k, v in kw.items() if k != "charge"})
+    return g, c
+
+
+def _espp(M, ctx, espp_positions):
+    gbox = M.make_box([0, 0, 0], [L_ESPP] * 3, 2.8)
+    cbox = O.make_box([0, 0, 0], [L_ESPP] * 3, 2.8)
+    g, c = _both(M, ctx, espp_positions)
+    return gbox, cbox, g, c
+
+
+def test_espp_ghosts_bit_exact(M, ctx, espp_positions):
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    assert gl.create_ghost_atoms(g, gbox) == 22104
+    og = O.Ghost()
+    og.create_xyz(c, cbox)
+    n = g.nlocal + g.nghost
+    assert n == c.nlocal + c.nghost
+    np.testing.assert_array_equal(g.download(M.POS, n), c.get(O.POS, n))
+    np.testing.assert_array_equal(gl.corresponding(n), og.corr(n))
+
+
+def test_espp_verlet_pairs_sets_energy_forces(M, ctx, espp_positions):
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    og = O.Ghost()
+    og.create_xyz(c, cbox)
+    vl = M.HalfVerletList(ctx)
+    vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+    assert vl.num_pairs() == 1310403
+    ov = O.Verlet()
+    ov.build(c, c.nlocal + c.nghost, 0, c.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+    # half-list neighbour sets, every row
+    gc, _ = vl.half_counts(g.nlocal)
+    np.testing.assert_array_equal(gc, ov.counts()[: g.nlocal])
+    grows = vl.half_rows(g.nlocal)
+    onb = ov.neighbors()
+    ocnt = ov.counts()
+    for i in range(g.nlocal):
+        assert grows[i] == set(onb[i, : ocnt[i]].tolist()), i
+    # full rows are the union of both directions
+    full, fcnt = vl.full_rows(g.nlocal)
+    assert int(fcnt.sum()) > 2 * 1310403 - 22104 * 40
+    # unshifted LJ energy of the ESPResSo++ comparison
+    lj = M.LennardJones(ctx, 2.5, 1.0, 1.0)
+    g.set_force(0.0)
+    lj.apply(g, vl)
+    gl.contribute_back_ghost_to_real(g)
+    E, W = lj.energy_virial()
+    assert abs(E - (-94795.927)) < 5e-4
+    olj = O.LJ(0.0, 2.5, 1.0, 1.0)
+    O.lib().or_set_threads(1)
+    cE, cW = olj.apply(c, ov)
+    og.fold(c)
+    assert E == pytest.approx(cE, rel=1e-12)
+    assert W == pytest.approx(cW, rel=1e-12)
+    Fg = g.download(M.FORCE, g.nlocal)
+    Fc = c.get(O.FORCE, c.nlocal)
+    assert np.abs(Fg - Fc).max() <= F_TOL * np.abs(Fc).max()
+
+
+def test_espp_integrators_bit_exact(M, ctx, espp_positions):
+    rng = np.random.default_rng(5)
+    n = espp_positions.shape[0]
+    vel = rng.standard_normal((n, 3))
+    force = rng.standard_normal((n, 3)) * 10
+    mass = 1.0 + rng.random(n)
+    g, c = _both(M, ctx, espp_positions, vel=vel, force=force, mass=mass)
+    d_g = M.VelocityVerlet.pre_force_integrate(g, 0.005)
+    d_c = O.lib().or_vv_pre(c.h, 0.005)
+    assert d_g == d_c
+    np.testing.assert_array_equal(g.download(M.POS, n), c.get(O.POS, n))
+    np.testing.assert_array_equal(g.download(M.VEL, n), c.get(O.VEL, n))
+    M.VelocityVerlet.post_force_integrate(g, 0.005)
+    O.lib().or_vv_post(c.h, 0.005)
+    np.testing.assert_array_equal(g.download(M.VEL, n), c.get(O.VEL, n))
+    # Langevin: counter-based noise gives identical bits on both sides
+    lang = M.VelocityVerletLangevinThermostat(1.0, 1.2, seed=99)
+    lang.call = 17
+    d_g = lang.pre_force_integrate(g, 0.005)
+    d_c = O.lib().or_langevin_pre(c.h, 0.005, 1.0, 1.2, 99, 17, None)
+    assert d_g == d_c
+    np.testing.assert_array_equal(g.download(M.VEL, n), c.get(O.VEL, n))
+    np.testing.assert_array_equal(g.download(M.POS, n), c.get(O.POS, n))
+    # predicate-restricted thermostat (symmetric slab around the box centre)
+    p = M.pred1(M.PRED_SYMMETRIC_SLAB, 0, a=L_ESPP / 2, b=0.0, c=5.0)
+    lang.call = 18
+    lang.pre_force_integrate(g, 0.005, pred=p)
+    op = O.pred1(kind=3, axis=0, a=L_ESPP / 2, b=0.0, c=5.0)
+    O.lib().or_langevin_pre(c.h, 0.005, 1.0, 1.2, 99, 18, O.C.byref(op))
+    np.testing.assert_array_equal(g.download(M.VEL, n), c.get(O.VEL, n))
+    assert g.kinetic_energy() == pytest.approx(O.lib().or_kinetic_energy(c.h), rel=1e-13)
+
+
+def test_cell_list_and_permute_bit_exact(M, ctx, espp_positions):
+    g, c = _both(M, ctx, espp_positions, vel=np.arange(3 * 32768, dtype=float).reshape(-1, 3))
+    delta = [2.8] * 3
+    gcl = M.LinkedCellList(ctx, g, 0, g.nlocal, delta, [0, 0, 0], [L_ESPP] * 3)
+    occ = O.Cells(c, 0, c.nlocal, delta, [0, 0, 0], [L_ESPP] * 3)
+    for a, b in zip(gcl.download(), occ.get()):
+        np.testing.assert_array_equal(a, b)
+    g.permute(gcl)
+    occ.permute(c)
+    np.testing.assert_array_equal(g.download(M.POS), c.get(O.POS))
+    np.testing.assert_array_equal(g.download(M.VEL), c.get(O.VEL))
+
+
+def test_ghost_update_and_fold(M, ctx, espp_positions):
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    og = O.Ghost()
+    og.create_xyz(c, cbox)
+    n = g.nlocal + g.nghost
+    rng = np.random.default_rng(3)
+    moved = c.get(O.POS, n)
+    moved[: c.nlocal] += rng.uniform(-0.1, 0.1, (c.nlocal, 3))
+    g.upload(M.POS, moved)
+    c.set(O.POS, moved)
+    gl.update_ghost_atoms(g, gbox)
+    og.update(c, cbox)
+    np.testing.assert_array_equal(g.download(M.POS, n), c.get(O.POS, n))
+    f = rng.standard_normal((n, 3))
+    g.upload(M.FORCE, f)
+    c.set(O.FORCE, f)
+    gl.contribute_back_ghost_to_real(g)
+    og.fold(c)
+    np.testing.assert_allclose(g.download(M.FORCE, n), c.get(O.FORCE, n), rtol=1e-13, atol=1e-13)
+
+
+def _fcc_system(M, ctx, cells, rho=0.8442, T=0.72, seed=42):
+    pos, L = fcc(cells, rho)
+    vel = maxwell_boltzmann(pos.shape[0], T, seed)
+    return pos, vel, L
+
+
+@pytest.mark.parametrize("pred_kind", ["symmetric", "asymmetric"])
+def test_lj_apply_if_predicates(M, ctx, pred_kind):
+    pos, vel, L = _fcc_system(M, ctx, 6)
+    pos = pos + np.random.default_rng(1).uniform(-0.15, 0.15, pos.shape)
+    pos = np.mod(pos, L)
+    g, c = _both(M, ctx, pos)
+    gbox = M.make_box([0, 0, 0], [L] * 3, 2.8)
+    cbox = O.make_box([0, 0, 0], [L] * 3, 2.8)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    og = O.Ghost()
+    og.create_xyz(c, cbox)
+    vl = M.HalfVerletList(ctx)
+    vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L + 2.8] * 3)
+    ov = O.Verlet()
+    ov.build(c, c.nlocal + c.nghost, 0, c.nlocal, 2.8, 1.0, [-2.8] * 3, [L + 2.8] * 3)
+    region = dict(kind=M.PRED_OPEN_RANGE, axis=0, a=0.3 * L, b=0.6 * L)
+    other = dict(kind=M.PRED_LESS, axis=1, a=0.5 * L)
+    if pred_kind == "symmetric":
+        gp = M.pred2(M.PAIR_OR, M.pred1(**region))
+        cp = O.pred2(2, O.pred1(**region))
+    else:
+        gp = M.pred2(M.PAIR_AND, M.pred1(**region), M.pred1(**other))
+        cp = O.pred2(1, O.pred1(**region), O.pred1(**other))
+    lj = M.LennardJones(ctx, 2.5, 1.0, 1.0, capping_distance=0.9, shifted=True)
+    olj = O.LJ(0.9, 2.5, 1.0, 1.0, 1, shifted=True)
+    g.set_force(0.0)
+    lj.apply(g, vl, gp)
+    gl.contribute_back_ghost_to_real(g)
+    cE, cW = olj.apply(c, ov, cp)
+    og.fold(c)
+    E, W = lj.energy_virial()
+    assert E == pytest.approx(cE, rel=1e-12)
+    assert W == pytest.approx(cW, rel=1e-12)
+    Fg = g.download(M.FORCE, g.nlocal)
+    Fc = c.get(O.FORCE, c.nlocal)
+    assert np.abs(Fg - Fc).max() <= F_TOL * np.abs(Fc).max()
+
+
+def test_lj_eval_and_table(M, ctx):
+    lj = M.LennardJones(ctx, 2.5 * 3.01, 3.01, 2.0, capping_distance=0.1, shifted=True)
+    olj = O.LJ(0.1, 2.5 * 3.01, 3.01, 2.0, 1, shifted=True)
+    np.testing.assert_array_equal(lj.table(), olj.table())
+    r = 0.2 + 0.1 * np.arange(100)
+    ff, e = lj.eval(r * r)
+    for k, x in enumerate(r):
+        cff, ce = olj.eval(x * x)
+        assert ff[k] == pytest.approx(cff, rel=1e-15, abs=0)
+        assert e[k] == pytest.approx(ce, rel=1e-14, abs=1e-300)
+
+
+def test_verlet_regrow_and_edge_cases(M, ctx):
+    # tiny max_neigh forces the re-count path; empty and single-atom systems
+    pos, _, L = _fcc_system(M, ctx, 3)
+    g, c = _both(M, ctx, pos)
+    gl = M.GhostLayer(ctx)
+    gbox = M.make_box([0, 0, 0], [L] * 3, 2.8)
+    gl.create_ghost_atoms(g, gbox)
+    vl = M.HalfVerletList(ctx)
+    vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L + 2.8] * 3, max_neigh=1)
+    og = O.Ghost()
+    og.create_xyz(c, O.make_box([0, 0, 0], [L] * 3, 2.8))
+    ov = O.Verlet()
+    ov.build(c, c.nlocal + c.nghost, 0, c.nlocal, 2.8, 1.0, [-2.8] * 3, [L + 2.8] * 3)
+    assert vl.num_pairs() == ov.total()
+    e = M.Atoms(ctx, 0)
+    vl2 = M.HalfVerletList(ctx)
+    vl2.build(e, 0, 0, 2.8, 1.0, [0, 0, 0], [10, 10, 10])
+    assert vl2.num_pairs() == 0
+    lj = M.LennardJones(ctx, 2.5, 1.0, 1.0)
+    lj.apply(e, vl2)
+    assert lj.energy_virial() == (0.0, 0.0)
+    assert M.VelocityVerlet.pre_force_integrate(e, 0.005) == 0.0
+    s = M.Atoms.from_arrays(ctx, np.array([[1.0, 1.0, 1.0]]))
+    vl2.build(s, 0, 1, 2.8, 1.0, [0, 0, 0], [10, 10, 10])
+    assert vl2.num_pairs() == 0
+
+
+def test_invalid_arguments_fail_loudly(M, ctx):
+    a = M.Atoms(ctx, 4)
+    with pytest.raises(M.MrmdError):
+        a.download(M.POS, 10)
+    with pytest.raises(M.MrmdError):
+        M.make_box  # noqa: B018 - attribute exists
+        M.GhostLayer(ctx).create_ghost_atoms(a, M.make_box([0, 0, 0], [1, 1, 1], 2.0))
+    bad = M.pred1(kind=99)
+    with pytest.raises(M.MrmdError):
+        M.VelocityVerletLangevinThermostat(1.0, 1.0).pre_force_integrate(a, 0.1, pred=bad)
