@@ -636,6 +636,9 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     MRMD_API_BEGIN(s ? s->ctx : nullptr)
     REQUIRE(s && m && vl && a, "invalid arguments");
     REQUIRE(vl->begin == 0, "the molecule list rows must start at record 0");
+    REQUIRE(a->maxType < s->numTypes, "atom type %lld outside the %lld-type table", (long long)a->maxType,
+            (long long)s->numTypes);
+    REQUIRE(vl->nrows <= m->nlocal + m->nghost, "the list was built for more molecules than exist");
     auto* ctx = s->ctx;
     vl->ensureHalf();
     const int64_t rows = std::min<int64_t>(m->nlocal, vl->end);
@@ -755,6 +758,7 @@ int mrmd_tf_sample(mrmd_thermo_force* t, const mrmd_atoms* a)
 {
     MRMD_API_BEGIN(t ? t->ctx : nullptr)
     REQUIRE(t && a, "invalid arguments");
+    REQUIRE(a->maxType < t->hd.numHist, "atom type outside the density profile");
     if (a->nlocal > 0)
     {
         KTimer k(t->ctx, "tf_sample");
@@ -788,6 +792,7 @@ int mrmd_tf_apply(const mrmd_thermo_force* tc, mrmd_atoms* a, const mrmd_pred1* 
     auto* t = const_cast<mrmd_thermo_force*>(tc);
     MRMD_API_BEGIN(t ? t->ctx : nullptr)
     REQUIRE(t && a, "invalid arguments");
+    REQUIRE(a->maxType < t->hd.numHist, "atom type outside the thermodynamic-force table");
     validatePred1(pred);
     if (a->nlocal == 0) return MRMD_OK;
     KTimer k(t->ctx, "tf_apply");
@@ -807,6 +812,7 @@ int mrmd_tf_apply_interpolated(const mrmd_thermo_force* tc, mrmd_atoms* a, const
     auto* t = const_cast<mrmd_thermo_force*>(tc);
     MRMD_API_BEGIN(t ? t->ctx : nullptr)
     REQUIRE(t && a, "invalid arguments");
+    REQUIRE(a->maxType < t->hd.numHist, "atom type outside the thermodynamic-force table");
     validatePred1(pred);
     if (a->nlocal == 0) return MRMD_OK;
     const mrmd_pred1 p = pred ? *pred : mrmd_pred1{};

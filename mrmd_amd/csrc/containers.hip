@@ -449,6 +449,15 @@ int mrmd_atoms_upload(mrmd_atoms* a, int field, const void* host, int64_t off, i
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && host && off >= 0 && n >= 0 && field >= 0 && field <= MRMD_RELMASS, "invalid arguments");
     if (n == 0) return MRMD_OK;
+    if (field == MRMD_TYPE)
+    {
+        const int64_t* t = static_cast<const int64_t*>(host);
+        for (int64_t k = 0; k < n; ++k)
+        {
+            REQUIRE(t[k] >= 0 && t[k] < 64, "atom type %lld outside [0, 64)", (long long)t[k]);
+            a->maxType = std::max(a->maxType, t[k]);
+        }
+    }
     if (off + n > a->size) a->setSize(off + n);
     auto* ctx = a->ctx;
     const bool vec = field <= MRMD_FORCE;

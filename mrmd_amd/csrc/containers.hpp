@@ -42,6 +42,7 @@ struct mrmd_atoms
 {
     mrmd_ctx* ctx = nullptr;
     int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
+    int64_t maxType = 0;  ///< largest type ever uploaded (kernels index tables with it)
     mrmd_hip::DBuf<double4> pos;
     mrmd_hip::DBuf<double> vel, force, mass, charge, relmass;
     mrmd_hip::DBuf<double> stage;  ///< transfer staging

@@ -22,6 +22,7 @@ struct mrmd_ghost_layer
 namespace mrmd_hip
 {
 const int32_t* ghostCorrDevice(const mrmd_ghost_layer* gl) { return gl ? gl->corr.p : nullptr; }
+int64_t ghostCorrSize(const mrmd_ghost_layer* gl) { return gl ? gl->corrSize : 0; }
 }  // namespace mrmd_hip
 
 namespace

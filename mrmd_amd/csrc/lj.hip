@@ -510,6 +510,8 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
     validatePred2(pred);
     REQUIRE(vl->begin == 0 && vl->end <= at->nlocal + at->nghost, "the list rows must start at record 0");
     REQUIRE(vl->nrows <= at->size, "the list was built for more records than the atoms hold");
+    REQUIRE(at->maxType * lj->typeStride + at->maxType < int64_t(lj->table.size()),
+            "atom type %lld has no entry in the %lld-type table", (long long)at->maxType, (long long)lj->numTypes);
     // the reference loops over local atoms with the list rows (LennardJones.hpp:146)
     const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
     auto* ctx = lj->ctx;
@@ -574,6 +576,7 @@ int mrmd_lj_energy_virial(mrmd_lj* lj, double* e, double* w)
 namespace mrmd_hip
 {
 const int32_t* ghostCorrDevice(const mrmd_ghost_layer* gl);
+int64_t ghostCorrSize(const mrmd_ghost_layer* gl);
 }
 
 extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_ghost_layer* gl,
@@ -586,6 +589,9 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     auto* ctx = lj->ctx;
     const int64_t nl = at->nlocal, nall = at->nlocal + at->nghost;
     REQUIRE(at->nghost == 0 || gl, "ghost records present: the ghost layer is required");
+    REQUIRE(at->maxType * lj->typeStride + at->maxType < int64_t(lj->table.size()), "atom type outside the table");
+    REQUIRE(vl->nrows <= nall, "the list was built for more records than the atoms hold");
+    REQUIRE(at->nghost == 0 || mrmd_hip::ghostCorrSize(gl) >= nall, "the ghost layer does not cover these atoms");
     const int64_t rows = std::min<int64_t>(nl, vl->end);
     const int nt2 = int(lj->table.size());
     // recover sigma / epsilon of each pair from the table: ef2 = 4 eps s^6, ef1 = 4 eps s^12

@@ -283,6 +283,7 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
         // Cabana VerletLayout2D: re-count and refill with the larger width
         vl->capFull = hmax + hmax / 8 + 8;
     }
+    if (vl->maxFull > vl->capFull) fail(MRMD_ERR_STATE, "neighbour rows did not fit after a re-count");
 }
 }  // namespace
 
