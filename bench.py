@@ -1,0 +1,138 @@
+"""Throughput benchmark of the plain LJ hot path on one MI355X.
+
+Workload (the reference hot path's headline throughput configuration): N = 4,000,000 atoms
+on an FCC lattice (100^3 cells x 4) at density 0.8442, FP64, Lennard-Jones rc = 2.5 sigma
+(energy-shifted), Verlet skin 0.3, Langevin thermostat T = 1.0, gamma = 1.0, dt = 0.005.
+One step = Langevin pre-force integration, skin check (with wrap + cell permutation +
+ordered ghost creation + Verlet rebuild when due, otherwise a ghost refresh), force
+zeroing, LJ force/energy/virial, ghost force fold, post-force integration: the loop of the
+reference's NVT example, through the library's action API.
+
+Prints one JSON line.  `value` is atom-steps/s of the timed steps with all data resident
+in HBM.  `roofline` reports the LJ force kernel (algorithmic FP64 FLOPs of the reference's
+half-list algorithm, 8 per candidate pair + 24 per pair inside rc with energy and virial,
+divided by the kernel's mean duration from HIP events on the library stream).
+`cpu_baseline` times the CPU restatement of the reference algorithm (AoS, half list with
+atomic F_j, OpenMP) on a bounded sample of the same system.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def build_system(M, ctx, cells, temperature, seed):
+    from mrmd_amd.drivers import LJSystem
+    from mrmd_amd.lattice import fcc, maxwell_boltzmann
+    pos, L = fcc(cells, 0.8442)
+    vel = maxwell_boltzmann(pos.shape[0], temperature, seed)
+    s = LJSystem(ctx, pos, vel, L, rc=2.5, skin=0.3, shifted=True, langevin=True, gamma=1.0,
+                 temperature=temperature, seed=seed, permute=True)
+    return s, pos, vel, L
+
+
+def cpu_baseline(pos, vel, L, steps, warm, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    O.lib().or_set_threads(threads)
+    a = O.Atoms.from_arrays(pos, vel=vel)
+    box = O.make_box([0, 0, 0], [L] * 3, 2.8)
+    prm = O.md_params(nsteps=0, langevin=1, permute=1, shifted=1, gamma=1.0, temperature=1.0, seed=1234)
+    md = O.lib().or_md_new(a.h, O.C.byref(box), O.C.byref(prm))
+    O.lib().or_md_steps(md, warm)
+    r0 = O.lib().or_md_steps(md, 0)
+    t0 = time.perf_counter()
+    r1 = O.lib().or_md_steps(md, steps)
+    dt = time.perf_counter() - t0
+    O.lib().or_md_free(md)
+    return pos.shape[0] * steps / dt, dt, r1 - r0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=100, help="FCC cells per edge (100 -> 4M atoms)")
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import mrmd_amd as M
+    ctx = M.Context(0)
+    s, pos, vel, L = build_system(M, ctx, args.cells, 1.0, 1234)
+    n = pos.shape[0]
+    dt = 0.005
+    for _ in range(args.warmup):
+        s.step(dt)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    reb0 = s.rebuilds
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(dt)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    rebuilds = s.rebuilds - reb0
+    launches, force_ms = ctx.kernel_stats("lj_force")
+    stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "langevin_pre", "vv_post", "halo_create",
+                                              "halo_update", "halo_fold", "atoms_permute", "cell_list_build")}
+    ctx.set_profiling(False)
+    half_rn, _ = s.vl.count_within(s.atoms, 2.8)
+    half_rc, _ = s.vl.count_within(s.atoms, 2.5)
+    flops = 8.0 * half_rn + 24.0 * half_rc
+    avg_s = force_ms / max(launches, 1) / 1e3
+    achieved_tf = flops / avg_s / 1e12
+    value = n * args.steps / elapsed
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cv, ct, creb = cpu_baseline(pos, vel, L, args.cpu_steps, args.cpu_warmup, threads)
+        cpu = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {n}-atom system after {args.cpu_warmup} "
+                         f"warm-up step(s), {creb} list rebuild(s) inside, {ct:.1f} s wall"}
+
+    out = {
+        "metric": "atom-steps/s at 4M-atom LJ",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic FCC lattice + Maxwell-Boltzmann velocities (seeded)",
+        "config": {"workload": "plain LJ NVT (Langevin), FP64, FCC 100^3 x 4", "atoms": n,
+                   "density": 0.8442, "rc": 2.5, "skin": 0.3, "dt": dt, "temperature": 1.0, "gamma": 1.0,
+                   "rebuilds_in_timed_steps": rebuilds},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_ljFull (LJ force)", "avg_ms": avg_s * 1e3, "launches": launches,
+                     "half_pairs_rn": half_rn, "half_pairs_rc": half_rc, "flop_per_launch": flops},
+        "cpu_baseline": cpu,
+        "kernel_ms": {k: {"launches": v[0], "total_ms": v[1]} for k, v in stats.items()},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

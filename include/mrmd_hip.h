@@ -248,6 +248,11 @@ int mrmd_verlet_full_neighbors(mrmd_verlet_list* vl, int32_t* host, int32_t* cou
                                int64_t row_stride);
 int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* max_full, int64_t* total_full,
                      int32_t* dims);
+/* pairs of the list closer than r at the atoms' current positions: half-list pairs and
+ * both-direction entries (the algorithmic pair counts behind the force-kernel FLOP figure;
+ * no reference counterpart) */
+int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* atoms, double r, int64_t* half_pairs,
+                             int64_t* full_entries);
 
 /* =========================================================================================
  * Lennard-Jones  (action::LennardJones, mrmd/action/LennardJones.hpp:98-206)

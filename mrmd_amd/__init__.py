@@ -359,6 +359,12 @@ class HalfVerletList:
         self.ctx.check(_L.mrmd_verlet_info(self.h, C.byref(rows), C.byref(mx), C.byref(tot), _p(dims)))
         return dict(rows=rows.value, max_full=mx.value, total_full=tot.value, dims=dims)
 
+    def count_within(self, atoms, r):
+        """(half-list pairs, both-direction entries) closer than r at the current positions"""
+        h, f = C.c_int64(), C.c_int64()
+        self.ctx.check(_L.mrmd_verlet_count_within(self.h, atoms.h, float(r), C.byref(h), C.byref(f)))
+        return h.value, f.value
+
     def half_counts(self, n):
         out = np.zeros(n, np.int32)
         tot = C.c_int64()

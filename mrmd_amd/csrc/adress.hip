@@ -514,7 +514,7 @@ HistD makeHist(double mn, double mx, int64_t nb, int64_t nh)
 
 struct mrmd_lj_idealgas
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     int64_t numTypes = 1;
     std::vector<Pre> table;
     double rcSqr = 0.0;
@@ -527,7 +527,7 @@ struct mrmd_lj_idealgas
 
 struct mrmd_thermo_force
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     HistD hd{};
     double max = 0.0;
     double binVolume = 1.0;
@@ -639,7 +639,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     REQUIRE(a->maxType < s->numTypes, "atom type %lld outside the %lld-type table", (long long)a->maxType,
             (long long)s->numTypes);
     REQUIRE(vl->nrows <= m->nlocal + m->nghost, "the list was built for more molecules than exist");
-    auto* ctx = s->ctx;
+    mrmd_ctx* ctx = s->ctx;
     vl->ensureHalf();
     const int64_t rows = std::min<int64_t>(m->nlocal, vl->end);
     const unsigned grid = gridFor(rows);

@@ -73,7 +73,45 @@ struct mrmd_ctx
     int64_t bytes = 0;         // device bytes held by objects of this context
     void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};  // per-purpose device scratch
     size_t scratchBytes[4] = {0, 0, 0, 0};
+    int refs = 1;  // the user's handle plus one per live object created on this context
 };
+
+namespace mrmd_hip
+{
+void ctxRetain(mrmd_ctx* c);
+void ctxRelease(mrmd_ctx* c);
+
+// owning pointer of an object to its context: the context is freed only after the last
+// object created on it (mrmd_ctx_destroy drops the user's reference)
+struct CtxPtr
+{
+    mrmd_ctx* p = nullptr;
+    CtxPtr() = default;
+    CtxPtr(std::nullptr_t) {}
+    CtxPtr(mrmd_ctx* c) : p(c)
+    {
+        if (p) ctxRetain(p);
+    }
+    CtxPtr(const CtxPtr& o) : p(o.p)
+    {
+        if (p) ctxRetain(p);
+    }
+    CtxPtr& operator=(const CtxPtr& o) { return *this = o.p; }
+    CtxPtr& operator=(mrmd_ctx* c)
+    {
+        if (c) ctxRetain(c);
+        if (p) ctxRelease(p);
+        p = c;
+        return *this;
+    }
+    ~CtxPtr()
+    {
+        if (p) ctxRelease(p);
+    }
+    operator mrmd_ctx*() const { return p; }
+    mrmd_ctx* operator->() const { return p; }
+};
+}  // namespace mrmd_hip
 
 namespace mrmd_hip
 {

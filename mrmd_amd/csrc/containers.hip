@@ -308,6 +308,16 @@ int mrmd_ctx_create(int device, mrmd_ctx** out)
 int mrmd_ctx_destroy(mrmd_ctx* c)
 {
     if (!c) return MRMD_OK;
+    ctxRelease(c);
+    return MRMD_OK;
+}
+}  // extern "C"
+
+void mrmd_hip::ctxRetain(mrmd_ctx* c) { ++c->refs; }
+
+void mrmd_hip::ctxRelease(mrmd_ctx* c)
+{
+    if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->stats)
@@ -322,8 +332,9 @@ int mrmd_ctx_destroy(mrmd_ctx* c)
         if (c->scratch[s]) (void)hipFree(c->scratch[s]);
     if (c->ownStream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
-    return MRMD_OK;
 }
+
+extern "C" {
 
 const char* mrmd_ctx_last_error(const mrmd_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
@@ -459,7 +470,7 @@ int mrmd_atoms_upload(mrmd_atoms* a, int field, const void* host, int64_t off, i
         }
     }
     if (off + n > a->size) a->setSize(off + n);
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     const bool vec = field <= MRMD_FORCE;
     const int64_t bytes = (vec ? 3 : 1) * n * 8;
     a->stage.reserve(ctx, (vec ? 3 : 1) * n);
@@ -503,7 +514,7 @@ int mrmd_atoms_download(const mrmd_atoms* ac, int field, void* host, int64_t off
             "invalid arguments (offset %lld, n %lld, size %lld)", (long long)off, (long long)n,
             a ? (long long)a->size : -1LL);
     if (n == 0) return MRMD_OK;
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     const bool vec = field <= MRMD_FORCE;
     a->stage.reserve(ctx, (vec ? 3 : 1) * n);
     const dim3 g(gridFor(n)), b(BLOCK);
@@ -546,7 +557,7 @@ int mrmd_atoms_fill(mrmd_atoms* a, int field, double value)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && field >= 0 && field <= MRMD_RELMASS, "invalid arguments");
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     const int64_t n = a->size;
     if (n == 0) return MRMD_OK;
     switch (field)
@@ -678,7 +689,7 @@ int mrmd_molecules_upload(mrmd_molecules* m, int field, const void* host, int64_
     REQUIRE(m && host && off >= 0 && n >= 0 && field >= 0 && field <= MRMD_MOL_NUMATOMS, "invalid arguments");
     if (n == 0) return MRMD_OK;
     if (off + n > m->size) m->setSize(off + n);
-    auto* ctx = m->ctx;
+    mrmd_ctx* ctx = m->ctx;
     const bool vec = field == MRMD_MOL_POS || field == MRMD_MOL_FORCE || field == MRMD_MOL_GRADLAMBDA;
     m->stage.reserve(ctx, (vec ? 3 : 1) * n);
     HIPCHECK(hipMemcpyAsync(m->stage.p, host, size_t((vec ? 3 : 1) * n) * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -719,7 +730,7 @@ int mrmd_molecules_download(const mrmd_molecules* mc, int field, void* host, int
     REQUIRE(m && host && off >= 0 && n >= 0 && off + n <= m->size && field >= 0 && field <= MRMD_MOL_NUMATOMS,
             "invalid arguments");
     if (n == 0) return MRMD_OK;
-    auto* ctx = m->ctx;
+    mrmd_ctx* ctx = m->ctx;
     const bool vec = field == MRMD_MOL_POS || field == MRMD_MOL_FORCE || field == MRMD_MOL_GRADLAMBDA;
     m->stage.reserve(ctx, (vec ? 3 : 1) * n);
     const dim3 g(gridFor(n)), b(BLOCK);
@@ -761,7 +772,7 @@ int mrmd_molecules_fill(mrmd_molecules* m, int field, double value)
 {
     MRMD_API_BEGIN(m ? m->ctx : nullptr)
     REQUIRE(m && field >= 0 && field <= MRMD_MOL_NUMATOMS, "invalid arguments");
-    auto* ctx = m->ctx;
+    mrmd_ctx* ctx = m->ctx;
     const int64_t n = m->size;
     if (n == 0) return MRMD_OK;
     const dim3 g(gridFor(n)), b(BLOCK);

@@ -40,7 +40,7 @@ struct MolView
 
 struct mrmd_atoms
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
     int64_t maxType = 0;  ///< largest type ever uploaded (kernels index tables with it)
     mrmd_hip::DBuf<double4> pos;
@@ -59,7 +59,7 @@ struct mrmd_atoms
 
 struct mrmd_molecules
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
     mrmd_hip::DBuf<double4> pos;
     mrmd_hip::DBuf<double> force, lambda, modLambda, grad;

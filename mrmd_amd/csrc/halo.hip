@@ -12,7 +12,7 @@ using namespace mrmd_hip;
 
 struct mrmd_ghost_layer
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     DBuf<int32_t> corr;  ///< correspondingRealAtom, -1 for real atoms
     int64_t corrSize = 0;
     DBuf<int32_t> tileCnt;   ///< 4 x tiles counts
@@ -402,7 +402,7 @@ void resetCorr(mrmd_ghost_layer* gl, int64_t nlocal)
 
 void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int axis)
 {
-    auto* ctx = gl->ctx;
+    mrmd_ctx* ctx = gl->ctx;
     const SubD s = makeSub(box);
     const int64_t total = a->nlocal + a->nghost;
     if (total >= (int64_t(1) << 31) - 1) fail(MRMD_ERR_UNSUP, "more than 2^31 records");
@@ -437,7 +437,7 @@ void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int ax
 
 void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box, int axis)
 {
-    auto* ctx = gl->ctx;
+    mrmd_ctx* ctx = gl->ctx;
     const SubD s = makeSub(box);
     const int64_t totM = m->nlocal + m->nghost;
     const int64_t totA = a->nlocal + a->nghost;

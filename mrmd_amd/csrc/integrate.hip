@@ -151,7 +151,7 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a, "null atoms");
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     double* ms = dispScalar(a);
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
@@ -181,7 +181,7 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a, "null atoms");
     validatePred1(pred);
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     double* ms = dispScalar(a);
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
@@ -206,7 +206,7 @@ int mrmd_kinetic_energy(const mrmd_atoms* ac, double* energy)
     auto* a = const_cast<mrmd_atoms*>(ac);
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && energy, "invalid arguments");
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     const unsigned g = gridFor(a->nlocal);
     double* part = static_cast<double*>(ctxScratch(ctx, 1, size_t(g + 1) * sizeof(double)));
     if (a->nlocal > 0)

@@ -83,7 +83,7 @@ void buildTable(std::vector<Pre>& t, const double* cap, const double* rc, const 
 
 struct mrmd_lj
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     int64_t numTypes = 1, typeStride = 1;
     bool shifted = false;
     bool symmetric = true;  ///< table(ti*stride+tj) == table(tj*stride+ti) for all types
@@ -482,7 +482,7 @@ int mrmd_lj_eval(mrmd_lj* lj, const double* d2, const int64_t* idx, int64_t n, d
     for (int64_t k = 0; k < n; ++k)
         REQUIRE(idx[k] >= 0 && idx[k] < int64_t(lj->table.size()), "type index %lld out of range", (long long)idx[k]);
     if (n == 0) return MRMD_OK;
-    auto* ctx = lj->ctx;
+    mrmd_ctx* ctx = lj->ctx;
     double *dd, *dff, *de;
     int64_t* di;
     HIPCHECK(hipMalloc(&dd, size_t(n) * 8));
@@ -514,7 +514,7 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
             "atom type %lld has no entry in the %lld-type table", (long long)at->maxType, (long long)lj->numTypes);
     // the reference loops over local atoms with the list rows (LennardJones.hpp:146)
     const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
-    auto* ctx = lj->ctx;
+    mrmd_ctx* ctx = lj->ctx;
     const unsigned grid = gridFor(rows);
     double* partials = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid) * 2 * sizeof(double)));
     LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, vl->stride, lj->dtable.p, int(lj->table.size()),
@@ -586,7 +586,7 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     REQUIRE(lj && at && vl && dLdF && dLdx && dLdeps && dLdsigma, "invalid arguments");
     REQUIRE(lj->symmetric, "the adjoint needs a symmetric parameter table");
     REQUIRE(vl->begin == 0, "the list rows must start at record 0");
-    auto* ctx = lj->ctx;
+    mrmd_ctx* ctx = lj->ctx;
     const int64_t nl = at->nlocal, nall = at->nlocal + at->nghost;
     REQUIRE(at->nghost == 0 || gl, "ghost records present: the ghost layer is required");
     REQUIRE(at->maxType * lj->typeStride + at->maxType < int64_t(lj->table.size()), "atom type outside the table");

@@ -226,6 +226,38 @@ __global__ void k_halfExtract(const uint32_t* nbr, const int32_t* cnt, int64_t n
     hcnt[r] = h;
 }
 
+__global__ void k_countWithin(const double4* pos, const uint32_t* nbr, const int32_t* cnt, int64_t begin, int64_t rows,
+                              int64_t stride, double r2, unsigned long long* out)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long h = 0, f = 0;
+    if (r < rows)
+    {
+        const double4 pi = pos[begin + r];
+        for (int k = 0; k < cnt[r]; ++k)
+        {
+            const uint32_t e = nbr[int64_t(k) * stride + r];
+            const double4 pj = pos[e & 0x7fffffffu];
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            if (dx * dx + dy * dy + dz * dz <= r2)
+            {
+                ++f;
+                if (e & 0x80000000u) ++h;
+            }
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+    {
+        h += __shfl_xor(h, off, WAVE);
+        f += __shfl_xor(f, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0)
+    {
+        atomicAdd(&out[0], h);
+        atomicAdd(&out[1], f);
+    }
+}
+
 __global__ void k_rowsToHost(const uint32_t* nbr, const int32_t* cnt, int64_t nrows, int64_t stride, int32_t* out,
                              int64_t rowStride, int mask)
 {
@@ -243,7 +275,7 @@ __global__ void k_rowsToHost(const uint32_t* nbr, const int32_t* cnt, int64_t nr
 void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t begin, int64_t end, double rn,
                  double ratio, const double* gmin, const double* gmax, int64_t maxNeigh)
 {
-    auto* ctx = vl->ctx;
+    mrmd_ctx* ctx = vl->ctx;
     REQUIRE(begin >= 0 && begin <= end && end <= nall, "invalid row range [%lld, %lld) of %lld", (long long)begin,
             (long long)end, (long long)nall);
     REQUIRE(rn > 0.0 && ratio > 0.0, "invalid radius / cell ratio");
@@ -373,7 +405,7 @@ int mrmd_cell_list_download(const mrmd_cell_list* clc, int32_t* cellOf, int32_t*
     auto* cl = const_cast<mrmd_cell_list*>(clc);
     MRMD_API_BEGIN(cl ? cl->ctx : nullptr)
     REQUIRE(cl, "null cell list");
-    auto* ctx = cl->ctx;
+    mrmd_ctx* ctx = cl->ctx;
     const int64_t n = cl->bin.end - cl->bin.begin;
     const int ncell = cl->bin.grid.total();
     if (cellOf && n) HIPCHECK(hipMemcpyAsync(cellOf, cl->bin.cellOf.p, size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -396,7 +428,7 @@ int mrmd_atoms_permute(mrmd_atoms* a, const mrmd_cell_list* cl)
     const int64_t n = cl->bin.end - cl->bin.begin;
     REQUIRE(cl->bin.end <= a->size, "cell list covers records beyond the atoms");
     if (n == 0) return MRMD_OK;
-    auto* ctx = a->ctx;
+    mrmd_ctx* ctx = a->ctx;
     KTimer t(ctx, "atoms_permute");
     // gather into scratch buffers of the same capacity, then swap them in
     mrmd_atoms tmp;
@@ -500,7 +532,7 @@ int mrmd_verlet_half_neighbors(mrmd_verlet_list* vl, int32_t* host, int64_t n, i
     MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
     REQUIRE(vl && host && rowStride > 0, "invalid arguments");
     vl->ensureHalf();
-    auto* ctx = vl->ctx;
+    mrmd_ctx* ctx = vl->ctx;
     const int64_t rows = std::min<int64_t>(n, vl->end - vl->begin);
     if (rows <= 0) return MRMD_OK;
     int32_t* dev = nullptr;
@@ -517,7 +549,7 @@ int mrmd_verlet_full_neighbors(mrmd_verlet_list* vl, int32_t* host, int32_t* cou
 {
     MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
     REQUIRE(vl && rowStride > 0, "invalid arguments");
-    auto* ctx = vl->ctx;
+    mrmd_ctx* ctx = vl->ctx;
     const int64_t rows = std::min<int64_t>(n, vl->end - vl->begin);
     if (rows <= 0) return MRMD_OK;
     if (counts) HIPCHECK(hipMemcpyAsync(counts, vl->cnt.p, size_t(rows) * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -532,6 +564,26 @@ int mrmd_verlet_full_neighbors(mrmd_verlet_list* vl, int32_t* host, int32_t* cou
         (void)hipFree(dev);
     }
     sync(ctx);
+    MRMD_API_END
+}
+
+int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* a, double r, int64_t* half, int64_t* full)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && a && vl->nrows <= a->size, "invalid arguments");
+    mrmd_ctx* ctx = vl->ctx;
+    const int64_t rows = vl->end - vl->begin;
+    unsigned long long* d = static_cast<unsigned long long*>(ctxScratch(ctx, 2, 64));
+    HIPCHECK(hipMemsetAsync(d, 0, 16, ctx->stream));
+    if (rows > 0)
+        hipLaunchKernelGGL(k_countWithin, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a->pos.p, vl->nbr.p,
+                           vl->cnt.p, vl->begin, rows, vl->stride, r * r, d);
+    checkLaunch();
+    unsigned long long h[2];
+    HIPCHECK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    if (half) *half = int64_t(h[0]);
+    if (full) *full = int64_t(h[1]);
     MRMD_API_END
 }
 

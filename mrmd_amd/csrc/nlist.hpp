@@ -66,13 +66,13 @@ struct Binning
 
 struct mrmd_cell_list
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     mrmd_hip::Binning bin;
 };
 
 struct mrmd_verlet_list
 {
-    mrmd_ctx* ctx = nullptr;
+    mrmd_hip::CtxPtr ctx;
     int64_t nrows = 0;          ///< records of the binned slice (counts exist for all)
     int64_t begin = 0, end = 0; ///< rows = records [begin, end)
     int64_t stride = 0;         ///< ELL column stride (>= end - begin, multiple of 64)

@@ -1778,4 +1778,78 @@ int64_t or_run_md(OAtoms* at, const or_box* box, const or_md_params* prm, double
     return rebuilds;
 }
 
+// ---- stateful form of the same loop (for bounded CPU baseline samples) ----------------
+struct OMD
+{
+    OAtoms* at = nullptr;
+    or_box box{};
+    or_md_params prm{};
+    OLJ* lj = nullptr;
+    OGhost ghost;
+    OVerlet vl;
+    double maxDisp = std::numeric_limits<double>::max();
+    int64_t step = 0, rebuilds = 0;
+};
+
+OMD* or_md_new(OAtoms* at, const or_box* box, const or_md_params* prm)
+{
+    auto* md = new OMD();
+    md->at = at;
+    md->box = *box;
+    md->prm = *prm;
+    const double cap = prm->cap, rc = prm->rc, sig = prm->sigma, eps = prm->epsilon;
+    md->lj = or_lj_new(&cap, &rc, &sig, &eps, 1, prm->shifted, 1);
+    return md;
+}
+
+void or_md_free(OMD* md)
+{
+    or_lj_free(md->lj);
+    delete md;
+}
+
+// runs n steps of the loop of or_run_md; returns the rebuild count so far
+int64_t or_md_steps(OMD* md, int64_t n)
+{
+    auto* at = md->at;
+    const auto* prm = &md->prm;
+    auto s = makeSub(md->box);
+    const double rn = prm->rc + prm->skin;
+    for (int64_t k = 0; k < n; ++k, ++md->step)
+    {
+        if (prm->langevin)
+            md->maxDisp += or_langevin_pre(at, prm->dt, prm->gamma, prm->temperature, prm->seed,
+                                           uint64_t(md->step), nullptr);
+        else
+            md->maxDisp += or_vv_pre(at, prm->dt);
+        if (md->maxDisp >= prm->skin * 0.5)
+        {
+            md->maxDisp = 0.0;
+            or_map_into_domain(at, &md->box);
+            if (prm->permute)
+            {
+                const double delta[3] = {rn, rn, rn};
+                OCells* cells = or_cells_build(at, 0, at->nlocal, delta, s.min, s.max);
+                or_atoms_permute(at, cells);
+                or_cells_free(cells);
+            }
+            or_ghost_create_xyz(&md->ghost, at, &md->box);
+            verletBuild(md->vl, at->a, at->nlocal + at->nghost, 0, at->nlocal, rn, prm->cell_ratio, s.minGhost,
+                        s.maxGhost, prm->max_neigh, true);
+            ++md->rebuilds;
+        }
+        else
+            or_ghost_update(&md->ghost, at, &md->box);
+#pragma omp parallel for
+        for (int64_t i = 0; i < at->nlocal + at->nghost; ++i)
+            for (int d = 0; d < 3; ++d) at->a[size_t(i)].force[d] = 0.0;
+        or_lj_apply(md->lj, at, &md->vl, nullptr);
+        or_ghost_fold(&md->ghost, at);
+        or_vv_post(at, prm->dt);
+    }
+    return md->rebuilds;
+}
+
+double or_md_energy(const OMD* md) { return md->lj->energy; }
+
 }  // extern "C"

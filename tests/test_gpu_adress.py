@@ -1,0 +1,220 @@
+"""GPU parity of the AdResS path (slab weighting, molecule update, multi-resolution ghosts,
+LJ_IdealGas, molecule force distribution, thermodynamic force) against the CPU oracle.
+
+Index work (ghost order, corresponding-atom map, neighbour sets) is bit-exact.  Sums that
+both sides accumulate with atomics in different orders (forces, energies, compensation
+energies) agree to 1e-12 of their largest magnitude; counts are exact.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from mrmd_amd.lattice import uniform
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mrmd_amd
+    return mrmd_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(M):
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def _close(a, b, rel=1e-12):
+    scale = max(np.abs(b).max(), 1e-300)
+    assert np.abs(a - b).max() <= rel * scale, (np.abs(a - b).max(), scale)
+
+
+# LJ_IdealGas.test.cpp:34-220
+@pytest.mark.parametrize("lam,scale", [(1.0, 1.0), (0.5, 0.5), (0.0, 0.0)])
+def test_lj_ideal_gas_four_atoms(M, ctx, lam, scale):
+    mols = M.Molecules(ctx, 2)
+    mols.upload(M.M_POS, [[-0.5, 0, 0], [0.5, 0, 0]])
+    mols.upload(M.M_OFFSET, [0, 2])
+    mols.upload(M.M_NUMATOMS, [2, 2])
+    mols.upload(M.M_MODLAMBDA, [lam, lam])
+    mols.set_counts(2, 0)
+    pos = np.array([[-0.5, -0.5, 0], [-0.5, 0.5, 0], [0.5, -0.5, 0], [0.5, 0.5, 0]])
+    f0 = 2.0 if lam == 0.0 else 0.0
+    atoms = M.Atoms.from_arrays(ctx, pos, relmass=np.full(4, 0.5), force=np.full((4, 3), f0))
+    vl = M.HalfVerletList(ctx)
+    vl.build(mols, 0, 2, 2.0, 1.0, [-1, -1, -1], [1, 1, 1], 4, molecules=True)
+    lj = M.LJ_IdealGas(ctx, 0.0, 2.25, 0.9, 2.0, True)
+    lj.run(mols, vl, atoms)
+    F = atoms.download(M.FORCE)
+    if lam == 0.0:
+        assert np.all(F == 2.0)
+        return
+    fx, fy = 0.22156665 * scale, 1.3825009 * scale
+    np.testing.assert_allclose(F, [[-fx, fy, 0], [-fx, -fy, 0], [fx, fy, 0], [fx, -fy, 0]], rtol=2e-7,
+                               atol=1e-12)
+
+
+def test_slab_and_histograms(M, ctx):
+    slab = M.Slab([10, 0, 0], 4.0, 2.0, 1)
+    os_ = O.make_slab([10, 0, 0], 4.0, 2.0, 1)
+    xs = np.linspace(0, 20, 2001)
+    g = slab(ctx, xs)
+    for k, x in enumerate(xs):
+        out = np.zeros(5)
+        O.lib().or_slab_eval(O.C.byref(os_), x, O.ptr(out))
+        np.testing.assert_allclose(g[k], out, rtol=1e-14, atol=1e-15)
+    d = np.random.default_rng(0).random((23, 3))
+    for periodic in (0, 1):
+        og = np.zeros_like(d)
+        O.lib().or_hist_gradient(0.0, 23.0, 23, 3, O.ptr(d), periodic, O.ptr(og))
+        np.testing.assert_array_equal(M.hist_gradient(ctx, 0.0, 23.0, d, bool(periodic)), og)
+        os2 = np.zeros_like(d)
+        O.lib().or_hist_smoothen(0.0, 23.0, 23, 3, O.ptr(d), 2.0, 3.0, periodic, O.ptr(os2))
+        np.testing.assert_allclose(M.hist_smoothen(ctx, 0.0, 23.0, d, 2.0, 3.0, bool(periodic)), os2, rtol=1e-14)
+
+
+def _grid_fixture(M, ctx, apm=2):
+    mp, ap = [], []
+    for x in range(3):
+        for y in range(3):
+            for z in range(3):
+                mp.append((x + 0.5, y + 0.5, z + 0.5))
+                for i in range(apm):
+                    ap.append((x + 0.5 + 0.1 * i, y + 0.5 + 0.2 * i, z + 0.5 + 0.3 * i))
+    gm = M.Molecules(ctx, 27)
+    gm.upload(M.M_POS, mp)
+    gm.upload(M.M_OFFSET, np.arange(27) * apm)
+    gm.upload(M.M_NUMATOMS, np.full(27, apm))
+    gm.set_counts(27, 0)
+    ga = M.Atoms.from_arrays(ctx, np.array(ap))
+    om = O.Molecules(27)
+    om.set(O.M_POS, mp)
+    om.set(O.M_OFFSET, np.arange(27) * apm)
+    om.set(O.M_NUMATOMS, np.full(27, apm))
+    om.set_counts(27, 0)
+    oa = O.Atoms.from_arrays(np.array(ap))
+    return gm, ga, om, oa
+
+
+def test_multires_ghosts_bit_exact(M, ctx):
+    gm, ga, om, oa = _grid_fixture(M, ctx)
+    gbox = M.make_box([0, 0, 0], [3, 3, 3], 0.7)
+    obox = O.make_box([0, 0, 0], [3, 3, 3], 0.7)
+    gl = M.GhostLayer(ctx)
+    gl.multires_create_ghost_atoms(gm, ga, gbox)
+    og = O.Ghost()
+    O.lib().or_multires_ghost_xyz(og.h, om.h, oa.h, O.C.byref(obox))
+    assert gm.nghost == 5 ** 3 - 27 == om.nghost
+    assert ga.nghost == (5 ** 3 - 27) * 2 == oa.nghost
+    nm, na = 27 + gm.nghost, 54 + ga.nghost
+    np.testing.assert_array_equal(gm.download(M.M_POS, nm), om.get(O.M_POS, nm))
+    np.testing.assert_array_equal(gm.download(M.M_OFFSET, nm), om.get(O.M_OFFSET, nm))
+    np.testing.assert_array_equal(ga.download(M.POS, na), oa.get(O.POS, na))
+    np.testing.assert_array_equal(gl.corresponding(na), og.corr(na))
+
+
+def _slab_system(M, ctx, n=6000, Lx=40.0, Lyz=12.0, seed=11):
+    pos = uniform(n, [0, 0, 0], [Lx, Lyz, Lyz], seed)
+    rng = np.random.default_rng(seed)
+    types = (rng.random(n) < 0.3).astype(np.int64)
+    return pos, types
+
+
+def test_adress_pipeline_matches_oracle(M, ctx):
+    """one full AdResS force evaluation: multi-res ghosts, molecules, molecule list,
+    LJ_IdealGas (sampling run), molecule -> atom forces, ghost fold, thermodynamic force"""
+    Lx, Lyz, rn = 40.0, 12.0, 2.8
+    pos, types = _slab_system(M, ctx, Lx=Lx, Lyz=Lyz)
+    n = pos.shape[0]
+    gbox = M.make_box([0, 0, 0], [Lx, Lyz, Lyz], rn)
+    obox = O.make_box([0, 0, 0], [Lx, Lyz, Lyz], rn)
+    ga = M.Atoms.from_arrays(ctx, pos, type=types)
+    oa = O.Atoms.from_arrays(pos, type=types)
+    gm = M.Molecules.for_each_atom(ctx, ga)
+    om = O.Molecules.for_each_atom(oa)
+    slab = M.Slab([Lx / 2, 0, 0], 8.0, 5.0, 1)
+    os_ = O.make_slab([Lx / 2, 0, 0], 8.0, 5.0, 1)
+    gm.update(ga, None)
+    O.lib().or_update_molecules(om.h, oa.h, None)
+    gl = M.GhostLayer(ctx)
+    og = O.Ghost()
+    gl.multires_exchange_real_atoms(gm, ga, gbox)
+    O.lib().or_multires_exchange_real(O.C.byref(obox), om.h, oa.h)
+    gl.multires_create_ghost_atoms(gm, ga, gbox)
+    O.lib().or_multires_ghost_xyz(og.h, om.h, oa.h, O.C.byref(obox))
+    nm, na = gm.nlocal + gm.nghost, ga.nlocal + ga.nghost
+    assert (nm, na) == (om.nlocal + om.nghost, oa.nlocal + oa.nghost)
+    gm.update(ga, slab)
+    O.lib().or_update_molecules(om.h, oa.h, O.C.byref(os_))
+    for f in (M.M_POS,):
+        np.testing.assert_array_equal(gm.download(f, nm), om.get(f, nm))
+    _close(gm.download(M.M_MODLAMBDA, nm), om.get(O.M_MODLAMBDA, nm), 1e-14)
+    vl = M.HalfVerletList(ctx)
+    lo, hi = [-rn] * 3, [Lx + rn, Lyz + rn, Lyz + rn]
+    vl.build(gm, 0, gm.nlocal, rn, 1.0, lo, hi, molecules=True)
+    ov = O.Verlet()
+    ov.build(om, nm, 0, om.nlocal, rn, 1.0, lo, hi, molecules=True)
+    assert vl.num_pairs() == ov.total()
+    cap, rc, sig, eps = [0.7] * 4, [2.5] * 4, [1.0, 0.95, 0.95, 0.9], [1.0, 0.8, 0.8, 0.6]
+    glj = M.LJ_IdealGas(ctx, cap, rc, sig, eps, True, num_types=2)
+    olj = O.LJIdealGas(cap, rc, sig, eps, 2, shifted=True)
+    ga.set_force(0.0)
+    oa.set(O.FORCE, np.zeros((na, 3)))
+    gm.set_force(0.0)
+    O.lib().or_mol_set_force(om.h, 0.0)
+    O.lib().or_set_threads(4)
+    Eg = glj.run(gm, vl, ga)
+    Ec = olj.run(om, ov, oa)
+    assert Eg == pytest.approx(Ec, rel=1e-12)
+    _close(ga.download(M.FORCE, na), oa.get(O.FORCE, na))
+    _close(gm.download(M.M_FORCE, nm), om.get(O.M_FORCE, nm))
+    for which in (0, 1, 2):
+        _close(glj.histogram(which), olj.hist(which), 1e-11)
+    # molecule forces to atoms, ghost fold
+    gm.contribute_force(ga)
+    O.lib().or_contribute_molecule_force(om.h, oa.h)
+    gl.contribute_back_ghost_to_real(ga)
+    og.fold(oa)
+    _close(ga.download(M.FORCE, ga.nlocal), oa.get(O.FORCE, oa.nlocal))
+    # thermodynamic force: sample, update (symmetric, periodic), apply interpolated
+    gtf = M.ThermodynamicForce(ctx, [0.5, 0.5], gbox, 1.0, [1.0, 1.0], True, True)
+    otf = O.ThermodynamicForce([0.5, 0.5], obox, 1.0, [1.0, 1.0], sym=True, periodic=True)
+    for _ in range(3):
+        gtf.sample(ga)
+        otf.sample(oa)
+    np.testing.assert_array_equal(gtf.get_density_profile(), otf.density())
+    hy = dict(kind=M.PRED_SLAB_HY, axis=0, a=Lx / 2, b=4.0, c=5.0)
+    gtf.update(2.0, 3.0, M.pred1(**hy))
+    otf.update(2.0, 3.0, O.pred1(**hy))
+    _close(gtf.get_force(), otf.force(), 1e-13)
+    gtf.apply_interpolated(ga)
+    otf.apply_interpolated(oa)
+    _close(ga.download(M.FORCE, ga.nlocal), oa.get(O.FORCE, oa.nlocal))
+    gl_, gr_ = gtf.mu()
+    ol_, or_ = otf.mu()
+    np.testing.assert_allclose(gl_, ol_, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(gr_, or_, rtol=1e-12, atol=1e-14)
+
+
+def test_thermodynamic_force_reference_cases(M, ctx):
+    # ThermodynamicForce.test.cpp:96-245 on the device
+    box = M.make_box([0, 0, 0], [10, 1, 1], 1.0)
+    tf = M.ThermodynamicForce(ctx, [1.0], box, 1.0, [1.0])
+    tf.set_force(np.arange(10, dtype=float).reshape(-1, 1))
+    pts = np.array([[i + 0.5, 0, 0] for i in range(10) for _ in range(i)])
+    a = M.Atoms.from_arrays(ctx, pts)
+    tf.sample(a)
+    np.testing.assert_array_equal(tf.get_density_profile()[:, 0], np.arange(10.0))
+    tf.update(1.0, 0.0)
+    np.testing.assert_allclose(tf.get_force()[:, 0], np.arange(10.0) - 1.0)
+    with pytest.raises(M.MrmdError):
+        tf.update(1.0, 0.0)
+    u = M.Atoms.from_arrays(ctx, np.array([[i / 10.0, 0, 0] for i in range(100)]))
+    tf.set_force(np.arange(10, dtype=float).reshape(-1, 1))
+    tf.apply_interpolated(u, M.pred1(M.PRED_OPEN_RANGE, 0, 0.5, 4.5))
+    x = u.download(M.POS)[:, 0]
+    inside = (x > 0.5) & (x < 4.5)
+    np.testing.assert_allclose(u.download(M.FORCE)[:, 0], np.where(inside, x - 0.5, 0.0), atol=1e-12)
