@@ -14,16 +14,16 @@ from . import (FORCE, POS, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones,
 class LJSystem:
     def __init__(self, ctx, pos, vel, L, rc=2.5, skin=0.3, sigma=1.0, epsilon=1.0, cap=0.0, shifted=True,
                  cell_ratio=1.0, max_neigh=0, langevin=False, gamma=1.0, temperature=1.0, seed=1234,
-                 permute=False):
+                 permute=False, types=None, num_types=1, type_stride=1):
         self.ctx = ctx
         self.L = L
         self.rc, self.skin = rc, skin
         self.rn = rc + skin
         self.box = make_box([0, 0, 0], [L] * 3, self.rn)
-        self.atoms = Atoms.from_arrays(ctx, pos, vel=vel)
+        self.atoms = Atoms.from_arrays(ctx, pos, vel=vel, type=types)
         self.ghosts = GhostLayer(ctx)
         self.vl = HalfVerletList(ctx)
-        self.lj = LennardJones(ctx, rc, sigma, epsilon, cap, 1, shifted)
+        self.lj = LennardJones(ctx, rc, sigma, epsilon, cap, num_types, shifted, type_stride)
         self.cell_ratio, self.max_neigh = cell_ratio, max_neigh
         self.langevin = VelocityVerletLangevinThermostat(gamma, temperature, seed) if langevin else None
         self.permute = permute

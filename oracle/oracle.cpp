@@ -1778,6 +1778,89 @@ int64_t or_run_md(OAtoms* at, const or_box* box, const or_md_params* prm, double
     return rebuilds;
 }
 
+// ---- adjoint of the LJ force map (new code, no reference counterpart) ------------------
+// L = sum_i g_i . F_i.  Each half-list pair (i, j) with j' the real atom of j contributes
+//   dL/dx_i += J u,  dL/dx_j' -= J u,  u = g_i - g_j',  J = s I + 2 s'(r^2) d d^T
+//   dL/deps_t += u.d s/eps,  dL/dsigma_t += u.d ds/dsigma  (t = type_i*stride + type_j)
+// and the parameter gradients are finally symmetrised over (a,b) and (b,a).
+void or_lj_adjoint(const OLJ* lj,
+                   const OAtoms* at,
+                   const OVerlet* v,
+                   const OGhost* g,
+                   const double* dLdF,
+                   double* dLdx,
+                   double* dLdeps,
+                   double* dLdsigma)
+{
+    const auto& a = at->a;
+    const int64_t nl = at->nlocal;
+    const int64_t nt2 = int64_t(lj->table.size());
+    std::vector<double> ge(size_t(nt2), 0.0), gs(size_t(nt2), 0.0);
+    for (int64_t k = 0; k < 3 * nl; ++k) dLdx[k] = 0.0;
+    auto realOf = [&](int64_t j) -> int64_t
+    {
+        if (j < nl) return j;
+        return g->corr[size_t(j)];
+    };
+    for (int64_t i = 0; i < nl; ++i)
+    {
+        const int nn = v->counts[size_t(i)];
+        for (int n = 0; n < nn; ++n)
+        {
+            const int64_t j = v->neigh[size_t(i * v->maxN + n)];
+            const int64_t jr = realOf(j);
+            const double dx[3] = {a[size_t(i)].pos[0] - a[size_t(j)].pos[0], a[size_t(i)].pos[1] - a[size_t(j)].pos[1],
+                                  a[size_t(i)].pos[2] - a[size_t(j)].pos[2]};
+            const double d2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (d2 > lj->rcSqr) continue;
+            const int64_t t = a[size_t(i)].type * lj->typeStride + a[size_t(j)].type;
+            const Pre& p = lj->table[size_t(t)];
+            const double sigma = std::pow(p.ef1 / p.ef2, 1.0 / 6.0);
+            const double eps = p.ef2 * p.ef2 / (4.0 * p.ef1);
+            double u[3], du = 0.0;
+            for (int d = 0; d < 3; ++d)
+            {
+                u[d] = dLdF[3 * i + d] - dLdF[3 * jr + d];
+                du += dx[d] * u[d];
+            }
+            double s, sp, dsds;
+            if (d2 >= p.capDistSqr)
+            {
+                const double r2i = 1.0 / d2, r6i = r2i * r2i * r2i;
+                s = r6i * (p.ff1 * r6i - p.ff2) * r2i;
+                sp = -(7.0 * p.ff1 * r6i * r6i * r2i * r2i - 4.0 * p.ff2 * r6i * r2i * r2i);
+                dsds = r6i * (12.0 * p.ff1 * r6i - 6.0 * p.ff2) * r2i / sigma;
+            }
+            else
+            {
+                const double r = std::sqrt(d2);
+                s = p.capCoeff / r;
+                sp = -0.5 * p.capCoeff / (r * d2);
+                const double c2i = 1.0 / p.capDistSqr, c6i = c2i * c2i * c2i;
+                dsds = (c6i * (12.0 * p.ff1 * c6i - 6.0 * p.ff2) * c2i / sigma) * p.capDist / r;
+            }
+            for (int d = 0; d < 3; ++d)
+            {
+                const double vd = s * u[d] + 2.0 * sp * du * dx[d];
+                dLdx[3 * i + d] += vd;
+                dLdx[3 * jr + d] -= vd;
+            }
+            ge[size_t(t)] += du * s / eps;
+            gs[size_t(t)] += du * dsds;
+        }
+    }
+    const int64_t n = lj->numTypes, st = lj->typeStride;
+    for (int64_t k = 0; k < nt2; ++k) dLdeps[k] = dLdsigma[k] = 0.0;
+    for (int64_t x = 0; x < n; ++x)
+        for (int64_t y = 0; y < n; ++y)
+        {
+            const int64_t i1 = x * st + y, i2 = y * st + x;
+            if (i1 >= nt2 || i2 >= nt2) continue;
+            dLdeps[i1] = i1 == i2 ? ge[size_t(i1)] : ge[size_t(i1)] + ge[size_t(i2)];
+            dLdsigma[i1] = i1 == i2 ? gs[size_t(i1)] : gs[size_t(i1)] + gs[size_t(i2)];
+        }
+}
+
 // ---- stateful form of the same loop (for bounded CPU baseline samples) ----------------
 struct OMD
 {

@@ -175,6 +175,7 @@ def lib():
         "or_hist_get_bin": (I64, [D, D, I64, D]),
         "or_hist_bin_position": (D, [D, D, I64, I64]),
         "or_run_md": (I64, [P, P, P, P]),
+        "or_lj_adjoint": (None, [P, P, P, P, P, P, P, P]),
         "or_md_new": (P, [P, P, P]),
         "or_md_free": (None, [P]),
         "or_md_steps": (I64, [P, I64]),
