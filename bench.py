@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -79,6 +80,15 @@ def main():
     dt = 0.005
     for _ in range(args.warmup):
         s.step(dt)
+    tiling = None
+    if not args.no_tune:
+        from mrmd_amd.autotune import tune_lj_system
+        best, _ = tune_lj_system(s)
+        tiling = {"tile_cells": best[0], "block": best[1]}
+        for _ in range(3):
+            s.step(dt)
+    else:
+        tiling = {k: v for k, v in s.vl.tiling().items() if k != "max_staged"}
     ctx.synchronize()
     ctx.set_profiling(True)
     ctx.reset_stats()
@@ -92,6 +102,8 @@ def main():
     launches, force_ms = ctx.kernel_stats("lj_force")
     stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "langevin_pre", "vv_post", "halo_create",
                                               "halo_update", "halo_fold", "atoms_permute", "cell_list_build")}
+    nrec = s.atoms.nlocal + s.atoms.nghost
+    full_entries = s.vl.info()["total_full"]
     ctx.set_profiling(False)
     half_rn, _ = s.vl.count_within(s.atoms, 2.8)
     half_rc, _ = s.vl.count_within(s.atoms, 2.5)
@@ -99,6 +111,18 @@ def main():
     avg_s = force_ms / max(launches, 1) / 1e3
     achieved_tf = flops / avg_s / 1e12
     value = n * args.steps / elapsed
+
+    def gbs(name, nbytes):
+        k, t = stats[name]
+        return {"GB/s": nbytes / (t / k / 1e3) / 1e9 if k else None, "bytes": nbytes,
+                "avg_ms": t / k if k else None, "frac_of_hbm_peak": nbytes / (t / k / 1e3) / 1e9 / HBM_PEAK_GBS
+                if k else None}
+
+    # algorithmic bytes: integrators 128 B/atom (pre) and 80 B/atom (post); list build:
+    # read 32 B + write 4 B cell index + 4 B slot + 32 B staged copy per record, 2 B per
+    # list entry and 4 B per row count
+    bandwidth = {"langevin_pre": gbs("langevin_pre", 128.0 * n), "vv_post": gbs("vv_post", 80.0 * n),
+                 "verlet_build": gbs("verlet_build", 72.0 * nrec + 2.0 * full_entries + 4.0 * nrec)}
 
     cpu = None
     if not args.no_cpu:
@@ -123,13 +147,14 @@ def main():
         "data": "synthetic FCC lattice + Maxwell-Boltzmann velocities (seeded)",
         "config": {"workload": "plain LJ NVT (Langevin), FP64, FCC 100^3 x 4", "atoms": n,
                    "density": 0.8442, "rc": 2.5, "skin": 0.3, "dt": dt, "temperature": 1.0, "gamma": 1.0,
-                   "rebuilds_in_timed_steps": rebuilds},
+                   "rebuilds_in_timed_steps": rebuilds, "tiling": tiling},
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_ljFull (LJ force)", "avg_ms": avg_s * 1e3, "launches": launches,
+                     "kernel": "k_ljTiled (LJ force)", "avg_ms": avg_s * 1e3, "launches": launches,
                      "half_pairs_rn": half_rn, "half_pairs_rc": half_rc, "flop_per_launch": flops},
         "cpu_baseline": cpu,
         "kernel_ms": {k: {"launches": v[0], "total_ms": v[1]} for k, v in stats.items()},
+        "bandwidth": bandwidth,
     }
     print(json.dumps(out))
 

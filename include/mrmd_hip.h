@@ -230,7 +230,8 @@ int mrmd_verlet_destroy(mrmd_verlet_list* vl);
  * maxNeigh) (examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:156-163): binning of every
  * record (local and ghost), 27-cell stencil with Cabana's min-distance prune, d^2 <= r^2.
  * The half list (Cabana's position-lexicographic discriminator) is exact; the device keeps
- * both directions so force kernels can run without atomics. */
+ * both directions so force kernels can run without atomics, in tiles of cells whose
+ * neighbourhood positions the force kernel stages in LDS. */
 int mrmd_verlet_build(mrmd_verlet_list* vl, const mrmd_atoms* atoms, int64_t begin, int64_t end,
                       double r_n, double cell_ratio, const double* grid_min, const double* grid_max,
                       int64_t max_neigh);
@@ -253,6 +254,13 @@ int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* max_ful
  * no reference counterpart) */
 int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* atoms, double r, int64_t* half_pairs,
                              int64_t* full_entries);
+/* Launch shape of the tiled list and the tiled force kernel (no reference counterpart; the
+ * reference's Cabana list has no tiling): tile_cells = cells per tile along z (0 = chosen
+ * from the density at the next build), block = workgroup size (64, 128, 192 or 256).
+ * Applies from the next build. */
+int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int block);
+/* the tiling in use: cells per tile, workgroup size, largest staged neighbourhood (records) */
+int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* block, int* max_staged);
 
 /* =========================================================================================
  * Lennard-Jones  (action::LennardJones, mrmd/action/LennardJones.hpp:98-206)

@@ -62,8 +62,11 @@ class Context:
     def set_stream(self, stream_handle):
         self.check(_L.mrmd_ctx_set_stream(self.h, C.c_void_p(stream_handle)))
 
+    profiling = False
+
     def set_profiling(self, on=True):
         self.check(_L.mrmd_ctx_set_profiling(self.h, 1 if on else 0))
+        self.profiling = bool(on)
 
     def kernel_stats(self, name):
         n = C.c_int64()
@@ -358,6 +361,16 @@ class HalfVerletList:
         dims = np.zeros(3, np.int32)
         self.ctx.check(_L.mrmd_verlet_info(self.h, C.byref(rows), C.byref(mx), C.byref(tot), _p(dims)))
         return dict(rows=rows.value, max_full=mx.value, total_full=tot.value, dims=dims)
+
+    def set_tiling(self, tile_cells=0, block=128):
+        """cells per tile along z (0 = from the density) and workgroup size of the tiled
+        list build and force kernel; applies from the next build"""
+        self.ctx.check(_L.mrmd_verlet_set_tiling(self.h, int(tile_cells), int(block)))
+
+    def tiling(self):
+        k, b, s = C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(_L.mrmd_verlet_get_tiling(self.h, C.byref(k), C.byref(b), C.byref(s)))
+        return dict(tile_cells=k.value, block=b.value, max_staged=s.value)
 
     def count_within(self, atoms, r):
         """(half-list pairs, both-direction entries) closer than r at the current positions"""

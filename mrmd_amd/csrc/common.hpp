@@ -329,8 +329,8 @@ __device__ inline double waveMax(double v)
     for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
     return v;
 }
-// block of BLOCK threads; result valid in thread 0
-__device__ inline double blockSum(double v, double* lds /*[BLOCK/WAVE]*/)
+// result valid in thread 0; lds holds blockDim.x / WAVE doubles (blockDim.x a multiple of WAVE, at most BLOCK)
+__device__ inline double blockSum(double v, double* lds)
 {
     v = waveSum(v);
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
@@ -338,7 +338,7 @@ __device__ inline double blockSum(double v, double* lds /*[BLOCK/WAVE]*/)
     __syncthreads();
     double r = 0.0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < BLOCK / WAVE; ++i) r += lds[i];
+        for (int i = 0; i < int(blockDim.x) / WAVE; ++i) r += lds[i];
     __syncthreads();
     return r;
 }
@@ -350,7 +350,7 @@ __device__ inline double blockMax(double v, double* lds)
     __syncthreads();
     double r = 0.0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < BLOCK / WAVE; ++i) r = fmax(r, lds[i]);
+        for (int i = 0; i < int(blockDim.x) / WAVE; ++i) r = fmax(r, lds[i]);
     __syncthreads();
     return r;
 }

@@ -3,9 +3,10 @@
 // Force kernel design (gfx950, FP64): one lane per local atom over its full neighbour row
 // (both directions of every pair), so F_i is owned by one lane and no atomics are needed;
 // the pair work is done twice, which on a chip with 78.6 TFLOP/s FP64 costs less than the
-// L2 atomics of a half list.  Positions are gathered as one 32-byte {x,y,z,type} record;
-// rows are read column-major (coalesced across the wave).  Energy and virial are reduced
-// with wave shuffles, per-workgroup partials and one ordered final pass (deterministic).
+// L2 atomics of a half list.  One workgroup per Verlet-list tile stages the neighbourhood's
+// {x,y,z,type} records in LDS once; rows then read neighbours from LDS through 16-bit slot
+// entries.  Energy and virial are reduced with wave shuffles, per-workgroup partials and
+// one ordered final pass (deterministic).
 // The reference's half-list algorithm (atomic F_j, LennardJones.hpp:158-205) is kept for
 // asymmetric pair predicates and asymmetric parameter tables.
 #include <algorithm>
@@ -132,43 +133,104 @@ struct LJArgs
     int usePred;
 };
 
-template <bool PRED>
-__global__ __launch_bounds__(BLOCK) void k_ljFull(LJArgs a, const uint32_t* __restrict__ nbr,
-                                                  const int32_t* __restrict__ cnt, double* partials)
+// Tiled full-row force: one workgroup per list tile stages the current positions of the
+// tile's neighbourhood (sorted-slot order) in LDS; each row reads its neighbours from LDS
+// through the 16-bit slot entries, 8 entries per 16-byte load with the next chunk's load
+// issued before the current chunk is processed.  ONE: single-type table held in registers.
+struct TiledList
 {
-    __shared__ Pre tab[MAX_TYPES2];
-    __shared__ double red[BLOCK / WAVE];
-    for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
-    __syncthreads();
-    const unsigned blk = xcdBlock();
-    const int64_t r = int64_t(blk) * blockDim.x + threadIdx.x;
-    double e = 0.0, w = 0.0;
-    if (r < a.rows)
+    const uint16_t* nbr;
+    const int32_t* cnt;
+    int64_t strideS;
+    const TileInfo* tiles;
+    const int32_t* sorted;
+    const double4* sortedPos;
+};
+
+// Branch-free pair terms of one chunk of eight entries, written stage by stage over the
+// eight pairs so that the eight dependency chains (LDS read, distance, reciprocal, force)
+// are independent and interleave.  Padding entries (the row itself, d^2 = 0) and pairs
+// beyond rc select zero.  CAP: the table has a capping distance, so the capped branch
+// (LennardJones.hpp:68-75) is evaluated too.
+template <bool PRED, bool ONE, bool CAP>
+__device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, int ti0, const double4& pi,
+                               const double4* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
+                               double& w)
+{
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    double dx[CHUNK], dy[CHUNK], dz[CHUNK], d2[CHUNK], ty[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
     {
-        const int64_t i = a.begin + r;
-        const double4 pi = a.pos[i];
-        const int ti = int(pi.w) * a.typeStride;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        const int c = cnt[r];
-        const uint32_t* col = nbr + r;
-        for (int k = 0; k < c; ++k)
+        const double4 pj = st[(wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu];
+        dx[u] = pi.x - pj.x;
+        dy[u] = pi.y - pj.y;
+        dz[u] = pi.z - pj.z;
+        ty[u] = pj.w;
+        if (PRED && !evalPred2(a.pred, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z)) ty[u] = -1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+    double frac2[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) frac2[u] = 1.0 / d2[u];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const Pre& p = ONE ? p1 : tab[ti0 + max(int(ty[u]), 0)];
+        const double frac6 = frac2[u] * frac2[u] * frac2[u];
+        double ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+        double en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+        if (CAP && d2[u] < p.capDistSqr)
         {
-            const uint32_t j = col[int64_t(k) * a.stride] & 0x7fffffffu;
-            const double4 pj = a.pos[j];
-            if (PRED && !evalPred2(a.pred, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z)) continue;
-            const double dx = pi.x - pj.x;
-            const double dy = pi.y - pj.y;
-            const double dz = pi.z - pj.z;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 > a.rcSqr) continue;
-            const Pre& p = tab[ti + int(pj.w)];
-            double ff, en;
-            forceEnergy(p, d2, ff, en);
-            fx += dx * ff;
-            fy += dy * ff;
-            fz += dz * ff;
-            e += en;
-            w += ff * d2;
+            const double dist = sqrt(d2[u]);
+            ff = p.capCoeff / dist;
+            en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
+        }
+        bool in = d2[u] <= a.rcSqr && d2[u] > 0.0;
+        if (PRED) in = in && ty[u] >= 0.0;
+        ff = in ? ff : 0.0;
+        en = in ? en : 0.0;
+        fx += dx[u] * ff;
+        fy += dy[u] * ff;
+        fz += dz[u] * ff;
+        e += en;
+        w += ff * d2[u];
+    }
+}
+
+template <bool PRED, bool ONE, bool CAP>
+__global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p1, double* partials)
+{
+    extern __shared__ double4 st[];
+    __shared__ Pre tab[ONE ? 1 : MAX_TYPES2];
+    __shared__ TileInfo ti;
+    __shared__ double red[BLOCK / WAVE];
+    const unsigned blk = xcdBlock();
+    if (!ONE)
+        for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
+    if (threadIdx.x == 0) ti = tl.tiles[blk];
+    __syncthreads();
+    stageTile(ti, tl.sortedPos, st);
+    __syncthreads();
+    double e = 0.0, w = 0.0;
+    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    {
+        const int c = tl.cnt[s];
+        const int32_t i = tl.sorted[s];
+        if (c == 0 || i >= a.rows) continue;
+        const double4 pi = tl.sortedPos[s];
+        const int ti0 = ONE ? 0 : int(pi.w) * a.typeStride;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const uint4* chunk = reinterpret_cast<const uint4*>(tl.nbr) + s;
+        const int nch = (c + CHUNK - 1) / CHUNK;
+        uint4 cur = chunk[0];
+        for (int ch = 0; ch < nch; ++ch)
+        {
+            uint4 nxt = cur;
+            if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * tl.strideS];
+            ljChunk<PRED, ONE, CAP>(a, tab, p1, ti0, pi, st, cur, fx, fy, fz, e, w);
+            cur = nxt;
         }
         a.force[i] += fx;
         a.force[a.cap + i] += fy;
@@ -515,26 +577,44 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
     // the reference loops over local atoms with the list rows (LennardJones.hpp:146)
     const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
     mrmd_ctx* ctx = lj->ctx;
-    const unsigned grid = gridFor(rows);
-    double* partials = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid) * 2 * sizeof(double)));
-    LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, vl->stride, lj->dtable.p, int(lj->table.size()),
-             int(lj->typeStride), lj->rcSqr, pred ? *pred : mrmd_pred2{}, pred && pred->op != MRMD_PAIR_ALWAYS};
     const bool full = lj->symmetric && predIsSymmetric(pred);
+    const unsigned grid = full ? unsigned(std::max<int64_t>(vl->ntiles, 1)) : gridFor(rows);
+    double* partials = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid) * 2 * sizeof(double)));
+    LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
+             int(lj->typeStride), lj->rcSqr, pred ? *pred : mrmd_pred2{}, pred && pred->op != MRMD_PAIR_ALWAYS};
     if (rows > 0)
     {
         if (full)
         {
+            REQUIRE(vl->ntiles > 0, "the Verlet list has not been built");
+            vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
-            if (a.usePred)
-                hipLaunchKernelGGL(k_ljFull<true>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p, vl->cnt.p,
-                                   partials);
-            else
-                hipLaunchKernelGGL(k_ljFull<false>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p, vl->cnt.p,
-                                   partials);
+            const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
+                               vl->bin.sortedPos.p};
+            const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
+            const dim3 g(grid), b(vl->block);
+            const Pre p1 = lj->table[0];
+            const bool one = lj->table.size() == 1;
+            bool cap = false;
+            for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
+            auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, partials); };
+            const int variant = (a.usePred ? 4 : 0) | (one ? 2 : 0) | (cap ? 1 : 0);
+            switch (variant)
+            {
+                case 0: launch(k_ljTiled<false, false, false>); break;
+                case 1: launch(k_ljTiled<false, false, true>); break;
+                case 2: launch(k_ljTiled<false, true, false>); break;
+                case 3: launch(k_ljTiled<false, true, true>); break;
+                case 4: launch(k_ljTiled<true, false, false>); break;
+                case 5: launch(k_ljTiled<true, false, true>); break;
+                case 6: launch(k_ljTiled<true, true, false>); break;
+                default: launch(k_ljTiled<true, true, true>); break;
+            }
         }
         else
         {
             vl->ensureHalf();
+            a.stride = vl->stride;
             KTimer t(ctx, "lj_force_half");
             if (a.usePred)
                 hipLaunchKernelGGL(k_ljHalf<true>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->hnbr.p, vl->hcnt.p,
@@ -593,6 +673,7 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     REQUIRE(vl->nrows <= nall, "the list was built for more records than the atoms hold");
     REQUIRE(at->nghost == 0 || mrmd_hip::ghostCorrSize(gl) >= nall, "the ghost layer does not cover these atoms");
     const int64_t rows = std::min<int64_t>(nl, vl->end);
+    vl->ensureGlobal();
     const int nt2 = int(lj->table.size());
     // recover sigma / epsilon of each pair from the table: ef2 = 4 eps s^6, ef1 = 4 eps s^12
     std::vector<double> sig(static_cast<size_t>(nt2)), ep(static_cast<size_t>(nt2));

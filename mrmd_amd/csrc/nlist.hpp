@@ -70,16 +70,83 @@ struct mrmd_cell_list
     mrmd_hip::Binning bin;
 };
 
+namespace mrmd_hip
+{
+constexpr int MAX_RANGES = 25;  // (2R+1)^2 stencil columns for R <= 2
+constexpr int STAGE_MAX = 2048; // staged records per tile (double4 -> 64 KB of LDS)
+
+// One tile = a run of K cells along z in one (i, j) column.  Its rows are the contiguous
+// sorted slots [rowBegin, rowEnd); its neighbourhood is the (2R+1)^2 columns around it,
+// each a contiguous range of sorted slots, staged back to back in LDS at base[r].
+struct TileInfo
+{
+    int32_t rowBegin, rowEnd, nr, staged;
+    int32_t begin[MAX_RANGES];
+    int32_t base[MAX_RANGES];
+};
+
+// Neighbour entries are 16-bit LDS slots stored in chunks of 8 per row: entry k of sorted
+// slot s sits at ((k / 8) * strideS + s) * 8 + k % 8, so one 16-byte load per lane brings
+// 8 entries and a wave's load is 1 KB contiguous.
+constexpr int CHUNK = 8;
+__host__ __device__ inline int64_t entryIndex(int64_t k, int64_t s, int64_t strideS)
+{
+    return ((k >> 3) * strideS + s) * CHUNK + (k & 7);
+}
+
+// copy the tile's neighbourhood (contiguous runs of sorted slots) into LDS; the copy is
+// unrolled so that four independent loads per lane are in flight
+__device__ inline void stageTile(const TileInfo& ti, const double4* __restrict__ sortedPos, double4* st)
+{
+    int r = 0;
+    const int step = blockDim.x;
+    auto source = [&](int q) -> int {
+        if (q >= ti.staged) return -1;
+        while (r + 1 < ti.nr && ti.base[r + 1] <= q) ++r;
+        return ti.begin[r] + (q - ti.base[r]);
+    };
+    for (int q0 = threadIdx.x; q0 < ti.staged; q0 += 4 * step)
+    {
+        const int i0 = source(q0), i1 = source(q0 + step), i2 = source(q0 + 2 * step), i3 = source(q0 + 3 * step);
+        double4 v0, v1, v2, v3;
+        v0 = sortedPos[i0];
+        if (i1 >= 0) v1 = sortedPos[i1];
+        if (i2 >= 0) v2 = sortedPos[i2];
+        if (i3 >= 0) v3 = sortedPos[i3];
+        st[q0] = v0;
+        if (i1 >= 0) st[q0 + step] = v1;
+        if (i2 >= 0) st[q0 + 2 * step] = v2;
+        if (i3 >= 0) st[q0 + 3 * step] = v3;
+    }
+}
+}  // namespace mrmd_hip
+
 struct mrmd_verlet_list
 {
     mrmd_hip::CtxPtr ctx;
     int64_t nrows = 0;          ///< records of the binned slice (counts exist for all)
     int64_t begin = 0, end = 0; ///< rows = records [begin, end)
-    int64_t stride = 0;         ///< ELL column stride (>= end - begin, multiple of 64)
     double rn = 0, ratio = 1;
     int range = 1;
     int32_t capFull = 0;
     int32_t maxFull = 0;
+
+    // ---- tiled form (built on the device; what the force kernels read) ----
+    int tileK = 0;              ///< cells per tile along z (0 = choose from the density)
+    int block = 128;            ///< workgroup size of the tiled kernels
+    int tilesPerCol = 0;
+    int64_t ntiles = 0;
+    int32_t maxStaged = 0;
+    int64_t strideS = 0;        ///< ELL stride over sorted slots (>= nrows, multiple of 64)
+    mrmd_hip::DBuf<mrmd_hip::TileInfo> tiles;
+    /// entries per sorted slot s: nbr16[entryIndex(k, s, strideS)] = LDS slot (15 bits) |
+    /// half flag (bit 15)
+    mrmd_hip::DBuf<uint16_t> nbr16;
+    mrmd_hip::DBuf<int32_t> cnt16;  ///< count per sorted slot (0 for non-row records)
+
+    // ---- global-index form (rows = records begin+r), converted on demand ----
+    bool globalValid = false;
+    int64_t stride = 0;         ///< ELL column stride (>= end - begin, multiple of 64)
     int64_t totalFull = 0;
     /// full rows, column-major nbr[k * stride + r]; bit 31 marks the entries that belong
     /// to the half list (Cabana discriminator evaluated at build time)
@@ -91,6 +158,10 @@ struct mrmd_verlet_list
     mrmd_hip::DBuf<int32_t> hcnt;
     mrmd_hip::Binning bin;
 
+    /// copy the current positions into sorted-slot order (bin.sortedPos) for the tiled kernels
+    void refreshSorted(const double4* pos);
+    /// materialise the global-index full rows from the tiled list (no-op when current)
+    void ensureGlobal();
     /// materialise the half list from the flagged full rows (no-op when current)
     void ensureHalf();
 };

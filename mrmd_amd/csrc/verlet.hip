@@ -1,0 +1,578 @@
+// Verlet list build (Cabana VerletList<HalfNeighborTag, VerletLayout2D> semantics) in a
+// tiled, LDS-staged form.
+//
+// All records (local and ghost) are binned on Cabana's grid and sorted by cell.  A tile
+// is a run of K cells along z in one (i, j) column; its rows are a contiguous range of
+// sorted slots, and every cell of its 27-cell stencils lies in the (2R+1)^2 columns around
+// it, each again a contiguous slot range.  One workgroup per tile copies those ranges into
+// LDS once and every row scans its stencil cells (Cabana's i, j, k order and min-distance
+// prune) from LDS.  An entry is the 15-bit LDS slot of the neighbour plus a flag bit for
+// the pairs Cabana's position-lexicographic half discriminator keeps, so the exact half
+// list is a sub-list and the force kernels stage the same tile and never gather from
+// global memory in their inner loop.  Global-index rows (for the API, the adjoint and the
+// half-list kernels) are converted from this form on demand.
+#include <algorithm>
+#include <cmath>
+
+#include "nlist.hpp"
+
+using namespace mrmd_hip;
+
+namespace
+{
+struct TileGeom
+{
+    int n0, n1, n2, K, tilesPerCol, R;
+};
+
+__device__ inline void tileCoords(const TileGeom& tg, int64_t t, int& i, int& j, int& k0, int& k1)
+{
+    const int col = int(t / tg.tilesPerCol);
+    const int kb = int(t % tg.tilesPerCol);
+    i = col / tg.n1;
+    j = col % tg.n1;
+    k0 = kb * tg.K;
+    k1 = min(k0 + tg.K, tg.n2);
+}
+
+// one thread per tile: row range, stencil column ranges and their LDS bases
+__global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets, const int32_t* counts,
+                            TileInfo* tiles, int32_t* maxStaged)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int i, j, k0, k1;
+    tileCoords(tg, t, i, j, k0, k1);
+    TileInfo& ti = tiles[t];
+    const int cFirst = (i * tg.n1 + j) * tg.n2 + k0;
+    const int cLast = (i * tg.n1 + j) * tg.n2 + (k1 - 1);
+    ti.rowBegin = offsets[cFirst];
+    ti.rowEnd = offsets[cLast] + counts[cLast];
+    const int kl = max(k0 - tg.R, 0), kh = min(k1 + tg.R, tg.n2);
+    int nr = 0, staged = 0;
+    for (int ii = max(i - tg.R, 0); ii < min(i + tg.R + 1, tg.n0); ++ii)
+        for (int jj = max(j - tg.R, 0); jj < min(j + tg.R + 1, tg.n1); ++jj)
+        {
+            const int c0 = (ii * tg.n1 + jj) * tg.n2 + kl;
+            const int c1 = (ii * tg.n1 + jj) * tg.n2 + (kh - 1);
+            ti.begin[nr] = offsets[c0];
+            ti.base[nr] = staged;
+            staged += offsets[c1] + counts[c1] - offsets[c0];
+            ++nr;
+        }
+    ti.nr = nr;
+    ti.staged = staged;
+    atomicMax(maxStaged, staged);
+}
+
+// LDS slot of sorted slot s that lies in column range r of the tile
+__device__ inline int ldsSlot(const TileInfo& ti, int r, int s) { return ti.base[r] + (s - ti.begin[r]); }
+
+constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
+
+template <int R>
+__global__ void k_verletTiled(const double4* __restrict__ sortedPos, const int32_t* __restrict__ sorted,
+                              const int32_t* __restrict__ cellOf, const int32_t* __restrict__ offsets,
+                              const TileInfo* __restrict__ tiles, TileGeom tg, GridD g, int64_t rowLo,
+                              int64_t rowHi, double rsq, uint16_t* nbr, int64_t strideS, int cap, int32_t* cnt,
+                              int32_t* maxCnt)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileInfo ti;
+    // LDS slot of the first record of cell kl + x of range r: cstart[r * (MAX_SPAN + 1) + x];
+    // entry x = nk closes the range
+    __shared__ int cstart[MAX_RANGES * (MAX_SPAN + 1)];
+    const int64_t t = blockIdx.x;
+    if (threadIdx.x == 0) ti = tiles[t];
+    __syncthreads();
+    if (ti.rowEnd <= ti.rowBegin) return;
+    int i, j, k0, k1;
+    tileCoords(tg, t, i, j, k0, k1);
+    const int iLo = max(i - R, 0), jLo = max(j - R, 0), jCount = min(j + R + 1, tg.n1) - jLo;
+    const int kl = max(k0 - R, 0), nk = min(k1 + R, tg.n2) - kl;
+    stageTile(ti, sortedPos, st);
+    for (int x = threadIdx.x; x < ti.nr * (nk + 1); x += blockDim.x)
+    {
+        const int r = x / (nk + 1), z = x % (nk + 1);
+        int v;
+        if (z < nk)
+        {
+            const int ii = iLo + r / jCount, jj = jLo + r % jCount;
+            v = ti.base[r] + (offsets[(ii * tg.n1 + jj) * tg.n2 + kl + z] - ti.begin[r]);
+        }
+        else
+            v = r + 1 < ti.nr ? ti.base[r + 1] : ti.staged;
+        cstart[r * (MAX_SPAN + 1) + z] = v;
+    }
+    __syncthreads();
+    // the tile's own column is the range with ii == i, jj == j
+    const int self = (i - iLo) * jCount + (j - jLo);
+    int localMax = 0;
+    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    {
+        const int32_t p = sorted[s];
+        if (p < rowLo || p >= rowHi)
+        {
+            cnt[s] = 0;
+            continue;
+        }
+        const int qSelf = ldsSlot(ti, self, s);
+        const double4 xp = st[qSelf];
+        const int ck = cellOf[p] % tg.n2;
+        int c = 0;
+        int r = 0;
+        for (int ii = iLo; ii < min(i + R + 1, tg.n0); ++ii)
+            for (int jj = jLo; jj < jLo + jCount; ++jj, ++r)
+                for (int kk = max(ck - R, 0); kk < min(ck + R + 1, tg.n2); ++kk)
+                {
+                    if (g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kk) > rsq) continue;
+                    const int* cs = cstart + r * (MAX_SPAN + 1) + (kk - kl);
+                    const int q1 = cs[1];
+                    // four candidates per pass: independent LDS reads and distances, then
+                    // the appends in ascending slot order
+                    for (int q = cs[0]; q < q1; q += 4)
+                    {
+                        double4 xn[4];
+                        bool hit[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                        {
+                            const bool valid = q + u < q1 && q + u != qSelf;
+                            xn[u] = st[valid ? q + u : qSelf];
+                            const double dx = xp.x - xn[u].x;
+                            const double dy = xp.y - xn[u].y;
+                            const double dz = xp.z - xn[u].z;
+                            hit[u] = valid && dx * dx + dy * dy + dz * dz <= rsq;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (hit[u])
+                            {
+                                const double4 n = xn[u];
+                                const bool half =
+                                    (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
+                                if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((q + u) | (half ? 0x8000 : 0));
+                                ++c;
+                            }
+                    }
+                }
+        // pad the last chunk with the row's own slot: d^2 = 0 entries that the force
+        // kernels mask, so their chunk loop needs no bounds test
+        for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
+            nbr[entryIndex(k, s, strideS)] = uint16_t(qSelf);
+        cnt[s] = c;
+        localMax = max(localMax, c);
+    }
+    if (localMax > 0) atomicMax(maxCnt, localMax);
+}
+
+// tiled rows (per sorted slot, LDS slots) -> global rows (per record begin+r, record indices)
+__global__ void k_toGlobal(const uint16_t* nbr16, const int32_t* cnt16, int64_t strideS, const TileInfo* tiles,
+                           TileGeom tg, const int32_t* sorted, const int32_t* cellOf, int64_t nslots, int64_t rowLo,
+                           int64_t rowHi, uint32_t* nbr, int32_t* cnt, int64_t stride)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int32_t p = sorted[s];
+    if (p < rowLo || p >= rowHi) return;
+    const int64_t r = p - rowLo;
+    const int cell = cellOf[p];
+    const int col = cell / tg.n2, kk = cell % tg.n2;
+    const int64_t t = int64_t(col) * tg.tilesPerCol + kk / tg.K;
+    const TileInfo& ti = tiles[t];
+    const int c = cnt16[s];
+    for (int k = 0; k < c; ++k)
+    {
+        const uint16_t e = nbr16[entryIndex(k, s, strideS)];
+        const int q = e & 0x7fff;
+        int rr = 0;
+        while (rr + 1 < ti.nr && ti.base[rr + 1] <= q) ++rr;
+        const int32_t jrec = sorted[ti.begin[rr] + (q - ti.base[rr])];
+        nbr[int64_t(k) * stride + r] = uint32_t(jrec) | ((e & 0x8000) ? 0x80000000u : 0u);
+    }
+    cnt[r] = c;
+}
+
+__global__ void k_gatherSorted(const double4* pos, const int32_t* sorted, int64_t n, double4* out)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = pos[sorted[s]];
+}
+
+__global__ void k_zeroI32v(int32_t* p, int64_t n)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+__global__ void k_halfExtract(const uint32_t* nbr, const int32_t* cnt, int64_t nrows, int64_t stride, int32_t* hnbr,
+                              int32_t* hcnt)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int c = cnt[r];
+    int h = 0;
+    for (int k = 0; k < c; ++k)
+    {
+        const uint32_t e = nbr[int64_t(k) * stride + r];
+        if (e & 0x80000000u) hnbr[int64_t(h++) * stride + r] = int32_t(e & 0x7fffffffu);
+    }
+    hcnt[r] = h;
+}
+
+__global__ void k_countWithin(const double4* pos, const uint32_t* nbr, const int32_t* cnt, int64_t begin, int64_t rows,
+                              int64_t stride, double r2, unsigned long long* out)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long h = 0, f = 0;
+    if (r < rows)
+    {
+        const double4 pi = pos[begin + r];
+        for (int k = 0; k < cnt[r]; ++k)
+        {
+            const uint32_t e = nbr[int64_t(k) * stride + r];
+            const double4 pj = pos[e & 0x7fffffffu];
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            if (dx * dx + dy * dy + dz * dz <= r2)
+            {
+                ++f;
+                if (e & 0x80000000u) ++h;
+            }
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+    {
+        h += __shfl_xor(h, off, WAVE);
+        f += __shfl_xor(f, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0)
+    {
+        atomicAdd(&out[0], h);
+        atomicAdd(&out[1], f);
+    }
+}
+
+__global__ void k_rowsToHost(const uint32_t* nbr, const int32_t* cnt, int64_t nrows, int64_t stride, int32_t* out,
+                             int64_t rowStride, int mask)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int c = cnt[r];
+    for (int k = 0; k < c && k < rowStride; ++k)
+    {
+        uint32_t e = nbr[int64_t(k) * stride + r];
+        if (mask) e &= 0x7fffffffu;
+        out[r * rowStride + k] = int32_t(e);
+    }
+}
+
+TileGeom geom(const mrmd_verlet_list* vl)
+{
+    const GridD& g = vl->bin.grid;
+    return TileGeom{g.n[0], g.n[1], g.n[2], vl->tileK, vl->tilesPerCol, vl->range};
+}
+
+void setupTiles(mrmd_verlet_list* vl, int K)
+{
+    mrmd_ctx* ctx = vl->ctx;
+    const GridD& g = vl->bin.grid;
+    vl->tileK = K;
+    vl->tilesPerCol = (g.n[2] + K - 1) / K;
+    vl->ntiles = int64_t(g.n[0]) * g.n[1] * vl->tilesPerCol;
+    vl->tiles.reserve(ctx, std::max<int64_t>(vl->ntiles, 1));
+    int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
+    hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
+    hipLaunchKernelGGL(k_tileSetup, dim3(gridFor(vl->ntiles)), dim3(BLOCK), 0, ctx->stream, geom(vl), vl->ntiles,
+                       vl->bin.offsets.p, vl->bin.counts.p, vl->tiles.p, dmax);
+    checkLaunch();
+    int32_t h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    vl->maxStaged = h;
+}
+
+void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t begin, int64_t end, double rn,
+                 double ratio, const double* gmin, const double* gmax, int64_t maxNeigh)
+{
+    mrmd_ctx* ctx = vl->ctx;
+    REQUIRE(begin >= 0 && begin <= end && end <= nall, "invalid row range [%lld, %lld) of %lld", (long long)begin,
+            (long long)end, (long long)nall);
+    REQUIRE(rn > 0.0 && ratio > 0.0, "invalid radius / cell ratio");
+    REQUIRE(nall < (int64_t(1) << 31) - 1, "more than 2^31 records");
+    KTimer t(ctx, "verlet_build");
+    vl->nrows = nall;
+    vl->begin = begin;
+    vl->end = end;
+    vl->rn = rn;
+    vl->ratio = ratio;
+    vl->range = int(std::ceil(1.0 / ratio));
+    if (vl->range > 2) fail(MRMD_ERR_UNSUP, "cell ratio %g needs a stencil wider than 5 cells", ratio);
+    const double gs = ratio * rn;
+    const double delta[3] = {gs, gs, gs};
+    const GridD g = makeGrid(gmin, gmax, delta);
+    vl->bin.build(ctx, pos, 0, nall, g, true);
+    vl->globalValid = false;
+    vl->halfValid = false;
+    // tile height: aim at ~1200 staged records, shrink until every tile fits in LDS
+    const int R = vl->range, W = 2 * R + 1;
+    int K = vl->tileK > 0 ? std::min(vl->tileK, g.n[2]) : 0;
+    if (K <= 0)
+    {
+        const double perCell = double(std::max<int64_t>(nall, 1)) / double(g.total());
+        K = int(1200.0 / (W * W * std::max(perCell, 1e-3))) - 2 * R;
+        K = std::max(1, std::min(K, g.n[2]));
+    }
+    K = std::min(K, MAX_SPAN - 2 * R);
+    for (;;)
+    {
+        setupTiles(vl, K);
+        if (vl->maxStaged <= STAGE_MAX) break;
+        if (K == 1)
+            fail(MRMD_ERR_UNSUP, "a tile neighbourhood holds %d records (limit %d): density too high for the tiled list",
+                 vl->maxStaged, STAGE_MAX);
+        K = std::max(1, K / 2);
+    }
+    vl->strideS = std::max<int64_t>(64, (nall + 63) / 64 * 64);
+    vl->cnt16.reserve(ctx, vl->strideS);
+    hipLaunchKernelGGL(k_zeroI32v, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, vl->cnt16.p, nall);
+    if (vl->capFull <= 0) vl->capFull = int32_t(std::max<int64_t>(2 * maxNeigh + 16, 96));
+    vl->capFull = (vl->capFull + CHUNK - 1) / CHUNK * CHUNK;
+    int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64)) + 1;
+    const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
+    for (int attempt = 0; attempt < 2; ++attempt)
+    {
+        vl->nbr16.reserve(ctx, int64_t(vl->capFull) * vl->strideS);
+        hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
+        if (vl->ntiles > 0 && end > begin)
+        {
+            if (R == 1)
+                hipLaunchKernelGGL(k_verletTiled<1>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
+                                   vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
+                                   vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
+                                   vl->capFull, vl->cnt16.p, dmax);
+            else
+                hipLaunchKernelGGL(k_verletTiled<2>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
+                                   vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
+                                   vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
+                                   vl->capFull, vl->cnt16.p, dmax);
+        }
+        checkLaunch();
+        int32_t hmax = 0;
+        HIPCHECK(hipMemcpyAsync(&hmax, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        vl->maxFull = hmax;
+        if (hmax <= vl->capFull) break;
+        // Cabana VerletLayout2D: re-count and refill with the larger width
+        vl->capFull = (hmax + hmax / 8 + 8 + CHUNK - 1) / CHUNK * CHUNK;
+    }
+    if (vl->maxFull > vl->capFull) fail(MRMD_ERR_STATE, "neighbour rows did not fit after a re-count");
+}
+}  // namespace
+
+void mrmd_verlet_list::refreshSorted(const double4* pos)
+{
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_gatherSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, pos, bin.sorted.p, nrows,
+                           bin.sortedPos.p);
+    checkLaunch();
+}
+
+void mrmd_verlet_list::ensureGlobal()
+{
+    if (globalValid) return;
+    const int64_t rows = end - begin;
+    stride = std::max<int64_t>(64, (rows + 63) / 64 * 64);
+    cnt.reserve(ctx, stride);
+    nbr.reserve(ctx, int64_t(std::max(maxFull, 1)) * stride);
+    if (rows > 0)
+    {
+        hipLaunchKernelGGL(k_zeroI32v, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, cnt.p, rows);
+        hipLaunchKernelGGL(k_toGlobal, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, nbr16.p, cnt16.p, strideS,
+                           tiles.p, geom(this), bin.sorted.p, bin.cellOf.p, nrows, begin, end, nbr.p, cnt.p, stride);
+    }
+    checkLaunch();
+    globalValid = true;
+}
+
+void mrmd_verlet_list::ensureHalf()
+{
+    if (halfValid) return;
+    ensureGlobal();
+    const int64_t rows = end - begin;
+    capHalf = std::max(maxFull, 1);
+    hnbr.reserve(ctx, int64_t(capHalf) * stride);
+    hcnt.reserve(ctx, stride);
+    if (rows > 0)
+        hipLaunchKernelGGL(k_halfExtract, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, nbr.p, cnt.p, rows, stride,
+                           hnbr.p, hcnt.p);
+    checkLaunch();
+    halfValid = true;
+}
+
+extern "C" {
+
+int mrmd_verlet_create(mrmd_ctx* c, mrmd_verlet_list** out)
+{
+    MRMD_API_BEGIN(c)
+    REQUIRE(out, "null output");
+    auto* v = new mrmd_verlet_list();
+    v->ctx = c;
+    *out = v;
+    MRMD_API_END
+}
+
+int mrmd_verlet_destroy(mrmd_verlet_list* v)
+{
+    if (!v) return MRMD_OK;
+    (void)hipStreamSynchronize(v->ctx->stream);
+    delete v;
+    return MRMD_OK;
+}
+
+int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int block)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && tile_cells >= 0, "invalid tiling");
+    REQUIRE(block == 64 || block == 128 || block == 192 || block == 256, "workgroup size must be 64, 128, 192 or 256");
+    vl->tileK = tile_cells;
+    vl->block = block;
+    MRMD_API_END
+}
+
+int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* block, int* max_staged)
+{
+    if (!vl) return MRMD_ERR_ARG;
+    if (tile_cells) *tile_cells = vl->tileK;
+    if (block) *block = vl->block;
+    if (max_staged) *max_staged = vl->maxStaged;
+    return MRMD_OK;
+}
+
+int mrmd_verlet_build(mrmd_verlet_list* vl, const mrmd_atoms* a, int64_t begin, int64_t end, double rn, double ratio,
+                      const double* gmin, const double* gmax, int64_t maxNeigh)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && a && gmin && gmax, "invalid arguments");
+    verletBuild(vl, a->pos.p, a->nlocal + a->nghost, begin, end, rn, ratio, gmin, gmax, maxNeigh);
+    MRMD_API_END
+}
+
+int mrmd_verlet_build_molecules(mrmd_verlet_list* vl, const mrmd_molecules* m, int64_t begin, int64_t end, double rn,
+                                double ratio, const double* gmin, const double* gmax, int64_t maxNeigh)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && m && gmin && gmax, "invalid arguments");
+    verletBuild(vl, m->pos.p, m->nlocal + m->nghost, begin, end, rn, ratio, gmin, gmax, maxNeigh);
+    MRMD_API_END
+}
+
+int mrmd_verlet_half_counts(mrmd_verlet_list* vl, int32_t* counts, int64_t n, int64_t* total)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl, "null list");
+    vl->ensureHalf();
+    const int64_t rows = vl->end - vl->begin;
+    std::vector<int32_t> h(static_cast<size_t>(rows));
+    if (rows) HIPCHECK(hipMemcpyAsync(h.data(), vl->hcnt.p, size_t(rows) * 4, hipMemcpyDeviceToHost, vl->ctx->stream));
+    sync(vl->ctx);
+    int64_t tot = 0;
+    for (auto v : h) tot += v;
+    if (total) *total = tot;
+    if (counts)
+        for (int64_t i = 0; i < n; ++i)
+            counts[i] = (i >= vl->begin && i < vl->end) ? h[size_t(i - vl->begin)] : 0;
+    MRMD_API_END
+}
+
+int mrmd_verlet_half_neighbors(mrmd_verlet_list* vl, int32_t* host, int64_t n, int64_t rowStride)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && host && rowStride > 0, "invalid arguments");
+    vl->ensureHalf();
+    mrmd_ctx* ctx = vl->ctx;
+    const int64_t rows = std::min<int64_t>(n, vl->end - vl->begin);
+    if (rows <= 0) return MRMD_OK;
+    DBuf<int32_t> dev;
+    dev.reserve(ctx, rows * rowStride);
+    hipLaunchKernelGGL(k_rowsToHost, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream,
+                       reinterpret_cast<const uint32_t*>(vl->hnbr.p), vl->hcnt.p, rows, vl->stride, dev.p, rowStride, 0);
+    HIPCHECK(hipMemcpyAsync(host, dev.p, size_t(rows * rowStride) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    MRMD_API_END
+}
+
+int mrmd_verlet_full_neighbors(mrmd_verlet_list* vl, int32_t* host, int32_t* counts, int64_t n, int64_t rowStride)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && rowStride > 0, "invalid arguments");
+    vl->ensureGlobal();
+    mrmd_ctx* ctx = vl->ctx;
+    const int64_t rows = std::min<int64_t>(n, vl->end - vl->begin);
+    if (rows <= 0) return MRMD_OK;
+    if (counts) HIPCHECK(hipMemcpyAsync(counts, vl->cnt.p, size_t(rows) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (host)
+    {
+        DBuf<int32_t> dev;
+        dev.reserve(ctx, rows * rowStride);
+        hipLaunchKernelGGL(k_rowsToHost, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, vl->nbr.p, vl->cnt.p, rows,
+                           vl->stride, dev.p, rowStride, 1);
+        HIPCHECK(hipMemcpyAsync(host, dev.p, size_t(rows * rowStride) * 4, hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+    }
+    sync(ctx);
+    MRMD_API_END
+}
+
+int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* a, double r, int64_t* half, int64_t* full)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && a && vl->nrows <= a->size, "invalid arguments");
+    vl->ensureGlobal();
+    mrmd_ctx* ctx = vl->ctx;
+    const int64_t rows = vl->end - vl->begin;
+    unsigned long long* d = static_cast<unsigned long long*>(ctxScratch(ctx, 2, 64));
+    HIPCHECK(hipMemsetAsync(d, 0, 16, ctx->stream));
+    if (rows > 0)
+        hipLaunchKernelGGL(k_countWithin, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a->pos.p, vl->nbr.p,
+                           vl->cnt.p, vl->begin, rows, vl->stride, r * r, d);
+    checkLaunch();
+    unsigned long long h[2];
+    HIPCHECK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    if (half) *half = int64_t(h[0]);
+    if (full) *full = int64_t(h[1]);
+    MRMD_API_END
+}
+
+int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* maxFull, int64_t* totalFull, int32_t* dims)
+{
+    if (!vl) return MRMD_ERR_ARG;
+    if (rows) *rows = vl->end - vl->begin;
+    if (maxFull) *maxFull = vl->maxFull;
+    if (totalFull)
+    {
+        auto* v = const_cast<mrmd_verlet_list*>(vl);
+        const int64_t r = v->end - v->begin;
+        std::vector<int32_t> h(static_cast<size_t>(r));
+        if (r)
+        {
+            try
+            {
+                v->ensureGlobal();
+            }
+            catch (...)
+            {
+                return MRMD_ERR_HIP;
+            }
+            if (hipMemcpy(h.data(), v->cnt.p, size_t(r) * 4, hipMemcpyDeviceToHost) != hipSuccess) return MRMD_ERR_HIP;
+        }
+        int64_t t = 0;
+        for (auto c : h) t += c;
+        *totalFull = t;
+    }
+    if (dims)
+        for (int d = 0; d < 3; ++d) dims[d] = vl->bin.grid.n[d];
+    return MRMD_OK;
+}
+
+}  // extern "C"

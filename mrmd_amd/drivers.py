@@ -1,14 +1,28 @@
 """Time loops of the reference's example drivers, written against the action API.
 
-run_lj mirrors examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:135-186 (velocity
-Verlet) and, with langevin=True, the pre-force integrator of
+LJSystem.step mirrors examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:135-186
+(velocity Verlet) and, with langevin=True, the pre-force integrator of
 examples/01_LennardJones_NVT/01_LennardJones_NVT.cpp:139-214; permute=True adds the
 LinkedCellList + Atoms::permute of tests/NVT/NVT.cpp:138-144 on every rebuild.
+
+AdResSSystem.step is the LJ/ideal-gas AdResS step composed from the reference's actions in
+the order their contracts force (molecule forces reach ghost atoms before the ghost fold;
+lambda is current before LJ_IdealGas::run): Langevin pre-force integration,
+UpdateMolecules, multi-resolution ghost exchange / Verlet list over molecule centres on a
+rebuild (else a ghost refresh), UpdateMolecules, LJ_IdealGas::run,
+ContributeMoleculeForceToAtoms, ghost fold, ThermodynamicForce sample / periodic update /
+apply, Langevin post-force integration (action/LJ_IdealGas.cpp:227-260,
+action/ContributeMoleculeForceToAtoms.cpp:23-48, action/ThermodynamicForce.hpp:98-152).
 """
 import numpy as np
 
-from . import (FORCE, POS, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList,
-               VelocityVerlet, VelocityVerletLangevinThermostat, make_box)
+from . import (FORCE, POS, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList, LJ_IdealGas,
+               Molecules, Slab, ThermodynamicForce, VelocityVerlet, VelocityVerletLangevinThermostat, make_box)
+
+
+def _edges(L):
+    e = np.broadcast_to(np.asarray(L, dtype=np.float64), (3,)).copy()
+    return e
 
 
 class LJSystem:
@@ -16,10 +30,10 @@ class LJSystem:
                  cell_ratio=1.0, max_neigh=0, langevin=False, gamma=1.0, temperature=1.0, seed=1234,
                  permute=False, types=None, num_types=1, type_stride=1):
         self.ctx = ctx
-        self.L = L
+        self.L = _edges(L)
         self.rc, self.skin = rc, skin
         self.rn = rc + skin
-        self.box = make_box([0, 0, 0], [L] * 3, self.rn)
+        self.box = make_box([0, 0, 0], self.L, self.rn)
         self.atoms = Atoms.from_arrays(ctx, pos, vel=vel, type=types)
         self.ghosts = GhostLayer(ctx)
         self.vl = HalfVerletList(ctx)
@@ -35,11 +49,11 @@ class LJSystem:
         a = self.atoms
         self.ghosts.exchange_real_atoms(a, self.box)
         if self.permute:
-            cl = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], [self.L] * 3)
+            cl = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
             a.permute(cl)
         self.ghosts.create_ghost_atoms(a, self.box)
-        lo, hi = -self.rn, self.L + self.rn
-        self.vl.build(a, 0, a.nlocal, self.rn, self.cell_ratio, [lo] * 3, [hi] * 3, self.max_neigh)
+        self.vl.build(a, 0, a.nlocal, self.rn, self.cell_ratio, np.full(3, -self.rn), self.L + self.rn,
+                      self.max_neigh)
         self.rebuilds += 1
 
     def step(self, dt):
@@ -67,6 +81,72 @@ class LJSystem:
                 e, w = self.lj.energy_virial()
                 rec.append((e, self.atoms.kinetic_energy(), w))
         return np.array(rec)
+
+    def local(self, field):
+        return self.atoms.download(field, self.atoms.nlocal)
+
+
+class AdResSSystem:
+    """monatomic LJ / ideal-gas AdResS slab: one molecule per atom (createMoleculeForEachAtom),
+    slab weighting along x centred in the box, thermodynamic force on x-bins"""
+
+    def __init__(self, ctx, pos, vel, L, atomistic_diameter, hybrid_diameter, nu=1, rc=2.5, skin=0.3,
+                 sigma=1.0, epsilon=1.0, cap=0.0, shifted=True, gamma=1.0, temperature=1.0, seed=1234,
+                 target_density=None, bin_width=1.0, tf_every=100, tf_sigma=2.0, tf_range=3.0,
+                 tf_symmetric=True, tf_periodic=True, modulation=1.0):
+        self.ctx = ctx
+        self.L = _edges(L)
+        self.rc, self.skin = rc, skin
+        self.rn = rc + skin
+        self.box = make_box([0, 0, 0], self.L, self.rn)
+        self.atoms = Atoms.from_arrays(ctx, pos, vel=vel)
+        self.mols = Molecules.for_each_atom(ctx, self.atoms)
+        self.slab = Slab(0.5 * self.L, atomistic_diameter, hybrid_diameter, nu)
+        self.ghosts = GhostLayer(ctx)
+        self.vl = HalfVerletList(ctx)
+        self.ljig = LJ_IdealGas(ctx, [cap], [rc], [sigma], [epsilon], shifted, 1)
+        rho = target_density if target_density is not None else pos.shape[0] / float(np.prod(self.L))
+        self.tf = ThermodynamicForce(ctx, [rho], self.box, bin_width, [modulation], tf_symmetric, tf_periodic)
+        self.tf_every, self.tf_sigma, self.tf_range = tf_every, tf_sigma, tf_range
+        self.langevin = VelocityVerletLangevinThermostat(gamma, temperature, seed)
+        self.max_disp = np.finfo(np.float64).max
+        self.rebuilds = 0
+        self.step_count = 0
+        self.energy = 0.0
+
+    def rebuild(self):
+        m, a = self.mols, self.atoms
+        m.update(a, None)  # molecule centres decide the exchange
+        self.ghosts.multires_exchange_real_atoms(m, a, self.box)
+        self.ghosts.multires_create_ghost_atoms(m, a, self.box)
+        m.update(a, None)
+        self.vl.build(m, 0, m.nlocal, self.rn, 1.0, np.full(3, -self.rn), self.L + self.rn, molecules=True)
+        self.rebuilds += 1
+
+    def step(self, dt):
+        m, a = self.mols, self.atoms
+        self.max_disp += self.langevin.pre_force_integrate(a, dt)
+        if self.max_disp >= self.skin * 0.5:
+            self.max_disp = 0.0
+            self.rebuild()
+        else:
+            self.ghosts.update_ghost_atoms(a, self.box)
+        m.update(a, self.slab)
+        a.set_force(0.0)
+        m.set_force(0.0)
+        self.energy = self.ljig.run(m, self.vl, a)
+        m.contribute_force(a)
+        self.ghosts.contribute_back_ghost_to_real(a)
+        self.tf.sample(a)
+        self.step_count += 1
+        if self.tf_every and self.step_count % self.tf_every == 0:
+            self.tf.update(self.tf_sigma, self.tf_range)
+        self.tf.apply(a)
+        self.langevin.post_force_integrate(a, dt)
+
+    def run(self, nsteps, dt):
+        for _ in range(nsteps):
+            self.step(dt)
 
     def local(self, field):
         return self.atoms.download(field, self.atoms.nlocal)
