@@ -9,11 +9,14 @@ zeroing, LJ force/energy/virial, ghost force fold, post-force integration: the l
 reference's NVT example, through the library's action API.
 
 Prints one JSON line.  `value` is atom-steps/s of the timed steps with all data resident
-in HBM.  `roofline` reports the LJ force kernel (algorithmic FP64 FLOPs of the reference's
+in HBM.  The launch shape of the tiled list and force kernels is autotuned during warm-up.  `roofline` reports the LJ force kernel (algorithmic FP64 FLOPs of the reference's
 half-list algorithm, 8 per candidate pair + 24 per pair inside rc with energy and virial,
 divided by the kernel's mean duration from HIP events on the library stream).
 `cpu_baseline` times the CPU restatement of the reference algorithm (AoS, half list with
-atomic F_j, OpenMP) on a bounded sample of the same system.
+atomic F_j, OpenMP) on a bounded sample of the same system.  `batched_vs_serial_speedup`
+compares the ensemble engine on a 256-replica ensemble with the same replicas run one after
+another through the single-system path; `adjoint_over_forward` is the adjoint LJ kernel's
+time over the forward force kernel's on a 32,000-atom two-type mixture.
 """
 import argparse
 import json
@@ -57,6 +60,93 @@ def cpu_baseline(pos, vel, L, steps, warm, threads):
     return pos.shape[0] * steps / dt, dt, r1 - r0
 
 
+def ensemble_bench(ctx, n_rep, steps, serial_steps, tune=True):
+    """256-replica ensemble (atom counts log-uniform in [500, 50000], densities in [0.3, 0.9],
+    temperatures in [0.7, 1.5], half AdResS slabs) through the ensemble engine with 10-step
+    graphs, against the same replicas run one after another through the single-system path
+    (LJSystem / AdResSSystem).  Both use the capped potential (r_cap = 0.7) that keeps
+    ideal-gas particles entering the hybrid region finite."""
+    from mrmd_amd.drivers import AdResSSystem, LJSystem
+    from mrmd_amd.ensemble import Ensemble, ensemble_config
+    reps, pos, vel = ensemble_config(n_rep=n_rep, seed=2024)
+    e = Ensemble(ctx, reps)
+    e.set_lj(cap=0.7)
+    e.upload(0, pos)
+    e.upload(1, vel)
+    e.run(200, 10)  # relaxation of the uniform-random starts
+    tuning = None
+    if tune:
+        best = None
+        for target in (700, 1000, 1400):
+            for block in (128, 256):
+                e.set_tiling(target, block)
+                e.run(10, 10)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                e.run(20, 10)
+                dt = time.perf_counter() - t0
+                if best is None or dt < best[0]:
+                    best = (dt, target, block)
+        e.set_tiling(best[1], best[2])
+        tuning = {"target_staged": best[1], "block": best[2]}
+    e.run(10, 10)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    e.run(steps, 10)
+    batched_s = time.perf_counter() - t0
+    n_total = e.n_total
+    batched = n_total * steps / batched_s
+    P, V = e.positions(), e.velocities()
+    # serial: every replica through the single-system path
+    serial_time, serial_atom_steps = 0.0, 0
+    for k, r in enumerate(reps):
+        common = dict(rc=2.5, skin=0.3, cap=0.7, shifted=True, gamma=r["gamma"], temperature=r["temperature"],
+                      seed=r["seed"])
+        if r["adress"]:
+            sysk = AdResSSystem(ctx, P[k], V[k], r["box"], r["atomistic_diameter"], r["hybrid_diameter"], r["nu"],
+                                **common)
+        else:
+            sysk = LJSystem(ctx, P[k], V[k], r["box"], langevin=True, **common)
+        sysk.run(2, r["dt"])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sysk.run(serial_steps, r["dt"])
+        ctx.synchronize()
+        serial_time += time.perf_counter() - t0
+        serial_atom_steps += r["num_atoms"] * serial_steps
+        del sysk
+    serial = serial_atom_steps / serial_time
+    return {"replicas": n_rep, "atoms": n_total, "adress_replicas": sum(r["adress"] for r in reps),
+            "batched_atom_steps_per_s": batched, "batched_steps": steps, "graph_steps": 10,
+            "serial_atom_steps_per_s": serial, "serial_steps_per_replica": serial_steps,
+            "speedup": batched / serial, "tiling": tuning}
+
+
+def adjoint_ratio(ctx, reps=5):
+    """adjoint / forward kernel time on a 32,000-atom two-type LJ mixture (80/20, density
+    0.8, Lorentz-Berthelot, rc 2.5), HIP events on the library stream"""
+    from mrmd_amd.adjoint import Snapshot
+    from mrmd_amd.lattice import fcc, rng
+    pos, L = fcc(20, 0.8)
+    g = rng(5)
+    pos = np.mod(pos + g.uniform(-0.1, 0.1, pos.shape), L)
+    types = (g.random(pos.shape[0]) < 0.2).astype(np.int64)
+    eAB, sAB = np.sqrt(0.5), 0.5 * (1.0 + 0.88)
+    snap = Snapshot(ctx, pos, types, L, 2, np.array([1.0, eAB, eAB, 0.5]), np.array([1.0, sAB, sAB, 0.88]))
+    up = g.standard_normal((pos.shape[0], 3))
+    snap.forces()
+    snap.adjoint(up)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    for _ in range(reps):
+        snap.forces()
+        snap.adjoint(up)
+    nf, tf = ctx.kernel_stats("lj_force")
+    na, ta = ctx.kernel_stats("lj_adjoint")
+    ctx.set_profiling(False)
+    return {"atoms": int(pos.shape[0]), "forward_ms": tf / nf, "adjoint_ms": ta / na, "ratio": (ta / na) / (tf / nf)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +157,10 @@ def main():
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint figures")
+    ap.add_argument("--ensemble-replicas", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--serial-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -132,6 +226,12 @@ def main():
                "sample": f"{args.cpu_steps} steps of the same {n}-atom system after {args.cpu_warmup} "
                          f"warm-up step(s), {creb} list rebuild(s) inside, {ct:.1f} s wall"}
 
+    extras = None
+    if not args.no_extras:
+        extras = {"ensemble": ensemble_bench(ctx, args.ensemble_replicas, args.ensemble_steps, args.serial_steps,
+                                             tune=not args.no_tune),
+                  "adjoint": adjoint_ratio(ctx)}
+
     out = {
         "metric": "atom-steps/s at 4M-atom LJ",
         "value": value,
@@ -155,6 +255,9 @@ def main():
         "cpu_baseline": cpu,
         "kernel_ms": {k: {"launches": v[0], "total_ms": v[1]} for k, v in stats.items()},
         "bandwidth": bandwidth,
+        "batched_vs_serial_speedup": extras["ensemble"]["speedup"] if extras else None,
+        "adjoint_over_forward": extras["adjoint"]["ratio"] if extras else None,
+        "extras": extras,
     }
     print(json.dumps(out))
 

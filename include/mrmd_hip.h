@@ -366,6 +366,61 @@ int mrmd_hist_gradient(mrmd_ctx* ctx, double min, double max, int64_t nbins, int
 int mrmd_hist_smoothen(mrmd_ctx* ctx, double min, double max, int64_t nbins, int64_t nhist,
                        const double* in, double sigma, double range, int periodic, double* out);
 
+/* =========================================================================================
+ * ensemble engine (no reference counterpart; for thermodynamic-force iterations and
+ * parameter sweeps): many independent periodic replicas of uneven size advanced together.
+ * Every phase of a step is one launch over all replicas; blocks of steps are captured once
+ * as a hipGraph and replayed.  A replica is either plain LJ under the Langevin thermostat
+ * (the loop of examples/01_LennardJones_NVT/01_LennardJones_NVT.cpp:139-214) or a
+ * monatomic LJ / ideal-gas AdResS slab along x (LJ_IdealGas.cpp:52-260 with one molecule
+ * per atom, Slab.hpp:96-187, ThermodynamicForce.hpp:98-152).  Each replica's result does
+ * not depend on the other replicas: a replica run alone gives the same bits.
+ * ========================================================================================= */
+typedef struct mrmd_replica {
+    int64_t num_atoms;
+    double box[3];               /* edges; the periodic box spans [0, box) */
+    double dt;
+    double gamma;                /* Langevin friction */
+    double temperature;
+    double mass;
+    uint64_t seed;               /* Langevin noise key */
+    int32_t adress;              /* 0 plain LJ, 1 AdResS slab centred in the box */
+    int32_t tf_every;            /* thermodynamic-force update interval in steps (0: never) */
+    double atomistic_diameter;   /* slab */
+    double hybrid_diameter;
+    int64_t nu;
+    double tf_bin_width;         /* thermodynamic force on x-bins */
+    double tf_target_density;
+    double tf_sigma;             /* smoothing width and range (MultiHistogram::smoothen) */
+    double tf_range;
+    double tf_modulation;
+    int32_t tf_symmetric;
+    int32_t tf_periodic;
+} mrmd_replica;
+
+int mrmd_batch_create(mrmd_ctx* ctx, const mrmd_replica* replicas, int64_t num_replicas, double rc, double skin,
+                      mrmd_batch** out);
+int mrmd_batch_destroy(mrmd_batch* b);
+/* one LJ table for all replicas: capping distance, sigma, epsilon, energy shift at rc */
+int mrmd_batch_set_lj(mrmd_batch* b, double cap, double sigma, double epsilon, int shifted);
+/* launch shape: target number of records a workgroup stages (sets each replica's tile
+ * height), workgroup size (64, 128, 192 or 256); forces a list rebuild at the next step */
+int mrmd_batch_set_tiling(mrmd_batch* b, int target_staged, int block);
+int mrmd_batch_get_tiling(const mrmd_batch* b, int* target_staged, int* block, int32_t* tile_cells);
+/* all replicas' atoms concatenated in replica order, n_total x 3 row-major (POS, VEL; FORCE
+ * for download) */
+int mrmd_batch_upload(mrmd_batch* b, int field, const double* host);
+int mrmd_batch_download(mrmd_batch* b, int field, double* host);
+/* advance every replica by nsteps; graph_steps > 0 replays a captured graph of that many
+ * steps (a remainder runs as plain launches), 0 launches every kernel directly */
+int mrmd_batch_run(mrmd_batch* b, int64_t nsteps, int graph_steps);
+/* per-replica potential energy of the last force evaluation and current kinetic energy */
+int mrmd_batch_energies(mrmd_batch* b, double* potential, double* kinetic);
+/* per-replica list rebuild counts; steps taken */
+int mrmd_batch_stats(mrmd_batch* b, int64_t* rebuilds, int64_t* steps);
+/* thermodynamic force (which 0) or density profile (which 1) of one replica's x-bins */
+int mrmd_batch_tf_get(mrmd_batch* b, int64_t replica, int which, double* host, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

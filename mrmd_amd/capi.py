@@ -33,7 +33,16 @@ class Slab(C.Structure):
                 ("pad", C.c_int32)]
 
 
-STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab}
+class Replica(C.Structure):
+    _fields_ = [("num_atoms", C.c_int64), ("box", C.c_double * 3), ("dt", C.c_double), ("gamma", C.c_double),
+                ("temperature", C.c_double), ("mass", C.c_double), ("seed", C.c_uint64), ("adress", C.c_int32),
+                ("tf_every", C.c_int32), ("atomistic_diameter", C.c_double), ("hybrid_diameter", C.c_double),
+                ("nu", C.c_int64), ("tf_bin_width", C.c_double), ("tf_target_density", C.c_double),
+                ("tf_sigma", C.c_double), ("tf_range", C.c_double), ("tf_modulation", C.c_double),
+                ("tf_symmetric", C.c_int32), ("tf_periodic", C.c_int32)]
+
+
+STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab, "mrmd_replica": Replica}
 
 _SCALARS = {
     "int": C.c_int, "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64,

@@ -1,0 +1,1211 @@
+// Ensemble engine: many independent periodic replicas (plain LJ under the Langevin
+// thermostat, or monatomic LJ / ideal-gas AdResS slabs) advanced together on one device.
+//
+// Layout.  One arena holds every replica's atoms (pos double4, vel and force
+// component-major); replica r owns atoms [a0, a0 + n), sorted slots [a0, a0 + n), its own
+// periodic cell grid (cells of edge >= r_n, at least two per axis) and its own tiles.  Each
+// phase of a step is one launch over all replicas, and no value of one replica depends on
+// another: the step's decisions (rebuild or not, sampling, thermodynamic-force updates) are
+// taken on the device per replica, so a block of steps is captured once as a hipGraph.
+//
+// Periodicity without ghost records.  A tile (a run of K cells along z in one (i, j)
+// column) stages the 3 x 3 neighbouring columns x (K + 2) cells in LDS; each staged piece
+// carries the periodic image shift that brings it next to the tile, applied as x + L or
+// x - L, i.e. exactly the position a ghost record would have, so pair distances equal those
+// of the ghost-layer path bit for bit.  Two cells per axis are enough: the two images of a
+// column then come from different stencil offsets.
+//
+// Determinism.  Per-replica sums are ordered (per-tile partials, then one pass per replica);
+// the compensation-energy samples of LJ_IdealGas are accumulated in 64-bit fixed point
+// (scale 2^32), whose integer atomics give the same total in any order.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "detmath.hpp"
+#include "nlist.hpp"
+#include "physics.hpp"
+
+using namespace mrmd_hip;
+
+namespace
+{
+constexpr int B_RANGES = 27;       // 3 x 3 columns, up to three image pieces each
+constexpr int B_SPAN = 18;         // cells of a tile's column run (K + 2), so K <= 16
+constexpr int COMP_BINS = 200;     // LJ_IdealGas compensation bins (LJ_IdealGas.hpp:54)
+constexpr double COMP_SCALE = 4294967296.0;
+constexpr int64_t SAMPLING_INTERVAL = 200;    // LJ_IdealGas defaults (LJ_IdealGas.hpp:56-57)
+constexpr int64_t COMP_UPDATE_INTERVAL = 20000;
+constexpr int MAX_BINS = 4096;     // thermodynamic-force bins of one replica
+
+struct RepD
+{
+    int64_t a0, n;
+    int32_t c0, nc[3];
+    int32_t t0, ntiles, K, tilesPerCol;
+    double L[3], dxc[3], rdxc[3];
+    double dt, zeta, temperature, mass, halfSkin;
+    uint64_t seed;
+    int32_t adress, tfEvery;
+    mrmd_slab slab;
+    int32_t h0, nbins;
+    double binWidth, invBinWidth, binVolume, forceFactor, tfSigma;
+    int32_t tfDelta, tfSym, tfPeriodic, cmp0;
+};
+
+// static geometry of a tile
+struct TileG
+{
+    int32_t rep, i, j, k0, k1;
+};
+
+// slot ranges of a tile, rebuilt with the replica's binning
+struct TileB
+{
+    int32_t rowBegin, rowEnd, nr, staged, selfPiece;
+    int32_t begin[B_RANGES], base[B_RANGES];
+    int8_t sx[B_RANGES], sy[B_RANGES], sz[B_RANGES];
+    int32_t colBase[9];
+};
+
+struct BS
+{
+    const RepD* reps;
+    int32_t nrep;
+    int64_t natoms, cap, ncells, ntiles;
+    double4* pos;
+    double* vel;
+    double* force;
+    const int32_t* atomRep;
+    const int32_t* cellRep;
+    int32_t* cellOf;
+    int32_t* counts;
+    int32_t* offsets;
+    int32_t* fill;
+    int32_t* sorted;
+    double4* sortedPos;
+    double* slotLam;
+    double* slotGrad;
+    const TileG* tg;
+    TileB* tiles;
+    uint16_t* nbr;
+    int32_t* cnt;
+    int64_t strideS;
+    int32_t capList, stageCap;
+    double* dispStep;
+    double* dispAcc;
+    int32_t* flag;
+    int64_t* rebuilds;
+    double* ePart;
+    double* tfDensity;
+    double* tfForce;
+    int64_t* tfSamples;
+    unsigned long long* comp;
+    unsigned long long* compCount;
+    double* meanComp;
+    int64_t* step;
+    int32_t* err;
+    Pre lj;
+    double rcSqr, rnSqr;
+};
+
+__device__ inline int cellIndex(const RepD& R, int ix, int iy, int iz) { return R.c0 + (ix * R.nc[1] + iy) * R.nc[2] + iz; }
+
+__device__ inline int locate(double x, double rdx, int n)
+{
+    int c = int(floor(x * rdx));
+    c = c < 0 ? 0 : c;
+    return c >= n ? n - 1 : c;
+}
+
+// ---------------------------------------------------------------------------------------
+// integration
+
+// VelocityVerletLangevinThermostat::preForceIntegrate (VelocityVerletLangevinThermostat.hpp:63-136)
+__global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < b.natoms;
+    const int r = valid ? b.atomRep[i] : -1;
+    double d2 = 0.0;
+    if (valid)
+    {
+        const RepD& R = b.reps[r];
+        const int64_t cap = b.cap;
+        const double dt = R.dt, m = R.mass, dtHalf = 0.5 * dt;
+        double4 p = b.pos[i];
+        double vx = b.vel[i], vy = b.vel[cap + i], vz = b.vel[2 * cap + i];
+        const double dtfm = dtHalf / m;
+        vx += dtfm * b.force[i];
+        vy += dtfm * b.force[cap + i];
+        vz += dtfm * b.force[2 * cap + i];
+        const double ox = p.x, oy = p.y, oz = p.z;
+        p.x += dtHalf * vx;
+        p.y += dtHalf * vy;
+        p.z += dtHalf * vz;
+        double xi[3];
+        normals3(R.seed, uint64_t(i - R.a0), uint64_t(*b.step), xi);
+        const double dtm = dt / m;
+        const double damping = detExp(-R.zeta * dtm);
+        const double sigma = sqrt(R.temperature / m * (1.0 - detExp(-2.0 * R.zeta * dtm)));
+        vx *= damping;
+        vy *= damping;
+        vz *= damping;
+        vx += sigma * xi[0];
+        vy += sigma * xi[1];
+        vz += sigma * xi[2];
+        p.x += dtHalf * vx;
+        p.y += dtHalf * vy;
+        p.z += dtHalf * vz;
+        b.pos[i] = p;
+        b.vel[i] = vx;
+        b.vel[cap + i] = vy;
+        b.vel[2 * cap + i] = vz;
+        const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    // wave-aggregated maximum when the whole wave belongs to one replica
+    const int r0 = __shfl(r, 0, WAVE);
+    if (__all(r == r0 || r < 0) && r0 >= 0)
+    {
+        const double mx = waveMax(d2);
+        if (threadIdx.x % WAVE == 0) atomicMaxNonNeg(&b.dispStep[r0], mx);
+    }
+    else if (valid)
+        atomicMaxNonNeg(&b.dispStep[r], d2);
+}
+
+// rebuild criterion per replica (examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:138-165)
+__global__ void k_bCheck(BS b)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= b.nrep) return;
+    const double d = sqrt(b.dispStep[r]);
+    b.dispStep[r] = 0.0;
+    const double acc = b.dispAcc[r] + d;
+    if (acc >= b.reps[r].halfSkin)
+    {
+        b.flag[r] = 1;
+        b.dispAcc[r] = 0.0;
+        b.rebuilds[r] += 1;
+    }
+    else
+    {
+        b.flag[r] = 0;
+        b.dispAcc[r] = acc;
+    }
+}
+
+// PeriodicMapping::mapIntoDomain (PeriodicMapping.cpp:30-58) on the box [0, L)
+__global__ void k_bWrap(BS b)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= b.natoms) return;
+    const int r = b.atomRep[i];
+    if (!b.flag[r]) return;
+    const RepD& R = b.reps[r];
+    double4 p = b.pos[i];
+    double x[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (R.L[d] <= x[d])
+        {
+            x[d] -= R.L[d];
+            x[d] = fmax(x[d], 0.0);
+        }
+        if (x[d] < 0.0)
+        {
+            x[d] += R.L[d];
+            if (R.L[d] <= x[d]) x[d] = 0.0;
+        }
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    b.pos[i] = p;
+}
+
+// ---------------------------------------------------------------------------------------
+// binning of flagged replicas (stable counting sort per replica)
+
+__global__ void k_bZeroCells(BS b)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= b.ncells || !b.flag[b.cellRep[c]]) return;
+    b.counts[c] = 0;
+    b.fill[c] = 0;
+}
+
+__global__ void k_bCount(BS b)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= b.natoms) return;
+    const int r = b.atomRep[i];
+    if (!b.flag[r]) return;
+    const RepD& R = b.reps[r];
+    const double4 p = b.pos[i];
+    const int c = cellIndex(R, locate(p.x, R.rdxc[0], R.nc[0]), locate(p.y, R.rdxc[1], R.nc[1]),
+                            locate(p.z, R.rdxc[2], R.nc[2]));
+    b.cellOf[i] = c;
+    atomicAdd(&b.counts[c], 1);
+}
+
+// one workgroup (BLOCK threads) per replica: offsets = a0 + exclusive scan of its counts
+__global__ __launch_bounds__(BLOCK) void k_bScan(BS b)
+{
+    __shared__ int32_t lds[BLOCK / WAVE];
+    const int r = blockIdx.x;
+    if (!b.flag[r]) return;
+    const RepD& R = b.reps[r];
+    const int ncell = R.nc[0] * R.nc[1] * R.nc[2];
+    int32_t carry = int32_t(R.a0);
+    for (int base = 0; base < ncell; base += BLOCK)
+    {
+        const int c = base + int(threadIdx.x);
+        const int32_t v = c < ncell ? b.counts[R.c0 + c] : 0;
+        int32_t tot;
+        const int32_t ex = blockExclusiveScan(v, lds, &tot);
+        if (c < ncell) b.offsets[R.c0 + c] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void k_bFill(BS b)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= b.natoms) return;
+    const int r = b.atomRep[i];
+    if (!b.flag[r]) return;
+    const int c = b.cellOf[i];
+    b.sorted[b.offsets[c] + atomicAdd(&b.fill[c], 1)] = int32_t(i);
+}
+
+// ascending atom index inside each cell (the fill order of the atomics is arbitrary)
+__global__ void k_bSortCells(BS b)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= b.ncells || !b.flag[b.cellRep[c]]) return;
+    int32_t* s = b.sorted + b.offsets[c];
+    const int n = b.counts[c];
+    for (int k = 1; k < n; ++k)
+    {
+        const int32_t v = s[k];
+        int j = k - 1;
+        while (j >= 0 && s[j] > v)
+        {
+            s[j + 1] = s[j];
+            --j;
+        }
+        s[j + 1] = v;
+    }
+}
+
+// slot ranges, image shifts and LDS bases of every tile of a flagged replica
+__global__ void k_bTiles(BS b)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= b.ntiles) return;
+    const TileG G = b.tg[t];
+    if (!b.flag[G.rep]) return;
+    const RepD& R = b.reps[G.rep];
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    TileB& ti = b.tiles[t];
+    const int first = cellIndex(R, G.i, G.j, G.k0), last = cellIndex(R, G.i, G.j, G.k1 - 1);
+    ti.rowBegin = b.offsets[first];
+    ti.rowEnd = b.offsets[last] + b.counts[last];
+    int nr = 0, staged = 0;
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+        {
+            int ii = G.i + di, jj = G.j + dj, sx = 0, sy = 0;
+            if (ii < 0) ii += nx, sx = -1;
+            else if (ii >= nx) ii -= nx, sx = 1;
+            if (jj < 0) jj += ny, sy = -1;
+            else if (jj >= ny) jj -= ny, sy = 1;
+            ti.colBase[(di + 1) * 3 + (dj + 1)] = staged;
+            // pieces of the z-run [k0 - 1, k1 + 1): below 0 (image -1), inside, from nz (image +1)
+            const int zs = G.k0 - 1, ze = G.k1 + 1;
+            const int lo[3] = {zs < 0 ? nz - 1 : 0, zs < 0 ? 0 : zs, 0};
+            const int hi[3] = {zs < 0 ? nz : 0, ze < nz ? ze : nz, ze > nz ? ze - nz : 0};
+            const int sh[3] = {-1, 0, 1};
+            for (int pc = 0; pc < 3; ++pc)
+            {
+                if (hi[pc] <= lo[pc]) continue;
+                const int cf = cellIndex(R, ii, jj, lo[pc]), cl = cellIndex(R, ii, jj, hi[pc] - 1);
+                ti.begin[nr] = b.offsets[cf];
+                ti.base[nr] = staged;
+                ti.sx[nr] = int8_t(sx);
+                ti.sy[nr] = int8_t(sy);
+                ti.sz[nr] = int8_t(sh[pc]);
+                if (di == 0 && dj == 0 && sh[pc] == 0) ti.selfPiece = nr;
+                staged += b.offsets[cl] + b.counts[cl] - b.offsets[cf];
+                ++nr;
+            }
+        }
+    ti.nr = nr;
+    ti.staged = staged;
+    if (staged > b.stageCap) atomicOr(b.err, 2);
+}
+
+// ---------------------------------------------------------------------------------------
+// per-step slot data: current positions in sorted order with the slab weights, and the
+// density sample of the thermodynamic force (AxialDensityProfile.cpp:21-51)
+__global__ void k_bGather(BS b)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= b.natoms) return;
+    const int32_t i = b.sorted[s];
+    const double4 p = b.pos[i];
+    const RepD& R = b.reps[b.atomRep[i]];
+    double lam = 1.0, mod = 1.0, g = 0.0;
+    if (R.adress)
+    {
+        slabEval(R.slab, p.x, lam, mod, g);
+        const int64_t bin = getBin(p.x, 0.0, R.invBinWidth, R.nbins);
+        if (bin != -1) atomicAddD(&b.tfDensity[R.h0 + bin], 1.0);
+    }
+    b.sortedPos[s] = make_double4(p.x, p.y, p.z, mod);
+    b.slotLam[s] = lam;
+    b.slotGrad[s] = g;
+}
+
+// copy a tile's neighbourhood into LDS, shifting each piece by its periodic image
+__device__ inline void stageShifted(const TileB& ti, const double4* __restrict__ sp, double4* st, const RepD& R)
+{
+    int r = 0;
+    for (int q = threadIdx.x; q < ti.staged; q += blockDim.x)
+    {
+        while (r + 1 < ti.nr && ti.base[r + 1] <= q) ++r;
+        double4 v = sp[ti.begin[r] + (q - ti.base[r])];
+        v.x += double(ti.sx[r]) * R.L[0];
+        v.y += double(ti.sy[r]) * R.L[1];
+        v.z += double(ti.sz[r]) * R.L[2];
+        st[q] = v;
+    }
+}
+
+// squared distance from p to the (unwrapped) cell (ix, iy, iz) of the replica grid
+__device__ inline double cellDistSqr(const RepD& R, double px, double py, double pz, int ix, int iy, int iz)
+{
+    const double p[3] = {px, py, pz};
+    const int c[3] = {ix, iy, iz};
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d)
+    {
+        const double centre = (double(c[d]) + 0.5) * R.dxc[d];
+        double r = fabs(p[d] - centre) - 0.5 * R.dxc[d];
+        r = r > 0.0 ? r : 0.0;
+        r2 += r * r;
+    }
+    return r2;
+}
+
+// full-row list of every tile of a flagged replica (entries: LDS slots, rows padded to
+// whole chunks with the row's own slot)
+__global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileB ti;
+    __shared__ int cstart[9 * (B_SPAN + 1)];
+    const unsigned t = xcdBlock();
+    const TileG G = b.tg[t];
+    if (!b.flag[G.rep]) return;
+    if (threadIdx.x == 0) ti = b.tiles[t];
+    __syncthreads();
+    if (ti.staged > b.stageCap || ti.rowEnd <= ti.rowBegin) return;
+    const RepD R = b.reps[G.rep];
+    stageShifted(ti, b.sortedPos, st, R);
+    const int nzr = G.k1 - G.k0 + 2;
+    if (threadIdx.x < 9)
+    {
+        const int c = threadIdx.x, di = c / 3 - 1, dj = c % 3 - 1;
+        int ii = G.i + di, jj = G.j + dj;
+        ii = ii < 0 ? ii + R.nc[0] : (ii >= R.nc[0] ? ii - R.nc[0] : ii);
+        jj = jj < 0 ? jj + R.nc[1] : (jj >= R.nc[1] ? jj - R.nc[1] : jj);
+        int run = ti.colBase[c];
+        for (int z = 0; z < nzr; ++z)
+        {
+            int kz = G.k0 - 1 + z;
+            kz = kz < 0 ? kz + R.nc[2] : (kz >= R.nc[2] ? kz - R.nc[2] : kz);
+            cstart[c * (B_SPAN + 1) + z] = run;
+            run += b.counts[cellIndex(R, ii, jj, kz)];
+        }
+        cstart[c * (B_SPAN + 1) + nzr] = run;
+    }
+    __syncthreads();
+    int localMax = 0;
+    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    {
+        const int qSelf = ti.base[ti.selfPiece] + (s - ti.begin[ti.selfPiece]);
+        const double4 xp = st[qSelf];
+        const int kz = (b.cellOf[b.sorted[s]] - R.c0) % R.nc[2];
+        int c = 0;
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int dk = -1; dk <= 1; ++dk)
+                {
+                    if (cellDistSqr(R, xp.x, xp.y, xp.z, G.i + di, G.j + dj, kz + dk) > b.rnSqr) continue;
+                    const int* cs = cstart + ((di + 1) * 3 + (dj + 1)) * (B_SPAN + 1) + (kz - G.k0 + 1 + dk);
+                    const int q1 = cs[1];
+                    for (int q = cs[0]; q < q1; q += 4)
+                    {
+                        bool hit[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                        {
+                            const bool valid = q + u < q1 && q + u != qSelf;
+                            const double4 xn = st[valid ? q + u : qSelf];
+                            const double dx = xp.x - xn.x, dy = xp.y - xn.y, dz = xp.z - xn.z;
+                            hit[u] = valid && dx * dx + dy * dy + dz * dz <= b.rnSqr;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (hit[u])
+                            {
+                                if (c < b.capList) b.nbr[entryIndex(c, s, b.strideS)] = uint16_t(q + u);
+                                ++c;
+                            }
+                    }
+                }
+        for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < b.capList; ++k)
+            b.nbr[entryIndex(k, s, b.strideS)] = uint16_t(qSelf);
+        b.cnt[s] = min(c, b.capList);  // an overflow is reported through err; reads stay in the list
+        localMax = max(localMax, c);
+    }
+    if (localMax > b.capList) atomicOr(b.err, 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// forces: weighted LJ (LJ_IdealGas.cpp:80-200 with one atom per molecule; weight 1 for
+// plain replicas, which reduces to LennardJones.hpp:168-197), drift and compensation terms
+
+template <bool CAP>
+__device__ inline void chunkW(const BS& b, const double4& pi, const double4* st, const uint4 cur, double& fx,
+                              double& fy, double& fz, double& ew, double& e0)
+{
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    double dx[CHUNK], dy[CHUNK], dz[CHUNK], d2[CHUNK], lj[CHUNK], frac2[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const double4 pj = st[(wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu];
+        dx[u] = pi.x - pj.x;
+        dy[u] = pi.y - pj.y;
+        dz[u] = pi.z - pj.z;
+        lj[u] = pj.w;
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) frac2[u] = 1.0 / d2[u];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const Pre& p = b.lj;
+        const double frac6 = frac2[u] * frac2[u] * frac2[u];
+        double ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+        double en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+        if (CAP && d2[u] < p.capDistSqr)
+        {
+            const double dist = sqrt(d2[u]);
+            ff = p.capCoeff / dist;
+            en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
+        }
+        // both partners coarse-grained: no interaction (LJ_IdealGas.cpp:95)
+        const bool in = d2[u] <= b.rcSqr && d2[u] > 0.0 && !(pi.w <= 0.0 && lj[u] <= 0.0);
+        const double w = 0.5 * (pi.w + lj[u]);
+        const double ffw = in ? ff * w : 0.0;
+        fx += dx[u] * ffw;
+        fy += dy[u] * ffw;
+        fz += dz[u] * ffw;
+        ew += in ? en * w : 0.0;
+        e0 += in ? en : 0.0;
+    }
+}
+
+template <bool CAP>
+__global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileB ti;
+    __shared__ double red[BLOCK / WAVE];
+    const unsigned t = xcdBlock();
+    const TileG G = b.tg[t];
+    if (threadIdx.x == 0) ti = b.tiles[t];
+    __syncthreads();
+    const bool ok = ti.staged <= b.stageCap;
+    const RepD R = b.reps[G.rep];
+    if (ok) stageShifted(ti, b.sortedPos, st, R);
+    __syncthreads();
+    const int64_t step = *b.step;
+    const bool sampling = R.adress && step % SAMPLING_INTERVAL == 0;
+    double eTile = 0.0;
+    for (int s = ti.rowBegin + threadIdx.x; ok && s < ti.rowEnd; s += blockDim.x)
+    {
+        const int c = b.cnt[s];
+        const int32_t i = b.sorted[s];
+        const double4 pi = b.sortedPos[s];
+        double fx = 0.0, fy = 0.0, fz = 0.0, ew = 0.0, e0 = 0.0;
+        const uint4* chunk = reinterpret_cast<const uint4*>(b.nbr) + s;
+        const int nch = (c + CHUNK - 1) / CHUNK;
+        uint4 cur = nch > 0 ? chunk[0] : uint4{0, 0, 0, 0};
+        for (int ch = 0; ch < nch; ++ch)
+        {
+            uint4 nxt = cur;
+            if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * b.strideS];
+            chunkW<CAP>(b, pi, st, cur, fx, fy, fz, ew, e0);
+            cur = nxt;
+        }
+        if (R.adress && pi.w > 0.0 && pi.w < 1.0)
+        {
+            // hybrid atom: drift -1/2 sum_j E_ij grad(lambda) and the mean compensation
+            // energy (LJ_IdealGas.cpp:150-170, 208-224)
+            const double g = b.slotGrad[s];
+            fx += -(0.5 * e0) * g;
+            const int64_t bin = getBin(b.slotLam[s], 0.0, double(COMP_BINS), COMP_BINS);
+            if (bin != -1)
+            {
+                fx += b.meanComp[R.cmp0 + bin] * g;
+                if (sampling)
+                {
+                    atomicAdd(&b.comp[R.cmp0 + bin], (unsigned long long)(long long)llrint(0.5 * e0 * COMP_SCALE));
+                    atomicAdd(&b.compCount[R.cmp0 + bin], 1ull);
+                }
+            }
+        }
+        b.force[i] = fx;
+        b.force[b.cap + i] = fy;
+        b.force[2 * b.cap + i] = fz;
+        eTile += ew;
+    }
+    eTile = blockSum(0.5 * eTile, red);
+    if (threadIdx.x == 0) b.ePart[t] = eTile;
+}
+
+// LJ_IdealGas::updateMeanCompensationEnergy (LJ_IdealGas.cpp:21-50) at the reference's
+// update interval; one workgroup per replica
+__global__ void k_bCompUpdate(BS b)
+{
+    const int r = blockIdx.x;
+    const RepD& R = b.reps[r];
+    if (!R.adress || *b.step % COMP_UPDATE_INTERVAL != 0) return;
+    for (int k = threadIdx.x; k < COMP_BINS; k += blockDim.x)
+    {
+        const int64_t idx = R.cmp0 + k;
+        const double cntv = double(b.compCount[idx]);
+        if (cntv < 0.5) continue;
+        const double sum = double((long long)b.comp[idx]) / COMP_SCALE;
+        b.meanComp[idx] = (10.0 * b.meanComp[idx] + sum / cntv) / 11.0;
+        b.comp[idx] = 0ull;
+        b.compCount[idx] = 0ull;
+    }
+}
+
+// ThermodynamicForce::update_if (ThermodynamicForce.hpp:124-152) every tfEvery steps with
+// the hybrid region as predicate; one workgroup per replica
+__global__ __launch_bounds__(BLOCK) void k_bTFUpdate(BS b)
+{
+    __shared__ double dens[MAX_BINS];
+    __shared__ double smooth[MAX_BINS];
+    const int r = blockIdx.x;
+    const RepD& R = b.reps[r];
+    if (!R.adress) return;
+    __syncthreads();
+    const int64_t samples = b.tfSamples[r] + 1;
+    if (R.tfEvery <= 0 || (*b.step + 1) % R.tfEvery != 0)
+    {
+        if (threadIdx.x == 0) b.tfSamples[r] = samples;
+        return;
+    }
+    const int nb = R.nbins;
+    double* density = b.tfDensity + R.h0;
+    double* force = b.tfForce + R.h0;
+    for (int c = threadIdx.x; c < nb; c += blockDim.x) dens[c] = density[c];
+    __syncthreads();
+    if (R.tfSym)
+    {
+        for (int c = threadIdx.x; c < nb / 2; c += blockDim.x)
+        {
+            const double v = 0.5 * (dens[c] + dens[nb - 1 - c]);
+            dens[c] = v;
+            dens[nb - 1 - c] = v;
+        }
+        __syncthreads();
+    }
+    const double norm = 1.0 / (R.binVolume * double(samples));
+    for (int c = threadIdx.x; c < nb; c += blockDim.x) dens[c] *= norm;
+    __syncthreads();
+    const HistD hd{0.0, R.binWidth, R.invBinWidth, nb, 1};
+    for (int c = threadIdx.x; c < nb; c += blockDim.x)
+        smooth[c] = smoothAt(dens, hd, c, 0, R.tfSigma, R.tfDelta, R.tfPeriodic);
+    __syncthreads();
+    mrmd_pred1 hy{};
+    hy.kind = MRMD_PRED_SLAB_HY;
+    hy.axis = 0;
+    hy.a = R.slab.center[0];
+    hy.b = 0.5 * R.slab.atomistic_diameter;
+    hy.c = R.slab.hybrid_diameter;
+    for (int c = threadIdx.x; c < nb; c += blockDim.x)
+    {
+        double gr = gradientAt(smooth, hd, c, 0, R.tfPeriodic) * R.forceFactor;
+        const double x = (double(c) + 0.5) * R.binWidth;
+        if (!evalPred1(hy, x, x, x)) gr = 0.0;
+        force[c] -= gr;
+        density[c] = 0.0;
+    }
+    if (threadIdx.x == 0) b.tfSamples[r] = 0;
+}
+
+// VelocityVerletLangevinThermostat::postForceIntegrate with the thermodynamic force
+// (ThermodynamicForce::apply, ThermodynamicForce.hpp:98-122) added first
+__global__ void k_bPost(BS b)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= b.natoms) return;
+    const RepD& R = b.reps[b.atomRep[i]];
+    double fx = b.force[i];
+    if (R.adress)
+    {
+        const int64_t bin = getBin(b.pos[i].x, 0.0, R.invBinWidth, R.nbins);
+        if (bin != -1)
+        {
+            fx += b.tfForce[R.h0 + bin];
+            b.force[i] = fx;
+        }
+    }
+    const double dtfm = (0.5 * R.dt) / R.mass;
+    b.vel[i] += dtfm * fx;
+    b.vel[b.cap + i] += dtfm * b.force[b.cap + i];
+    b.vel[2 * b.cap + i] += dtfm * b.force[2 * b.cap + i];
+}
+
+__global__ void k_bStepEnd(BS b)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *b.step += 1;
+}
+
+// per-replica ordered sums: potential energy from the tile partials, kinetic energy
+__global__ __launch_bounds__(BLOCK) void k_bEnergies(BS b, double* out)
+{
+    __shared__ double red[BLOCK / WAVE];
+    const int r = blockIdx.x;
+    const RepD& R = b.reps[r];
+    double e = 0.0;
+    for (int t = threadIdx.x; t < R.ntiles; t += blockDim.x) e += b.ePart[R.t0 + t];
+    e = blockSum(e, red);
+    double k = 0.0;
+    for (int64_t i = R.a0 + threadIdx.x; i < R.a0 + R.n; i += blockDim.x)
+    {
+        const double vx = b.vel[i], vy = b.vel[b.cap + i], vz = b.vel[2 * b.cap + i];
+        k += vx * vx + vy * vy + vz * vz;
+    }
+    k = blockSum(k, red);
+    if (threadIdx.x == 0)
+    {
+        out[2 * r] = e;
+        out[2 * r + 1] = 0.5 * R.mass * k;
+    }
+}
+
+__global__ void k_bFillD(double* p, int64_t n, double v)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+}  // namespace
+
+struct mrmd_batch
+{
+    CtxPtr ctx;
+    std::vector<mrmd_replica> host;
+    std::vector<RepD> reps;
+    std::vector<TileG> tg;
+    int64_t natoms = 0, cap = 0, ncells = 0, ntiles = 0, nbins = 0;
+    double rc = 2.5, skin = 0.3;
+    Pre lj{};
+    bool cap_ = false;
+    int targetStaged = 1000, block = 256, stageCap = 1600;
+    int32_t capList = 0;
+    int64_t steps = 0;
+
+    DBuf<RepD> dreps;
+    DBuf<double4> pos, sortedPos;
+    DBuf<double> vel, force, slotLam, slotGrad;
+    DBuf<int32_t> atomRep, cellRep, cellOf, counts, offsets, fill, sorted, cnt, flag, err;
+    DBuf<TileG> dtg;
+    DBuf<TileB> tiles;
+    DBuf<uint16_t> nbr;
+    DBuf<double> dispStep, dispAcc, ePart, tfDensity, tfForce, meanComp, energies;
+    DBuf<int64_t> rebuilds, tfSamples, step;
+    DBuf<unsigned long long> comp, compCount;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graphSteps = 0;
+
+    ~mrmd_batch() { dropGraph(); }
+    void dropGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graphSteps = 0;
+    }
+    BS state();
+    void layoutTiles();
+    void enqueueStep();
+};
+
+BS mrmd_batch::state()
+{
+    BS b{};
+    b.reps = dreps.p;
+    b.nrep = int32_t(reps.size());
+    b.natoms = natoms;
+    b.cap = cap;
+    b.ncells = ncells;
+    b.ntiles = ntiles;
+    b.pos = pos.p;
+    b.vel = vel.p;
+    b.force = force.p;
+    b.atomRep = atomRep.p;
+    b.cellRep = cellRep.p;
+    b.cellOf = cellOf.p;
+    b.counts = counts.p;
+    b.offsets = offsets.p;
+    b.fill = fill.p;
+    b.sorted = sorted.p;
+    b.sortedPos = sortedPos.p;
+    b.slotLam = slotLam.p;
+    b.slotGrad = slotGrad.p;
+    b.tg = dtg.p;
+    b.tiles = tiles.p;
+    b.nbr = nbr.p;
+    b.cnt = cnt.p;
+    b.strideS = cap;
+    b.capList = capList;
+    b.stageCap = stageCap;
+    b.dispStep = dispStep.p;
+    b.dispAcc = dispAcc.p;
+    b.flag = flag.p;
+    b.rebuilds = rebuilds.p;
+    b.ePart = ePart.p;
+    b.tfDensity = tfDensity.p;
+    b.tfForce = tfForce.p;
+    b.tfSamples = tfSamples.p;
+    b.comp = comp.p;
+    b.compCount = compCount.p;
+    b.meanComp = meanComp.p;
+    b.step = step.p;
+    b.err = err.p;
+    b.lj = lj;
+    b.rcSqr = rc * rc;
+    b.rnSqr = (rc + skin) * (rc + skin);
+    return b;
+}
+
+// per-replica tile height from the target staged size, tile geometry, LDS capacity
+void mrmd_batch::layoutTiles()
+{
+    tg.clear();
+    double maxStage = 0.0;
+    int64_t t0 = 0;
+    for (auto& R : reps)
+    {
+        const double perCell = double(R.n) / double(R.nc[0] * R.nc[1] * R.nc[2]);
+        int K = int(std::lround(double(targetStaged) / (9.0 * std::max(perCell, 1e-3)))) - 2;
+        K = std::max(1, std::min({K, R.nc[2], B_SPAN - 2}));
+        R.K = K;
+        R.tilesPerCol = (R.nc[2] + K - 1) / K;
+        R.t0 = int32_t(t0);
+        R.ntiles = R.nc[0] * R.nc[1] * R.tilesPerCol;
+        for (int i = 0; i < R.nc[0]; ++i)
+            for (int j = 0; j < R.nc[1]; ++j)
+                for (int kb = 0; kb < R.tilesPerCol; ++kb)
+                    tg.push_back(TileG{int32_t(&R - reps.data()), i, j, kb * K, std::min((kb + 1) * K, R.nc[2])});
+        t0 += R.ntiles;
+        maxStage = std::max(maxStage, 9.0 * (K + 2) * perCell);
+    }
+    ntiles = t0;
+    // LDS capacity: the expected largest neighbourhood with room for density fluctuations
+    stageCap = int(std::min(double(STAGE_MAX), std::ceil(1.6 * maxStage + 64.0)));
+    mrmd_ctx* c = ctx;
+    dreps.reserve(c, int64_t(reps.size()));
+    HIPCHECK(hipMemcpyAsync(dreps.p, reps.data(), reps.size() * sizeof(RepD), hipMemcpyHostToDevice, c->stream));
+    dtg.reserve(c, ntiles);
+    HIPCHECK(hipMemcpyAsync(dtg.p, tg.data(), tg.size() * sizeof(TileG), hipMemcpyHostToDevice, c->stream));
+    tiles.reserve(c, ntiles);
+    ePart.reserve(c, ntiles);
+    hipLaunchKernelGGL(k_bFillD, dim3(gridFor(ntiles)), dim3(BLOCK), 0, c->stream, ePart.p, ntiles, 0.0);
+    // a new tiling needs new tile ranges and lists: force a rebuild of every replica
+    hipLaunchKernelGGL(k_bFillD, dim3(gridFor(int64_t(reps.size()))), dim3(BLOCK), 0, c->stream, dispAcc.p,
+                       int64_t(reps.size()), DBL_MAX);
+    checkLaunch();
+    sync(c);
+    dropGraph();
+}
+
+void mrmd_batch::enqueueStep()
+{
+    mrmd_ctx* c = ctx;
+    const BS b = state();
+    const unsigned ga = gridFor(natoms), gc = gridFor(ncells), gt = gridFor(ntiles);
+    const unsigned nr = unsigned(reps.size());
+    const size_t lds = size_t(stageCap) * sizeof(double4);
+    hipLaunchKernelGGL(k_bPre, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bCheck, dim3(gridFor(nr)), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bWrap, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bZeroCells, dim3(gc), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bCount, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bSortCells, dim3(gc), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    if (cap_)
+        hipLaunchKernelGGL(k_bForce<true>, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    else
+        hipLaunchKernelGGL(k_bForce<false>, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    hipLaunchKernelGGL(k_bCompUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bTFUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bPost, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bStepEnd, dim3(1), dim3(64), 0, c->stream, b);
+}
+
+extern "C" {
+
+int mrmd_batch_create(mrmd_ctx* c, const mrmd_replica* reps, int64_t nrep, double rc, double skin, mrmd_batch** out)
+{
+    MRMD_API_BEGIN(c)
+    REQUIRE(out && reps && nrep > 0, "invalid arguments");
+    REQUIRE(rc > 0.0 && skin >= 0.0, "invalid cutoff / skin");
+    const double rn = rc + skin;
+    auto* b = new mrmd_batch();
+    b->ctx = c;
+    b->rc = rc;
+    b->skin = skin;
+    try
+    {
+        b->host.assign(reps, reps + nrep);
+        int64_t a0 = 0, c0 = 0, h0 = 0;
+        for (int64_t r = 0; r < nrep; ++r)
+        {
+            const mrmd_replica& H = reps[r];
+            REQUIRE(H.num_atoms > 0, "replica %lld has no atoms", (long long)r);
+            REQUIRE(H.dt > 0.0 && H.mass > 0.0 && H.temperature >= 0.0 && H.gamma >= 0.0,
+                    "replica %lld: invalid integration parameters", (long long)r);
+            RepD R{};
+            R.a0 = a0;
+            R.n = H.num_atoms;
+            R.c0 = int32_t(c0);
+            for (int d = 0; d < 3; ++d)
+            {
+                REQUIRE(H.box[d] >= 2.0 * rn, "replica %lld: box edge %g is below 2 (rc + skin)", (long long)r,
+                        H.box[d]);
+                R.nc[d] = int(std::floor(H.box[d] / rn));
+                R.L[d] = H.box[d];
+                R.dxc[d] = H.box[d] / R.nc[d];
+                R.rdxc[d] = 1.0 / R.dxc[d];
+            }
+            R.dt = H.dt;
+            R.zeta = H.gamma;
+            R.temperature = H.temperature;
+            R.mass = H.mass;
+            R.halfSkin = 0.5 * skin;
+            R.seed = H.seed;
+            R.adress = H.adress ? 1 : 0;
+            R.tfEvery = H.tf_every;
+            R.cmp0 = int32_t(r * COMP_BINS);
+            if (R.adress)
+            {
+                REQUIRE(H.tf_bin_width > 0.0 && H.tf_target_density > 0.0 && H.tf_sigma > 0.0,
+                        "replica %lld: invalid thermodynamic-force parameters", (long long)r);
+                R.slab.center[0] = 0.5 * H.box[0];
+                R.slab.center[1] = 0.5 * H.box[1];
+                R.slab.center[2] = 0.5 * H.box[2];
+                R.slab.atomistic_diameter = H.atomistic_diameter;
+                R.slab.hybrid_diameter = H.hybrid_diameter;
+                R.slab.nu = H.nu;
+                R.slab.interface_type = 0;
+                // ThermodynamicForce ctor (ThermodynamicForce.cpp:25-57)
+                R.nbins = int32_t(std::ceil(H.box[0] / H.tf_bin_width));
+                REQUIRE(R.nbins <= MAX_BINS, "replica %lld: more than %d thermodynamic-force bins", (long long)r,
+                        MAX_BINS);
+                R.h0 = int32_t(h0);
+                R.binWidth = H.box[0] / R.nbins;
+                R.invBinWidth = 1.0 / R.binWidth;
+                R.binVolume = H.box[1] * H.box[2] * R.binWidth;
+                R.forceFactor = H.tf_modulation / H.tf_target_density;
+                R.tfSigma = H.tf_sigma;
+                R.tfDelta = int32_t(H.tf_range * H.tf_sigma / R.binWidth);
+                R.tfSym = H.tf_symmetric;
+                R.tfPeriodic = H.tf_periodic;
+                h0 += R.nbins;
+            }
+            a0 += R.n;
+            c0 += int64_t(R.nc[0]) * R.nc[1] * R.nc[2];
+            b->reps.push_back(R);
+        }
+        REQUIRE(a0 < (int64_t(1) << 31) - 1 && c0 < (int64_t(1) << 31) - 1, "ensemble too large");
+        b->natoms = a0;
+        b->cap = std::max<int64_t>(64, (a0 + 63) / 64 * 64);
+        b->ncells = c0;
+        b->nbins = std::max<int64_t>(h0, 1);
+        // list width from the densest replica: Poisson tail of the count within r_n
+        double rhoMax = 0.0;
+        for (auto& R : b->reps) rhoMax = std::max(rhoMax, double(R.n) / (R.L[0] * R.L[1] * R.L[2]));
+        const double mean = 4.0 / 3.0 * PI * rn * rn * rn * rhoMax;
+        b->capList = int32_t((int64_t(mean + 8.0 * std::sqrt(mean) + 48.0) + CHUNK - 1) / CHUNK * CHUNK);
+        mrmd_ctx* x = c;
+        const int64_t na = b->natoms, cp = b->cap, nr = nrep;
+        b->pos.reserve(x, cp);
+        b->sortedPos.reserve(x, cp);
+        b->vel.reserve(x, 3 * cp);
+        b->force.reserve(x, 3 * cp);
+        b->slotLam.reserve(x, cp);
+        b->slotGrad.reserve(x, cp);
+        b->atomRep.reserve(x, cp);
+        b->cellOf.reserve(x, cp);
+        b->sorted.reserve(x, cp);
+        b->cnt.reserve(x, cp);
+        b->cellRep.reserve(x, b->ncells);
+        b->counts.reserve(x, b->ncells);
+        b->offsets.reserve(x, b->ncells);
+        b->fill.reserve(x, b->ncells);
+        b->nbr.reserve(x, int64_t(b->capList) * cp);
+        b->flag.reserve(x, nr);
+        b->err.reserve(x, 1);
+        b->dispStep.reserve(x, nr);
+        b->dispAcc.reserve(x, nr);
+        b->rebuilds.reserve(x, nr);
+        b->tfSamples.reserve(x, nr);
+        b->energies.reserve(x, 2 * nr);
+        b->tfDensity.reserve(x, b->nbins);
+        b->tfForce.reserve(x, b->nbins);
+        b->comp.reserve(x, nr * COMP_BINS);
+        b->compCount.reserve(x, nr * COMP_BINS);
+        b->meanComp.reserve(x, nr * COMP_BINS);
+        b->step.reserve(x, 1);
+        std::vector<int32_t> ar(static_cast<size_t>(na)), cr(static_cast<size_t>(b->ncells));
+        for (int64_t r = 0; r < nr; ++r)
+        {
+            const RepD& R = b->reps[size_t(r)];
+            std::fill(ar.begin() + R.a0, ar.begin() + R.a0 + R.n, int32_t(r));
+            const int64_t nc = int64_t(R.nc[0]) * R.nc[1] * R.nc[2];
+            std::fill(cr.begin() + R.c0, cr.begin() + R.c0 + nc, int32_t(r));
+        }
+        HIPCHECK(hipMemcpyAsync(b->atomRep.p, ar.data(), size_t(na) * 4, hipMemcpyHostToDevice, x->stream));
+        HIPCHECK(hipMemcpyAsync(b->cellRep.p, cr.data(), size_t(b->ncells) * 4, hipMemcpyHostToDevice, x->stream));
+        HIPCHECK(hipMemsetAsync(b->vel.p, 0, size_t(3 * cp) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->force.p, 0, size_t(3 * cp) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->pos.p, 0, size_t(cp) * sizeof(double4), x->stream));
+        HIPCHECK(hipMemsetAsync(b->dispStep.p, 0, size_t(nr) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->rebuilds.p, 0, size_t(nr) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->tfSamples.p, 0, size_t(nr) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->flag.p, 0, size_t(nr) * 4, x->stream));
+        HIPCHECK(hipMemsetAsync(b->err.p, 0, 4, x->stream));
+        HIPCHECK(hipMemsetAsync(b->step.p, 0, 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->tfDensity.p, 0, size_t(b->nbins) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->tfForce.p, 0, size_t(b->nbins) * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->comp.p, 0, size_t(nr) * COMP_BINS * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->compCount.p, 0, size_t(nr) * COMP_BINS * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->meanComp.p, 0, size_t(nr) * COMP_BINS * 8, x->stream));
+        HIPCHECK(hipMemsetAsync(b->cnt.p, 0, size_t(cp) * 4, x->stream));
+        // default table: epsilon = sigma = 1, shifted, uncapped
+        const double one = 1.0, zero = 0.0;
+        std::vector<Pre> t;
+        buildTable(t, &zero, &rc, &one, &one, 1, true);
+        b->lj = t[0];
+        b->layoutTiles();
+    }
+    catch (...)
+    {
+        delete b;
+        throw;
+    }
+    *out = b;
+    MRMD_API_END
+}
+
+int mrmd_batch_destroy(mrmd_batch* b)
+{
+    if (!b) return MRMD_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return MRMD_OK;
+}
+
+int mrmd_batch_set_lj(mrmd_batch* b, double cap, double sigma, double epsilon, int shifted)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && cap >= 0.0 && sigma > 0.0 && epsilon >= 0.0, "invalid LJ parameters");
+    std::vector<Pre> t;
+    buildTable(t, &cap, &b->rc, &sigma, &epsilon, 1, shifted != 0);
+    b->lj = t[0];
+    b->cap_ = cap > 0.0;
+    b->dropGraph();
+    MRMD_API_END
+}
+
+int mrmd_batch_set_tiling(mrmd_batch* b, int target, int block)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && target >= 64 && target <= STAGE_MAX, "target staged records must lie in [64, %d]", STAGE_MAX);
+    REQUIRE(block == 64 || block == 128 || block == 192 || block == 256, "workgroup size must be 64, 128, 192 or 256");
+    b->targetStaged = target;
+    b->block = block;
+    b->layoutTiles();
+    MRMD_API_END
+}
+
+int mrmd_batch_get_tiling(const mrmd_batch* b, int* target, int* block, int32_t* tileCells)
+{
+    if (!b) return MRMD_ERR_ARG;
+    if (target) *target = b->targetStaged;
+    if (block) *block = b->block;
+    if (tileCells)
+        for (size_t r = 0; r < b->reps.size(); ++r) tileCells[r] = b->reps[r].K;
+    return MRMD_OK;
+}
+
+int mrmd_batch_upload(mrmd_batch* b, int field, const double* host)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && host && (field == MRMD_POS || field == MRMD_VEL), "upload takes POS or VEL");
+    const int64_t n = b->natoms;
+    mrmd_ctx* c = b->ctx;
+    if (field == MRMD_POS)
+    {
+        std::vector<double4> h(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) h[size_t(i)] = make_double4(host[3 * i], host[3 * i + 1], host[3 * i + 2], 0.0);
+        HIPCHECK(hipMemcpyAsync(b->pos.p, h.data(), size_t(n) * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+        sync(c);
+        // positions changed outside the time loop: every replica rebuilds at the next step
+        hipLaunchKernelGGL(k_bFillD, dim3(gridFor(int64_t(b->reps.size()))), dim3(BLOCK), 0, c->stream, b->dispAcc.p,
+                           int64_t(b->reps.size()), DBL_MAX);
+        checkLaunch();
+    }
+    else
+    {
+        std::vector<double> h(static_cast<size_t>(3 * b->cap), 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) h[size_t(d * b->cap + i)] = host[3 * i + d];
+        HIPCHECK(hipMemcpyAsync(b->vel.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    sync(c);
+    MRMD_API_END
+}
+
+int mrmd_batch_download(mrmd_batch* b, int field, double* host)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && host && (field == MRMD_POS || field == MRMD_VEL || field == MRMD_FORCE),
+            "download takes POS, VEL or FORCE");
+    const int64_t n = b->natoms;
+    mrmd_ctx* c = b->ctx;
+    if (field == MRMD_POS)
+    {
+        std::vector<double4> h(static_cast<size_t>(n));
+        HIPCHECK(hipMemcpyAsync(h.data(), b->pos.p, size_t(n) * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        for (int64_t i = 0; i < n; ++i)
+        {
+            host[3 * i] = h[size_t(i)].x;
+            host[3 * i + 1] = h[size_t(i)].y;
+            host[3 * i + 2] = h[size_t(i)].z;
+        }
+    }
+    else
+    {
+        std::vector<double> h(static_cast<size_t>(3 * b->cap));
+        HIPCHECK(hipMemcpyAsync(h.data(), field == MRMD_VEL ? b->vel.p : b->force.p, h.size() * 8,
+                                hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) host[3 * i + d] = h[size_t(d * b->cap + i)];
+    }
+    MRMD_API_END
+}
+
+int mrmd_batch_run(mrmd_batch* b, int64_t nsteps, int graphSteps)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && nsteps >= 0 && graphSteps >= 0, "invalid arguments");
+    mrmd_ctx* c = b->ctx;
+    KTimer t(c, "batch_run");
+    int64_t done = 0;
+    if (graphSteps > 0 && nsteps >= graphSteps)
+    {
+        if (!b->exec || b->graphSteps != graphSteps)
+        {
+            b->dropGraph();
+            HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < graphSteps; ++s) b->enqueueStep();
+            HIPCHECK(hipStreamEndCapture(c->stream, &b->graph));
+            HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graphSteps = graphSteps;
+        }
+        for (; done + graphSteps <= nsteps; done += graphSteps) HIPCHECK(hipGraphLaunch(b->exec, c->stream));
+    }
+    for (; done < nsteps; ++done) b->enqueueStep();
+    checkLaunch();
+    int32_t e = 0;
+    HIPCHECK(hipMemcpyAsync(&e, b->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    b->steps += nsteps;
+    if (e & 1) fail(MRMD_ERR_STATE, "a neighbour row exceeded the list width %d (density too high)", b->capList);
+    if (e & 2)
+        fail(MRMD_ERR_STATE, "a tile neighbourhood exceeded %d staged records: lower the target with "
+             "mrmd_batch_set_tiling", b->stageCap);
+    MRMD_API_END
+}
+
+int mrmd_batch_energies(mrmd_batch* b, double* potential, double* kinetic)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b, "null batch");
+    mrmd_ctx* c = b->ctx;
+    const size_t nr = b->reps.size();
+    hipLaunchKernelGGL(k_bEnergies, dim3(unsigned(nr)), dim3(BLOCK), 0, c->stream, b->state(), b->energies.p);
+    checkLaunch();
+    std::vector<double> h(2 * nr);
+    HIPCHECK(hipMemcpyAsync(h.data(), b->energies.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (size_t r = 0; r < nr; ++r)
+    {
+        if (potential) potential[r] = h[2 * r];
+        if (kinetic) kinetic[r] = h[2 * r + 1];
+    }
+    MRMD_API_END
+}
+
+int mrmd_batch_stats(mrmd_batch* b, int64_t* rebuilds, int64_t* steps)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b, "null batch");
+    mrmd_ctx* c = b->ctx;
+    if (rebuilds)
+        HIPCHECK(hipMemcpyAsync(rebuilds, b->rebuilds.p, b->reps.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (steps) HIPCHECK(hipMemcpyAsync(steps, b->step.p, 8, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    MRMD_API_END
+}
+
+int mrmd_batch_tf_get(mrmd_batch* b, int64_t replica, int which, double* host, int64_t n)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && host && replica >= 0 && replica < int64_t(b->reps.size()), "invalid arguments");
+    const RepD& R = b->reps[size_t(replica)];
+    REQUIRE(R.adress, "replica %lld is not an AdResS replica", (long long)replica);
+    REQUIRE(n >= R.nbins, "the output holds %lld of %d bins", (long long)n, R.nbins);
+    const double* src = (which == 0 ? b->tfForce.p : b->tfDensity.p) + R.h0;
+    HIPCHECK(hipMemcpyAsync(host, src, size_t(R.nbins) * 8, hipMemcpyDeviceToHost, b->ctx->stream));
+    sync(b->ctx);
+    MRMD_API_END
+}
+
+}  // extern "C"

@@ -14,72 +14,12 @@
 #include <cstring>
 
 #include "nlist.hpp"
+#include "physics.hpp"
 
 using namespace mrmd_hip;
 
 namespace
 {
-// PrecomputedValues (LennardJones.hpp:27-39)
-struct Pre
-{
-    double ff1, ff2, ef1, ef2, rcSqr, capDist, capDistSqr, capCoeff, shift, eCap;
-};
-static_assert(sizeof(Pre) == 80, "table record");
-
-constexpr int MAX_TYPES2 = 64;
-
-// computeForceAndEnergy (LennardJones.hpp:52-78)
-__host__ __device__ inline void forceEnergy(const Pre& p, double distSqr, double& ff, double& e)
-{
-    if (distSqr >= p.capDistSqr)
-    {
-        const double frac2 = 1.0 / distSqr;
-        const double frac6 = frac2 * frac2 * frac2;
-        ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2;
-        e = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
-        return;
-    }
-    const double dist = sqrt(distSqr);
-    ff = p.capCoeff / dist;
-    e = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
-}
-
-// CappedLennardJonesPotential ctor + init functor (LennardJones.cpp:81-116, 61-79)
-void buildTable(std::vector<Pre>& t, const double* cap, const double* rc, const double* sigma, const double* eps,
-                int64_t nt2, bool shifted)
-{
-    t.assign(size_t(nt2), Pre{});
-    for (int64_t k = 0; k < nt2; ++k)
-    {
-        auto& p = t[size_t(k)];
-        const double sig2 = sigma[k] * sigma[k];
-        const double sig6 = sig2 * sig2 * sig2;
-        p.ff1 = 48.0 * eps[k] * sig6 * sig6;
-        p.ff2 = 24.0 * eps[k] * sig6;
-        p.ef1 = 4.0 * eps[k] * sig6 * sig6;
-        p.ef2 = 4.0 * eps[k] * sig6;
-        p.rcSqr = rc[k] * rc[k];
-        p.capDist = cap[k];
-    }
-    for (int64_t k = 0; k < nt2; ++k)
-    {
-        auto& p = t[size_t(k)];
-        const double capDist = p.capDist;
-        p.capDist = 0.0;
-        p.capDistSqr = 0.0;
-        double ff, e;
-        forceEnergy(p, capDist * capDist, ff, e);
-        p.capCoeff = ff * capDist;
-        p.eCap = e;
-        p.capDist = capDist;
-        p.capDistSqr = capDist * capDist;
-        if (shifted)
-        {
-            forceEnergy(p, p.rcSqr, ff, e);
-            p.shift = e;
-        }
-    }
-}
 }  // namespace
 
 struct mrmd_lj

@@ -160,7 +160,7 @@ __global__ void k_verletTiled(const double4* __restrict__ sortedPos, const int32
         // kernels mask, so their chunk loop needs no bounds test
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
             nbr[entryIndex(k, s, strideS)] = uint16_t(qSelf);
-        cnt[s] = c;
+        cnt[s] = min(c, cap);  // the true width goes to maxCnt; reads stay inside the list
         localMax = max(localMax, c);
     }
     if (localMax > 0) atomicMax(maxCnt, localMax);

@@ -16,8 +16,9 @@ action/ContributeMoleculeForceToAtoms.cpp:23-48, action/ThermodynamicForce.hpp:9
 """
 import numpy as np
 
-from . import (FORCE, POS, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList, LJ_IdealGas,
-               Molecules, Slab, ThermodynamicForce, VelocityVerlet, VelocityVerletLangevinThermostat, make_box)
+from . import (FORCE, POS, PRED_SLAB_HY, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList,
+               LJ_IdealGas, Molecules, Slab, ThermodynamicForce, VelocityVerlet, VelocityVerletLangevinThermostat,
+               make_box, pred1)
 
 
 def _edges(L):
@@ -108,6 +109,9 @@ class AdResSSystem:
         rho = target_density if target_density is not None else pos.shape[0] / float(np.prod(self.L))
         self.tf = ThermodynamicForce(ctx, [rho], self.box, bin_width, [modulation], tf_symmetric, tf_periodic)
         self.tf_every, self.tf_sigma, self.tf_range = tf_every, tf_sigma, tf_range
+        # update_if over the hybrid region: the thermodynamic force acts where lambda varies
+        self.tf_pred = pred1(kind=PRED_SLAB_HY, axis=0, a=0.5 * self.L[0], b=0.5 * atomistic_diameter,
+                             c=hybrid_diameter)
         self.langevin = VelocityVerletLangevinThermostat(gamma, temperature, seed)
         self.max_disp = np.finfo(np.float64).max
         self.rebuilds = 0
@@ -140,7 +144,7 @@ class AdResSSystem:
         self.tf.sample(a)
         self.step_count += 1
         if self.tf_every and self.step_count % self.tf_every == 0:
-            self.tf.update(self.tf_sigma, self.tf_range)
+            self.tf.update(self.tf_sigma, self.tf_range, self.tf_pred)
         self.tf.apply(a)
         self.langevin.post_force_integrate(a, dt)
 

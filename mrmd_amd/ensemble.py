@@ -1,0 +1,146 @@
+"""Ensemble engine: many independent periodic replicas advanced together (mrmd_batch_*).
+
+`Ensemble` wraps the C ABI; `ensemble_config` draws the replicas of a thermodynamic-force
+or parameter-sweep ensemble (atom counts log-uniform, densities and temperatures uniform,
+half plain LJ NVT and half AdResS slabs with the proportions of the 200 x 40 x 40 slab:
+atomistic diameter 0.15 Lx, hybrid width 0.05 Lx) with uniform-random initial positions.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import FORCE, POS, VEL, _L, _p
+from .capi import Replica
+from .lattice import maxwell_boltzmann, rng
+
+
+class Ensemble:
+    def __init__(self, ctx, replicas, rc=2.5, skin=0.3):
+        """replicas: sequence of dicts with the fields of mrmd_replica"""
+        self.ctx = ctx
+        self.n_rep = len(replicas)
+        arr = (Replica * self.n_rep)()
+        for k, r in enumerate(replicas):
+            for name, _ in Replica._fields_:
+                if name in r:
+                    v = r[name]
+                    if name == "box":
+                        for d in range(3):
+                            arr[k].box[d] = float(v[d])
+                    else:
+                        setattr(arr[k], name, v)
+        self.replicas = [dict(r) for r in replicas]
+        self.counts = np.array([int(r["num_atoms"]) for r in replicas], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)])
+        self.n_total = int(self.offsets[-1])
+        h = C.c_void_p()
+        ctx.check(_L.mrmd_batch_create(ctx.h, arr, self.n_rep, float(rc), float(skin), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _L.mrmd_batch_destroy(self.h)
+            self.h = None
+
+    def set_lj(self, cap=0.0, sigma=1.0, epsilon=1.0, shifted=True):
+        self.ctx.check(_L.mrmd_batch_set_lj(self.h, float(cap), float(sigma), float(epsilon), 1 if shifted else 0))
+
+    def set_tiling(self, target_staged, block):
+        self.ctx.check(_L.mrmd_batch_set_tiling(self.h, int(target_staged), int(block)))
+
+    def tiling(self):
+        t, b = C.c_int(), C.c_int()
+        k = np.zeros(self.n_rep, np.int32)
+        self.ctx.check(_L.mrmd_batch_get_tiling(self.h, C.byref(t), C.byref(b), _p(k)))
+        return dict(target_staged=t.value, block=b.value, tile_cells=k)
+
+    def _concat(self, arrays):
+        if isinstance(arrays, np.ndarray) and arrays.ndim == 2:
+            out = arrays
+        else:
+            out = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1, 3) for a in arrays])
+        out = np.ascontiguousarray(out, dtype=np.float64)
+        assert out.shape == (self.n_total, 3)
+        return out
+
+    def upload(self, field, arrays):
+        a = self._concat(arrays)
+        self.ctx.check(_L.mrmd_batch_upload(self.h, field, _p(a)))
+
+    def download(self, field):
+        out = np.zeros((self.n_total, 3))
+        self.ctx.check(_L.mrmd_batch_download(self.h, field, _p(out)))
+        return out
+
+    def split(self, a):
+        return [a[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n_rep)]
+
+    def positions(self):
+        return self.split(self.download(POS))
+
+    def velocities(self):
+        return self.split(self.download(VEL))
+
+    def forces(self):
+        return self.split(self.download(FORCE))
+
+    def run(self, nsteps, graph_steps=10):
+        self.ctx.check(_L.mrmd_batch_run(self.h, int(nsteps), int(graph_steps)))
+
+    def energies(self):
+        pe, ke = np.zeros(self.n_rep), np.zeros(self.n_rep)
+        self.ctx.check(_L.mrmd_batch_energies(self.h, _p(pe), _p(ke)))
+        return pe, ke
+
+    def stats(self):
+        reb = np.zeros(self.n_rep, np.int64)
+        steps = C.c_int64()
+        self.ctx.check(_L.mrmd_batch_stats(self.h, _p(reb), C.byref(steps)))
+        return reb, steps.value
+
+    def tf(self, replica, which=0):
+        nb = int(np.ceil(self.replicas[replica]["box"][0] / self.replicas[replica]["tf_bin_width"]))
+        out = np.zeros(nb)
+        self.ctx.check(_L.mrmd_batch_tf_get(self.h, int(replica), int(which), _p(out), nb))
+        return out
+
+
+def adress_box(n, rho, rn=2.8, aspect=5.0):
+    """edges of an AdResS slab box of n atoms at density rho with Lx : Ly = aspect where the
+    short edges allow it (every edge at least 2 (rc + skin))"""
+    vol = n / rho
+    ly = max((vol / aspect) ** (1.0 / 3.0), 2.0 * rn + 1e-6)
+    lx = max(vol / (ly * ly), 2.0 * rn + 1e-6)
+    return [lx, ly, ly]
+
+
+def cubic_box(n, rho, rn=2.8):
+    edge = max((n / rho) ** (1.0 / 3.0), 2.0 * rn + 1e-6)
+    return [edge] * 3
+
+
+def replica(n, box, temperature, seed, adress=False, dt=0.005, gamma=1.0, tf_every=100):
+    r = dict(num_atoms=int(n), box=list(box), dt=dt, gamma=gamma, temperature=temperature, mass=1.0,
+             seed=int(seed), adress=1 if adress else 0, tf_every=tf_every if adress else 0,
+             atomistic_diameter=0.15 * box[0], hybrid_diameter=0.05 * box[0], nu=1, tf_bin_width=1.0,
+             tf_target_density=n / float(np.prod(box)), tf_sigma=2.0, tf_range=3.0, tf_modulation=1.0,
+             tf_symmetric=1, tf_periodic=1)
+    return r
+
+
+def ensemble_config(n_rep=256, n_min=500, n_max=50000, rho=(0.3, 0.9), temp=(0.7, 1.5), seed=2024,
+                    adress_fraction=0.5, rn=2.8):
+    """replica dicts, initial positions (uniform random in the box) and Maxwell-Boltzmann
+    velocities, all drawn from the seeded generator"""
+    g = rng(seed)
+    reps, pos, vel = [], [], []
+    for k in range(n_rep):
+        n = int(round(np.exp(g.uniform(np.log(n_min), np.log(n_max)))))
+        d = g.uniform(*rho)
+        t = g.uniform(*temp)
+        ad = k < int(round(adress_fraction * n_rep))
+        box = adress_box(n, d, rn) if ad else cubic_box(n, d, rn)
+        reps.append(replica(n, box, t, seed * 1000 + k, adress=ad))
+        pos.append(g.uniform(0.0, 1.0, (n, 3)) * np.asarray(box))
+        vel.append(maxwell_boltzmann(n, t, seed * 7919 + k))
+    return reps, pos, vel
