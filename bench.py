@@ -33,6 +33,20 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per force-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a device copy), or None"""
+    path = os.path.join(ROOT, "profiles", "lj_force_4M_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def build_system(M, ctx, cells, temperature, seed):
     from mrmd_amd.drivers import LJSystem
     from mrmd_amd.lattice import fcc, maxwell_boltzmann
@@ -67,13 +81,14 @@ def ensemble_bench(ctx, n_rep, steps, serial_steps, tune=True):
     (LJSystem / AdResSSystem).  Both use the capped potential (r_cap = 0.7) that keeps
     ideal-gas particles entering the hybrid region finite."""
     from mrmd_amd.drivers import AdResSSystem, LJSystem
-    from mrmd_amd.ensemble import Ensemble, ensemble_config
-    reps, pos, vel = ensemble_config(n_rep=n_rep, seed=2024)
+    from mrmd_amd.ensemble import Ensemble, ensemble_config, relax
+    reps, pos, _ = ensemble_config(n_rep=n_rep, seed=2024)
+    pos, vel = relax(ctx, reps, pos)  # capped relaxation of the uniform-random starts
     e = Ensemble(ctx, reps)
     e.set_lj(cap=0.7)
     e.upload(0, pos)
     e.upload(1, vel)
-    e.run(200, 10)  # relaxation of the uniform-random starts
+    e.run(100, 10)
     tuning = None
     if tune:
         best = None
@@ -147,6 +162,19 @@ def adjoint_ratio(ctx, reps=5):
     return {"atoms": int(pos.shape[0]), "forward_ms": tf / nf, "adjoint_ms": ta / na, "ratio": (ta / na) / (tf / nf)}
 
 
+def copy_calibration(mib):
+    """a plain device-to-device copy of known size, for scaling rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters to bytes"""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +189,12 @@ def main():
     ap.add_argument("--ensemble-replicas", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--serial-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -249,7 +282,8 @@ def main():
                    "density": 0.8442, "rc": 2.5, "skin": 0.3, "dt": dt, "temperature": 1.0, "gamma": 1.0,
                    "rebuilds_in_timed_steps": rebuilds, "tiling": tiling},
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic() if args.cells == 100 else None,
                      "kernel": "k_ljTiled (LJ force)", "avg_ms": avg_s * 1e3, "launches": launches,
                      "half_pairs_rn": half_rn, "half_pairs_rc": half_rc, "flop_per_launch": flops},
         "cpu_baseline": cpu,

@@ -10,7 +10,7 @@ time plus the build time spread over the expected steps between rebuilds.
 """
 import itertools
 
-DEFAULT_TILE_CELLS = (2, 3, 4, 5, 6, 8)
+DEFAULT_TILE_CELLS = (3, 4, 5, 6, 8)
 DEFAULT_BLOCKS = (128, 256)
 
 
@@ -19,7 +19,7 @@ def _stat(ctx, name):
     return t / n if n else 0.0
 
 
-def tune_lj_system(system, tile_cells=DEFAULT_TILE_CELLS, blocks=DEFAULT_BLOCKS, force_reps=3,
+def tune_lj_system(system, tile_cells=DEFAULT_TILE_CELLS, blocks=DEFAULT_BLOCKS, force_reps=6,
                    steps_per_rebuild=6.0, verbose=False):
     """pick (tile cells, workgroup size) for an LJSystem-like object with .ctx, .vl, .atoms,
     .lj and .rebuild(); leaves the best setting applied and the list rebuilt.  Returns

@@ -133,7 +133,7 @@ struct mrmd_verlet_list
 
     // ---- tiled form (built on the device; what the force kernels read) ----
     int tileK = 0;              ///< cells per tile along z (0 = choose from the density)
-    int block = 128;            ///< workgroup size of the tiled kernels
+    int block = 256;            ///< workgroup size of the tiled kernels
     int tilesPerCol = 0;
     int64_t ntiles = 0;
     int32_t maxStaged = 0;

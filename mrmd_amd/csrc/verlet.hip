@@ -313,13 +313,13 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     vl->bin.build(ctx, pos, 0, nall, g, true);
     vl->globalValid = false;
     vl->halfValid = false;
-    // tile height: aim at ~1200 staged records, shrink until every tile fits in LDS
+    // tile height: aim at ~1500 staged records, shrink until every tile fits in LDS
     const int R = vl->range, W = 2 * R + 1;
     int K = vl->tileK > 0 ? std::min(vl->tileK, g.n[2]) : 0;
     if (K <= 0)
     {
         const double perCell = double(std::max<int64_t>(nall, 1)) / double(g.total());
-        K = int(1200.0 / (W * W * std::max(perCell, 1e-3))) - 2 * R;
+        K = int(1500.0 / (W * W * std::max(perCell, 1e-3))) - 2 * R;
         K = std::max(1, std::min(K, g.n[2]));
     }
     K = std::min(K, MAX_SPAN - 2 * R);

@@ -128,6 +128,22 @@ def replica(n, box, temperature, seed, adress=False, dt=0.005, gamma=1.0, tf_eve
     return r
 
 
+def relax(ctx, reps, pos, steps=200, cap=0.7, gamma=50.0, seed=11, rc=2.5, skin=0.3):
+    """relax uniform-random starts with the capped potential under strong Langevin friction
+    (the overlap energy leaves within a few steps), then draw fresh Maxwell-Boltzmann
+    velocities at each replica's temperature; returns (positions, velocities)"""
+    strong = [dict(r, gamma=gamma) for r in reps]
+    e = Ensemble(ctx, strong, rc, skin)
+    e.set_lj(cap=cap)
+    e.upload(POS, pos)
+    e.upload(VEL, [np.zeros((r["num_atoms"], 3)) for r in reps])
+    e.run(steps, 10)
+    p = e.positions()
+    del e
+    vel = [maxwell_boltzmann(r["num_atoms"], r["temperature"], seed * 7919 + k) for k, r in enumerate(reps)]
+    return p, vel
+
+
 def ensemble_config(n_rep=256, n_min=500, n_max=50000, rho=(0.3, 0.9), temp=(0.7, 1.5), seed=2024,
                     adress_fraction=0.5, rn=2.8):
     """replica dicts, initial positions (uniform random in the box) and Maxwell-Boltzmann

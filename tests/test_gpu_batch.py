@@ -7,7 +7,7 @@
 * forces of plain and AdResS replicas equal the oracle's ghost-layer algorithms (half list,
   atomic partner forces; LJ_IdealGas with molecules and multi-resolution ghosts) to 1e-12;
 * a plain replica's trajectory follows the single-system GPU path (LJSystem);
-* temperatures after relaxation match the thermostat.
+* every replica reaches its thermostat temperature.
 """
 import numpy as np
 import pytest
@@ -62,10 +62,10 @@ M_POS, M_VEL = 0, 1
 @pytest.fixture(scope="module")
 def relaxed(ctx):
     """uniform-random starts relaxed 200 steps with the capped potential"""
-    reps, pos, vel = _replicas()
-    e = _ensemble(ctx, reps, pos, vel)
-    e.run(200, 10)
-    return reps, e.positions(), e.velocities()
+    from mrmd_amd.ensemble import relax
+    reps, pos, _ = _replicas()
+    pos, vel = relax(ctx, reps, pos)
+    return reps, pos, vel
 
 
 def test_batch_equals_single_runs(ctx, relaxed):
@@ -164,12 +164,19 @@ def test_plain_replica_follows_single_system_path(M, ctx, relaxed):
     assert np.abs(d).max() < 1e-9
 
 
-def test_relaxed_temperatures(ctx, relaxed):
+def test_thermostat_temperatures(ctx, relaxed):
+    """after 4 time units (the relaxed structure releases its excess potential energy,
+    which the thermostat removes with time constant 1/(2 gamma)) every replica sits at its
+    temperature; averaged over the last 200 steps"""
     reps, pos, vel = relaxed
     e = _ensemble(ctx, reps, pos, vel)
-    e.run(100, 10)
-    pe, ke = e.energies()
+    e.run(600, 10)
+    ts = []
+    for _ in range(10):
+        e.run(20, 10)
+        pe, ke = e.energies()
+        assert np.isfinite(pe).all()
+        ts.append(2.0 * ke / (3.0 * np.array([r["num_atoms"] for r in reps])))
+    t = np.mean(ts, axis=0)
     for k, r in enumerate(reps):
-        t = 2.0 * ke[k] / (3.0 * r["num_atoms"])
-        assert t == pytest.approx(r["temperature"], rel=0.12), k
-    assert np.isfinite(pe).all()
+        assert t[k] == pytest.approx(r["temperature"], rel=0.08), (k, t[k])
