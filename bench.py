@@ -211,7 +211,7 @@ def main():
     if not args.no_tune:
         from mrmd_amd.autotune import tune_lj_system
         best, _ = tune_lj_system(s)
-        tiling = {"tile_cells": best[0], "block": best[1]}
+        tiling = {"tile_columns": best[0], "tile_cells": best[1], "block": best[2]}
         for _ in range(3):
             s.step(dt)
     else:

@@ -255,12 +255,14 @@ int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* max_ful
 int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* atoms, double r, int64_t* half_pairs,
                              int64_t* full_entries);
 /* Launch shape of the tiled list and the tiled force kernel (no reference counterpart; the
- * reference's Cabana list has no tiling): tile_cells = cells per tile along z (0 = chosen
- * from the density at the next build), block = workgroup size (64, 128, 192 or 256).
- * Applies from the next build. */
-int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int block);
-/* the tiling in use: cells per tile, workgroup size, largest staged neighbourhood (records) */
-int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* block, int* max_staged);
+ * reference's Cabana list has no tiling): a tile is tile_cells cells along z (0 = chosen
+ * from the density at the next build) in tile_columns x tile_columns cell columns (1-3;
+ * 1 when the cell ratio is below 1); block = workgroup size (64, 128, 192 or 256).
+ * Applies from the next build; results do not depend on it. */
+int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_columns, int block);
+/* the tiling in use: cells per tile, columns, workgroup size, largest staged neighbourhood */
+int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* tile_columns, int* block,
+                           int* max_staged);
 
 /* =========================================================================================
  * Lennard-Jones  (action::LennardJones, mrmd/action/LennardJones.hpp:98-206)

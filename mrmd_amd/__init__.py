@@ -362,15 +362,15 @@ class HalfVerletList:
         self.ctx.check(_L.mrmd_verlet_info(self.h, C.byref(rows), C.byref(mx), C.byref(tot), _p(dims)))
         return dict(rows=rows.value, max_full=mx.value, total_full=tot.value, dims=dims)
 
-    def set_tiling(self, tile_cells=0, block=128):
-        """cells per tile along z (0 = from the density) and workgroup size of the tiled
-        list build and force kernel; applies from the next build"""
-        self.ctx.check(_L.mrmd_verlet_set_tiling(self.h, int(tile_cells), int(block)))
+    def set_tiling(self, tile_cells=0, block=256, tile_columns=1):
+        """cells per tile along z (0 = from the density), tile width in cell columns and
+        workgroup size of the tiled list build and force kernel; applies from the next build"""
+        self.ctx.check(_L.mrmd_verlet_set_tiling(self.h, int(tile_cells), int(tile_columns), int(block)))
 
     def tiling(self):
-        k, b, s = C.c_int(), C.c_int(), C.c_int()
-        self.ctx.check(_L.mrmd_verlet_get_tiling(self.h, C.byref(k), C.byref(b), C.byref(s)))
-        return dict(tile_cells=k.value, block=b.value, max_staged=s.value)
+        k, a, b, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(_L.mrmd_verlet_get_tiling(self.h, C.byref(k), C.byref(a), C.byref(b), C.byref(s)))
+        return dict(tile_cells=k.value, tile_columns=a.value, block=b.value, max_staged=s.value)
 
     def count_within(self, atoms, r):
         """(half-list pairs, both-direction entries) closer than r at the current positions"""

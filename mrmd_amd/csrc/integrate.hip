@@ -1,6 +1,8 @@
 // Velocity-Verlet and Langevin (BAOAB-style) integration and the kinetic energy.
 // Streaming kernels: one lane per local atom, coalesced SoA reads/writes, the maximum
 // displacement reduced with wave shuffles and one 64-bit atomic max per workgroup.
+#include <algorithm>
+
 #include "containers.hpp"
 #include "detmath.hpp"
 
@@ -20,9 +22,8 @@ __global__ void k_zero1(double* p) { *p = 0.0; }
 __global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double dt, double* maxSq)
 {
     __shared__ double red[BLOCK / WAVE];
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double d2 = 0.0;
-    if (i < n)
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     {
         const double dtHalf = 0.5 * dt;
         double4 p = a.pos[i];
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double 
         a.vel[a.cap + i] = vy;
         a.vel[2 * a.cap + i] = vz;
         const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
-        d2 = dx * dx + dy * dy + dz * dz;
+        d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
     }
     const double m = blockMax(d2, red);
     if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, m);
@@ -66,9 +67,8 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
                                                        uint64_t seed, uint64_t call, mrmd_pred1 pred, double* maxSq)
 {
     __shared__ double red[BLOCK / WAVE];
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double d2 = 0.0;
-    if (i < n)
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     {
         const double dtHalf = 0.5 * dt;
         double4 p = a.pos[i];
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
         a.vel[a.cap + i] = vy;
         a.vel[2 * a.cap + i] = vz;
         const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
-        d2 = dx * dx + dy * dy + dz * dz;
+        d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
     }
     const double mx = blockMax(d2, red);
     if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, mx);
@@ -136,6 +136,10 @@ __global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
 
 double* dispScalar(mrmd_atoms* a) { return static_cast<double*>(ctxScratch(a->ctx, 3, 64)); }
 
+// grid of the displacement-reducing integrators: enough workgroups to fill the chip, few
+// enough that the per-workgroup atomic maximum on one address stays cheap
+unsigned strideGrid(int64_t n) { return std::min<unsigned>(gridFor(n), 2048u); }
+
 void finishDisp(mrmd_atoms* a, double* maxDisp)
 {
     if (!maxDisp) return;
@@ -157,7 +161,7 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     if (a->nlocal > 0)
     {
         KTimer t(ctx, "vv_pre");
-        hipLaunchKernelGGL(k_vvPre, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, ms);
+        hipLaunchKernelGGL(k_vvPre, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, ms);
     }
     checkLaunch();
     finishDisp(a, maxDisp);
@@ -190,10 +194,10 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
         const bool usePred = pred && pred->kind != MRMD_PRED_ALWAYS;
         const mrmd_pred1 p = pred ? *pred : mrmd_pred1{};
         if (usePred)
-            hipLaunchKernelGGL(k_langevinPre<true>, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
+            hipLaunchKernelGGL(k_langevinPre<true>, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
                                a->nlocal, dt, zeta, temperature, seed, call, p, ms);
         else
-            hipLaunchKernelGGL(k_langevinPre<false>, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
+            hipLaunchKernelGGL(k_langevinPre<false>, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
                                a->nlocal, dt, zeta, temperature, seed, call, p, ms);
     }
     checkLaunch();

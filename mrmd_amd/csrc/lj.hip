@@ -33,7 +33,8 @@ struct mrmd_lj
     double rcSqr = 0.0;
     DBuf<Pre> dtable;
     DBuf<double> ev;  ///< energy, virial on the device
-    DBuf<double> partials;
+    DBuf<double> partials;   ///< per-workgroup (energy, virial) of the last apply
+    int64_t nPartials = 0;   ///< > 0: partials not yet summed into ev
     bool evValid = true;
     double hE = 0.0, hW = 0.0;
 
@@ -94,7 +95,7 @@ struct TiledList
 // (LennardJones.hpp:68-75) is evaluated too.
 template <bool PRED, bool ONE, bool CAP>
 __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, int ti0, const double4& pi,
-                               const double4* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
+                               const void* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
                                double& w)
 {
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -102,7 +103,8 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u)
     {
-        const double4 pj = st[(wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu];
+        // single-type tables stage {x, y, z} only
+        const double4 pj = Staged<ONE>::get(st, int((wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu));
         dx[u] = pi.x - pj.x;
         dy[u] = pi.y - pj.y;
         dz[u] = pi.z - pj.z;
@@ -151,11 +153,13 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
         for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
     if (threadIdx.x == 0) ti = tl.tiles[blk];
     __syncthreads();
-    stageTile(ti, tl.sortedPos, st);
+    stageTile<ONE>(ti, tl.sortedPos, st);
     __syncthreads();
     double e = 0.0, w = 0.0;
-    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
+        int self;
+        const int s = tileRow(ti, t, self);
         const int c = tl.cnt[s];
         const int32_t i = tl.sorted[s];
         if (c == 0 || i >= a.rows) continue;
@@ -267,112 +271,138 @@ __global__ void k_ljEval(const Pre* table, const double* d2, const int64_t* idx,
 //   dL/dsigma_ab = sum_{pairs ab} (g_i - g_j) . d ds/dsigma
 // With full rows every pair is seen from both ends: dL/dx_i is complete per row, and the
 // parameter sums take half of each row's term.  Capped pairs use the capped expression.
-template <bool FEW>
-__global__ __launch_bounds__(BLOCK) void k_ljAdjoint(LJArgs a, const uint32_t* __restrict__ nbr,
-                                                     const int32_t* __restrict__ cnt, const double* __restrict__ g,
-                                                     int64_t gstride, const double* __restrict__ sigma,
-                                                     const double* __restrict__ eps, double* dLdx, int64_t xstride,
-                                                     double* pgrad /*[2*nt2] per block region*/)
+constexpr int ADJ_BLOCK = 64;  // small workgroups: a 32k-atom snapshot still fills every CU
+constexpr int ADJ_LANES = 4;   // lanes per row: each takes every 4th neighbour, shuffles combine
+constexpr int ADJ_CHUNK = 2;   // neighbours a lane evaluates together (independent streams)
+
+template <bool FEW, bool CAP>
+__global__ __launch_bounds__(ADJ_BLOCK) void k_ljAdjoint(LJArgs a, const uint32_t* __restrict__ nbr,
+                                                         const int32_t* __restrict__ cnt,
+                                                         const double* __restrict__ g, int64_t gstride,
+                                                         const double* __restrict__ sigma,
+                                                         const double* __restrict__ eps, double* dLdx,
+                                                         int64_t xstride, double* pgrad /*[block][2*nt2]*/)
 {
     __shared__ Pre tab[MAX_TYPES2];
-    __shared__ double sg[MAX_TYPES2], se[MAX_TYPES2];
+    __shared__ double isg[MAX_TYPES2], ise[MAX_TYPES2];
     __shared__ double acc[2 * MAX_TYPES2];
     for (int t = threadIdx.x; t < a.nt2; t += blockDim.x)
     {
         tab[t] = a.table[t];
-        sg[t] = sigma[t];
-        se[t] = eps[t];
+        isg[t] = 1.0 / sigma[t];
+        ise[t] = 1.0 / eps[t];
         acc[t] = 0.0;
         acc[MAX_TYPES2 + t] = 0.0;
     }
     __syncthreads();
     // FEW: up to 4 type pairs accumulate in registers, otherwise in LDS with atomics
     double re[4] = {0.0, 0.0, 0.0, 0.0}, rs[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t r = int64_t(xcdBlock()) * blockDim.x + threadIdx.x;
+    const int64_t gt = int64_t(xcdBlock()) * blockDim.x + threadIdx.x;
+    const int64_t r = gt / ADJ_LANES;
+    const int sub = int(gt % ADJ_LANES);
+    double ox = 0.0, oy = 0.0, oz = 0.0;
     if (r < a.rows)
     {
         const int64_t i = a.begin + r;
         const double4 pi = a.pos[i];
         const int ti = int(pi.w) * a.typeStride;
         const double gix = g[r], giy = g[gstride + r], giz = g[2 * gstride + r];
-        double ox = 0.0, oy = 0.0, oz = 0.0;
         const int c = cnt[r];
-        for (int k = 0; k < c; ++k)
+        for (int k0 = sub; k0 < c; k0 += ADJ_LANES * ADJ_CHUNK)
         {
-            const uint32_t j = nbr[int64_t(k) * a.stride + r] & 0x7fffffffu;
-            const double4 pj = a.pos[j];
-            const double dx = pi.x - pj.x;
-            const double dy = pi.y - pj.y;
-            const double dz = pi.z - pj.z;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 > a.rcSqr) continue;
-            const int tIdx = ti + int(pj.w);
-            const Pre& p = tab[tIdx];
-            // upstream of the partner: local atoms carry their own row index, ghosts map to
-            // the real atom through the sign-free index in pj (handled by the caller's g)
-            const int64_t jr = int64_t(j) - a.begin;
-            const double gjx = g[jr], gjy = g[gstride + jr], gjz = g[2 * gstride + jr];
-            const double ux = gix - gjx, uy = giy - gjy, uz = giz - gjz;
-            const double du = dx * ux + dy * uy + dz * uz;
-            double s, sp, dsde, dsds;
-            if (d2 >= p.capDistSqr)
+            // entries past the row end read the row itself: d^2 = 0 selects zero below
+            int64_t jr[ADJ_CHUNK];
+            double4 pj[ADJ_CHUNK];
+#pragma unroll
+            for (int u = 0; u < ADJ_CHUNK; ++u)
             {
+                const int k = k0 + u * ADJ_LANES;
+                const int64_t j = k < c ? int64_t(nbr[int64_t(k) * a.stride + r] & 0x7fffffffu) : i;
+                jr[u] = j - a.begin;
+                pj[u] = a.pos[j];
+            }
+#pragma unroll
+            for (int u = 0; u < ADJ_CHUNK; ++u)
+            {
+                const double dx = pi.x - pj[u].x;
+                const double dy = pi.y - pj[u].y;
+                const double dz = pi.z - pj[u].z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const bool in = d2 <= a.rcSqr && d2 > 0.0;
+                const int tIdx = ti + int(pj[u].w);
+                const Pre& p = tab[tIdx];
+                // upstream of the partner: ghosts carry their real atom's upstream (expanded g)
+                const double ux = gix - g[jr[u]], uy = giy - g[gstride + jr[u]], uz = giz - g[2 * gstride + jr[u]];
+                const double du = dx * ux + dy * uy + dz * uz;
                 const double f2 = 1.0 / d2;
                 const double f6 = f2 * f2 * f2;
-                s = f6 * (p.ff1 * f6 - p.ff2) * f2;
+                double sv = f6 * (p.ff1 * f6 - p.ff2) * f2;
                 // ds/d(r^2) = -(7 ff1 r^-16 - 4 ff2 r^-10)
-                sp = -(7.0 * p.ff1 * f6 * f6 * f2 * f2 - 4.0 * p.ff2 * f6 * f2 * f2);
-                dsde = s / se[tIdx];
-                dsds = f6 * (12.0 * p.ff1 * f6 - 6.0 * p.ff2) * f2 / sg[tIdx];
-            }
-            else
-            {
-                const double dist = sqrt(d2);
-                s = p.capCoeff / dist;
-                sp = -0.5 * p.capCoeff / (dist * d2);
-                // capCoeff = s(rcap) rcap is linear in eps; its sigma-derivative follows s
-                const double rc2 = p.capDistSqr;
-                const double f2 = 1.0 / rc2;
-                const double f6 = f2 * f2 * f2;
-                dsde = s / se[tIdx];
-                dsds = (f6 * (12.0 * p.ff1 * f6 - 6.0 * p.ff2) * f2 / sg[tIdx]) * p.capDist / dist;
-            }
-            const double two_sp_du = 2.0 * sp * du;
-            ox += s * ux + two_sp_du * dx;
-            oy += s * uy + two_sp_du * dy;
-            oz += s * uz + two_sp_du * dz;
-            const double h = 0.5 * du;
-            if (FEW)
-            {
+                double sp = -(7.0 * p.ff1 * f6 * f6 * f2 * f2 - 4.0 * p.ff2 * f6 * f2 * f2);
+                double dsds = f6 * (12.0 * p.ff1 * f6 - 6.0 * p.ff2) * f2 * isg[tIdx];
+                if (CAP && d2 < p.capDistSqr)
+                {
+                    const double dist = sqrt(d2);
+                    sv = p.capCoeff / dist;
+                    sp = -0.5 * p.capCoeff / (dist * d2);
+                    // capCoeff = s(rcap) rcap is linear in eps; its sigma-derivative follows s
+                    const double c2 = 1.0 / p.capDistSqr;
+                    const double c6 = c2 * c2 * c2;
+                    dsds = (c6 * (12.0 * p.ff1 * c6 - 6.0 * p.ff2) * c2 * isg[tIdx]) * p.capDist / dist;
+                }
+                sv = in ? sv : 0.0;
+                sp = in ? sp : 0.0;
+                dsds = in ? dsds : 0.0;
+                const double dsde = sv * ise[tIdx];
+                const double two_sp_du = 2.0 * sp * du;
+                ox += sv * ux + two_sp_du * dx;
+                oy += sv * uy + two_sp_du * dy;
+                oz += sv * uz + two_sp_du * dz;
+                const double h = 0.5 * du;
+                if (FEW)
+                {
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (q == tIdx)
+                    for (int q = 0; q < 4; ++q)
                     {
-                        re[q] += h * dsde;
-                        rs[q] += h * dsds;
+                        re[q] += q == tIdx ? h * dsde : 0.0;
+                        rs[q] += q == tIdx ? h * dsds : 0.0;
                     }
-            }
-            else
-            {
-                atomicAdd(&acc[tIdx], h * dsde);
-                atomicAdd(&acc[MAX_TYPES2 + tIdx], h * dsds);
+                }
+                else if (in)
+                {
+                    atomicAdd(&acc[tIdx], h * dsde);
+                    atomicAdd(&acc[MAX_TYPES2 + tIdx], h * dsds);
+                }
             }
         }
+    }
+    // the row's lanes are adjacent in the wave: combine their partial sums
+#pragma unroll
+    for (int off = 1; off < ADJ_LANES; off <<= 1)
+    {
+        ox += __shfl_xor(ox, off, WAVE);
+        oy += __shfl_xor(oy, off, WAVE);
+        oz += __shfl_xor(oz, off, WAVE);
+    }
+    if (r < a.rows && sub == 0)
+    {
         dLdx[r] = ox;
         dLdx[xstride + r] = oy;
         dLdx[2 * xstride + r] = oz;
     }
+    // per-workgroup partial parameter gradients, summed in order by k_sumPartials
+    double* part = pgrad + int64_t(blockIdx.x) * 2 * a.nt2;
     if (FEW)
     {
-        __shared__ double red[BLOCK / WAVE];
+        __shared__ double red[ADJ_BLOCK / WAVE];
         for (int q = 0; q < a.nt2; ++q)
         {
             const double se_ = blockSum(re[q], red);
             const double ss_ = blockSum(rs[q], red);
             if (threadIdx.x == 0)
             {
-                atomicAddD(&pgrad[q], se_);
-                atomicAddD(&pgrad[a.nt2 + q], ss_);
+                part[q] = se_;
+                part[a.nt2 + q] = ss_;
             }
         }
         return;
@@ -380,8 +410,8 @@ __global__ __launch_bounds__(BLOCK) void k_ljAdjoint(LJArgs a, const uint32_t* _
     __syncthreads();
     for (int t = threadIdx.x; t < a.nt2; t += blockDim.x)
     {
-        atomicAddD(&pgrad[t], acc[t]);
-        atomicAddD(&pgrad[a.nt2 + t], acc[MAX_TYPES2 + t]);
+        part[t] = acc[t];
+        part[a.nt2 + t] = acc[MAX_TYPES2 + t];
     }
 }
 
@@ -519,7 +549,8 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
     mrmd_ctx* ctx = lj->ctx;
     const bool full = lj->symmetric && predIsSymmetric(pred);
     const unsigned grid = full ? unsigned(std::max<int64_t>(vl->ntiles, 1)) : gridFor(rows);
-    double* partials = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid) * 2 * sizeof(double)));
+    lj->partials.reserve(ctx, int64_t(grid) * 2);
+    double* partials = lj->partials.p;
     LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
              int(lj->typeStride), lj->rcSqr, pred ? *pred : mrmd_pred2{}, pred && pred->op != MRMD_PAIR_ALWAYS};
     if (rows > 0)
@@ -531,10 +562,11 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
                                vl->bin.sortedPos.p};
-            const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
-            const dim3 g(grid), b(vl->block);
             const Pre p1 = lj->table[0];
             const bool one = lj->table.size() == 1;
+            const size_t lds =
+                size_t(std::max(vl->maxStaged, 1)) * (one ? Staged<true>::bytes : Staged<false>::bytes);
+            const dim3 g(grid), b(vl->block);
             bool cap = false;
             for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
             auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, partials); };
@@ -563,10 +595,14 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
                 hipLaunchKernelGGL(k_ljHalf<false>, dim3(grid), dim3(BLOCK), 0, ctx->stream, a, vl->hnbr.p, vl->hcnt.p,
                                    partials);
         }
-        hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, partials, int64_t(grid), 2, lj->ev.p);
+        // summed when energy or virial is requested (most steps never ask)
+        lj->nPartials = grid;
     }
     else
+    {
         hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, ctx->stream, lj->ev.p, int64_t(2));
+        lj->nPartials = 0;
+    }
     checkLaunch();
     lj->evValid = false;
     MRMD_API_END
@@ -578,6 +614,11 @@ int mrmd_lj_energy_virial(mrmd_lj* lj, double* e, double* w)
     REQUIRE(lj, "null lj");
     if (!lj->evValid)
     {
+        if (lj->nPartials > 0)
+            hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, lj->ctx->stream, lj->partials.p, lj->nPartials, 2,
+                               lj->ev.p);
+        checkLaunch();
+        lj->nPartials = 0;
         HIPCHECK(hipMemcpyAsync(lj->ctx->pinned, lj->ev.p, 16, hipMemcpyDeviceToHost, lj->ctx->stream));
         sync(lj->ctx);
         lj->hE = lj->ctx->pinned[0];
@@ -628,7 +669,9 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     DBuf<double> g, out, pg, sd;
     g.reserve(ctx, 3 * gstride);
     out.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
-    pg.reserve(ctx, 2 * nt2);
+    const unsigned adjGrid = gridFor(std::max<int64_t>(rows, 1) * ADJ_LANES, ADJ_BLOCK);
+    pg.reserve(ctx, int64_t(adjGrid + 1) * 2 * nt2);
+    double* pgSum = pg.p + int64_t(adjGrid) * 2 * nt2;
     sd.reserve(ctx, 2 * nt2);
     HIPCHECK(hipMemcpyAsync(sd.p, sig.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpyAsync(sd.p + nt2, ep.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -644,18 +687,26 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
                                mrmd_hip::ghostCorrDevice(gl), nl, nall);
         sync(ctx);
     }
-    hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(BLOCK), 0, ctx->stream, pg.p, int64_t(2 * nt2));
+    hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(BLOCK), 0, ctx->stream, pgSum, int64_t(2 * nt2));
     LJArgs a{at->pos.p, nullptr, at->cap, 0, rows, vl->stride, lj->dtable.p, nt2, int(lj->typeStride), lj->rcSqr,
              mrmd_pred2{}, 0};
     if (rows > 0)
     {
         KTimer t(ctx, "lj_adjoint");
+        bool cap = false;
+        for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
+        const dim3 grid(adjGrid), blk(ADJ_BLOCK);
+        const int64_t xs = std::max<int64_t>(nl, 1);
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, blk, 0, ctx->stream, a, vl->nbr.p, vl->cnt.p, g.p, gstride, sd.p,
+                               sd.p + nt2, out.p, xs, pg.p);
+        };
         if (nt2 <= 4)
-            hipLaunchKernelGGL(k_ljAdjoint<true>, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p,
-                               vl->cnt.p, g.p, gstride, sd.p, sd.p + nt2, out.p, std::max<int64_t>(nl, 1), pg.p);
+            cap ? launch(k_ljAdjoint<true, true>) : launch(k_ljAdjoint<true, false>);
         else
-            hipLaunchKernelGGL(k_ljAdjoint<false>, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, a, vl->nbr.p,
-                               vl->cnt.p, g.p, gstride, sd.p, sd.p + nt2, out.p, std::max<int64_t>(nl, 1), pg.p);
+            cap ? launch(k_ljAdjoint<false, true>) : launch(k_ljAdjoint<false, false>);
+        hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, pg.p, int64_t(adjGrid), 2 * nt2,
+                           pgSum);
     }
     checkLaunch();
     // outputs
@@ -666,7 +717,7 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     HIPCHECK(hipMemcpyAsync(dLdx, rm.p, size_t(3 * nl) * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                             ctx->stream));
     std::vector<double> hp(static_cast<size_t>(2 * nt2));
-    HIPCHECK(hipMemcpyAsync(hp.data(), pg.p, size_t(2 * nt2) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(hp.data(), pgSum, size_t(2 * nt2) * 8, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
     // symmetrise: the parameter of pair {a,b} is shared by table entries (a,b) and (b,a)
     const int64_t n = lj->numTypes, st = lj->typeStride;

@@ -72,18 +72,30 @@ struct mrmd_cell_list
 
 namespace mrmd_hip
 {
-constexpr int MAX_RANGES = 25;  // (2R+1)^2 stencil columns for R <= 2
+constexpr int MAX_RANGES = 25;  // (a + 2R)^2 staged columns: a <= 3 with R = 1, a = 1 with R = 2
+constexpr int MAX_SELF = 9;     // a x a columns of rows
 constexpr int STAGE_MAX = 2048; // staged records per tile (double4 -> 64 KB of LDS)
 
-// One tile = a run of K cells along z in one (i, j) column.  Its rows are the contiguous
-// sorted slots [rowBegin, rowEnd); its neighbourhood is the (2R+1)^2 columns around it,
-// each a contiguous range of sorted slots, staged back to back in LDS at base[r].
+// One tile = a run of K cells along z in a x a neighbouring (i, j) columns.  Its rows are
+// the sorted slots of those cells (one contiguous run per column); its neighbourhood is
+// the (a + 2R)^2 columns around them, each a contiguous range of sorted slots (empty
+// outside the grid), staged back to back in LDS at base[r].
 struct TileInfo
 {
-    int32_t rowBegin, rowEnd, nr, staged;
+    int32_t nr, staged, nself, nrows;
     int32_t begin[MAX_RANGES];
     int32_t base[MAX_RANGES];
+    int32_t selfRange[MAX_SELF], selfBegin[MAX_SELF], selfPrefix[MAX_SELF + 1];
 };
+
+// sorted slot of the t-th row of a tile (its columns' rows one column after the other)
+__device__ inline int tileRow(const TileInfo& ti, int t, int& self)
+{
+    int r = 0;
+    while (r + 1 < ti.nself && ti.selfPrefix[r + 1] <= t) ++r;
+    self = r;
+    return ti.selfBegin[r] + (t - ti.selfPrefix[r]);
+}
 
 // Neighbour entries are 16-bit LDS slots stored in chunks of 8 per row: entry k of sorted
 // slot s sits at ((k / 8) * strideS + s) * 8 + k % 8, so one 16-byte load per lane brings
@@ -94,9 +106,40 @@ __host__ __device__ inline int64_t entryIndex(int64_t k, int64_t s, int64_t stri
     return ((k >> 3) * strideS + s) * CHUNK + (k & 7);
 }
 
+// A staged LDS record: {x, y, z} (24 bytes) where the kernel needs positions only, the full
+// {x, y, z, type} record (32 bytes) where it needs the type.  The smaller record fits more
+// workgroups into a CU's LDS.
+template <bool XYZ>
+struct Staged;
+template <>
+struct Staged<true>
+{
+    static constexpr int bytes = 24;
+    __device__ static void put(void* st, int q, const double4& v)
+    {
+        double* p = static_cast<double*>(st) + 3 * q;
+        p[0] = v.x;
+        p[1] = v.y;
+        p[2] = v.z;
+    }
+    __device__ static double4 get(const void* st, int q)
+    {
+        const double* p = static_cast<const double*>(st) + 3 * q;
+        return make_double4(p[0], p[1], p[2], 0.0);
+    }
+};
+template <>
+struct Staged<false>
+{
+    static constexpr int bytes = 32;
+    __device__ static void put(void* st, int q, const double4& v) { static_cast<double4*>(st)[q] = v; }
+    __device__ static double4 get(const void* st, int q) { return static_cast<const double4*>(st)[q]; }
+};
+
 // copy the tile's neighbourhood (contiguous runs of sorted slots) into LDS; the copy is
 // unrolled so that four independent loads per lane are in flight
-__device__ inline void stageTile(const TileInfo& ti, const double4* __restrict__ sortedPos, double4* st)
+template <bool XYZ>
+__device__ inline void stageTile(const TileInfo& ti, const double4* __restrict__ sortedPos, void* st)
 {
     int r = 0;
     const int step = blockDim.x;
@@ -113,10 +156,10 @@ __device__ inline void stageTile(const TileInfo& ti, const double4* __restrict__
         if (i1 >= 0) v1 = sortedPos[i1];
         if (i2 >= 0) v2 = sortedPos[i2];
         if (i3 >= 0) v3 = sortedPos[i3];
-        st[q0] = v0;
-        if (i1 >= 0) st[q0 + step] = v1;
-        if (i2 >= 0) st[q0 + 2 * step] = v2;
-        if (i3 >= 0) st[q0 + 3 * step] = v3;
+        Staged<XYZ>::put(st, q0, v0);
+        if (i1 >= 0) Staged<XYZ>::put(st, q0 + step, v1);
+        if (i2 >= 0) Staged<XYZ>::put(st, q0 + 2 * step, v2);
+        if (i3 >= 0) Staged<XYZ>::put(st, q0 + 3 * step, v3);
     }
 }
 }  // namespace mrmd_hip
@@ -133,6 +176,7 @@ struct mrmd_verlet_list
 
     // ---- tiled form (built on the device; what the force kernels read) ----
     int tileK = 0;              ///< cells per tile along z (0 = choose from the density)
+    int tileCols = 2;           ///< a: tiles span a x a cell columns
     int block = 256;            ///< workgroup size of the tiled kernels
     int tilesPerCol = 0;
     int64_t ntiles = 0;

@@ -2,11 +2,11 @@
 // tiled, LDS-staged form.
 //
 // All records (local and ghost) are binned on Cabana's grid and sorted by cell.  A tile
-// is a run of K cells along z in one (i, j) column; its rows are a contiguous range of
-// sorted slots, and every cell of its 27-cell stencils lies in the (2R+1)^2 columns around
-// it, each again a contiguous slot range.  One workgroup per tile copies those ranges into
-// LDS once and every row scans its stencil cells (Cabana's i, j, k order and min-distance
-// prune) from LDS.  An entry is the 15-bit LDS slot of the neighbour plus a flag bit for
+// is a run of K cells along z in a x a neighbouring (i, j) columns; its rows are one
+// contiguous range of sorted slots per column, and every cell of its stencils lies in the
+// (a + 2R)^2 columns around it, each again a contiguous slot range.  One workgroup per tile
+// copies those ranges into LDS once and every row scans its stencil cells (Cabana's i, j, k
+// order and min-distance prune) from LDS.  An entry is the 15-bit LDS slot of the neighbour plus a flag bit for
 // the pairs Cabana's position-lexicographic half discriminator keeps, so the exact half
 // list is a sub-list and the force kernels stage the same tile and never gather from
 // global memory in their inner loop.  Global-index rows (for the API, the adjoint and the
@@ -22,46 +22,65 @@ namespace
 {
 struct TileGeom
 {
-    int n0, n1, n2, K, tilesPerCol, R;
+    int n0, n1, n2, K, tilesPerCol, R, cols, nbj;
 };
 
-__device__ inline void tileCoords(const TileGeom& tg, int64_t t, int& i, int& j, int& k0, int& k1)
+// first column (i0, j0) and z-run [k0, k1) of tile t
+__device__ inline void tileCoords(const TileGeom& tg, int64_t t, int& i0, int& j0, int& k0, int& k1)
 {
     const int col = int(t / tg.tilesPerCol);
     const int kb = int(t % tg.tilesPerCol);
-    i = col / tg.n1;
-    j = col % tg.n1;
+    i0 = (col / tg.nbj) * tg.cols;
+    j0 = (col % tg.nbj) * tg.cols;
     k0 = kb * tg.K;
     k1 = min(k0 + tg.K, tg.n2);
 }
 
-// one thread per tile: row range, stencil column ranges and their LDS bases
+__device__ inline int64_t tileOfCell(const TileGeom& tg, int cell)
+{
+    const int k = cell % tg.n2, ij = cell / tg.n2, i = ij / tg.n1, j = ij % tg.n1;
+    return (int64_t(i / tg.cols) * tg.nbj + j / tg.cols) * tg.tilesPerCol + k / tg.K;
+}
+
+// one thread per tile: row ranges, neighbourhood column ranges and their LDS bases
 __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets, const int32_t* counts,
                             TileInfo* tiles, int32_t* maxStaged)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
-    int i, j, k0, k1;
-    tileCoords(tg, t, i, j, k0, k1);
+    int i0, j0, k0, k1;
+    tileCoords(tg, t, i0, j0, k0, k1);
     TileInfo& ti = tiles[t];
-    const int cFirst = (i * tg.n1 + j) * tg.n2 + k0;
-    const int cLast = (i * tg.n1 + j) * tg.n2 + (k1 - 1);
-    ti.rowBegin = offsets[cFirst];
-    ti.rowEnd = offsets[cLast] + counts[cLast];
+    const int span = tg.cols + 2 * tg.R;
     const int kl = max(k0 - tg.R, 0), kh = min(k1 + tg.R, tg.n2);
-    int nr = 0, staged = 0;
-    for (int ii = max(i - tg.R, 0); ii < min(i + tg.R + 1, tg.n0); ++ii)
-        for (int jj = max(j - tg.R, 0); jj < min(j + tg.R + 1, tg.n1); ++jj)
+    int nr = 0, staged = 0, nself = 0, nrows = 0;
+    for (int a = 0; a < span; ++a)
+        for (int b = 0; b < span; ++b, ++nr)
         {
+            const int ii = i0 - tg.R + a, jj = j0 - tg.R + b;
+            ti.base[nr] = staged;
+            ti.begin[nr] = 0;
+            if (ii < 0 || ii >= tg.n0 || jj < 0 || jj >= tg.n1) continue;
             const int c0 = (ii * tg.n1 + jj) * tg.n2 + kl;
             const int c1 = (ii * tg.n1 + jj) * tg.n2 + (kh - 1);
             ti.begin[nr] = offsets[c0];
-            ti.base[nr] = staged;
             staged += offsets[c1] + counts[c1] - offsets[c0];
-            ++nr;
+            if (ii >= i0 && ii < i0 + tg.cols && jj >= j0 && jj < j0 + tg.cols)
+            {
+                const int r0 = (ii * tg.n1 + jj) * tg.n2 + k0;
+                const int r1 = (ii * tg.n1 + jj) * tg.n2 + (k1 - 1);
+                ti.selfRange[nself] = nr;
+                ti.selfBegin[nself] = offsets[r0];
+                ti.selfPrefix[nself] = nrows;
+                nrows += offsets[r1] + counts[r1] - offsets[r0];
+                ++nself;
+            }
         }
+    ti.selfPrefix[nself] = nrows;
     ti.nr = nr;
     ti.staged = staged;
+    ti.nself = nself;
+    ti.nrows = nrows;
     atomicMax(maxStaged, staged);
 }
 
@@ -70,66 +89,73 @@ __device__ inline int ldsSlot(const TileInfo& ti, int r, int s) { return ti.base
 
 constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
 
+// One lane per row.  Every row scans its stencil cells in Cabana's i, j, k order with the
+// min-distance prune, four candidates per pass (independent LDS reads and distances, then
+// the appends in slot order).
 template <int R>
-__global__ void k_verletTiled(const double4* __restrict__ sortedPos, const int32_t* __restrict__ sorted,
-                              const int32_t* __restrict__ cellOf, const int32_t* __restrict__ offsets,
-                              const TileInfo* __restrict__ tiles, TileGeom tg, GridD g, int64_t rowLo,
-                              int64_t rowHi, double rsq, uint16_t* nbr, int64_t strideS, int cap, int32_t* cnt,
-                              int32_t* maxCnt)
+__global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict__ sortedPos,
+                                                       const int32_t* __restrict__ sorted,
+                                                       const int32_t* __restrict__ cellOf,
+                                                       const int32_t* __restrict__ offsets,
+                                                       const TileInfo* __restrict__ tiles, TileGeom tg, GridD g,
+                                                       int64_t rowLo, int64_t rowHi, double rsq, uint16_t* nbr,
+                                                       int64_t strideS, int cap, int32_t* cnt, int32_t* maxCnt)
 {
-    extern __shared__ double4 st[];
+    extern __shared__ double4 stRaw[];
+    void* st = stRaw;
     __shared__ TileInfo ti;
-    // LDS slot of the first record of cell kl + x of range r: cstart[r * (MAX_SPAN + 1) + x];
-    // entry x = nk closes the range
+    // LDS slot of the first record of cell kl + z of range r: cstart[r * (MAX_SPAN + 1) + z];
+    // entry z = nk closes the range (ranges outside the grid are empty)
     __shared__ int cstart[MAX_RANGES * (MAX_SPAN + 1)];
     const int64_t t = blockIdx.x;
     if (threadIdx.x == 0) ti = tiles[t];
     __syncthreads();
-    if (ti.rowEnd <= ti.rowBegin) return;
-    int i, j, k0, k1;
-    tileCoords(tg, t, i, j, k0, k1);
-    const int iLo = max(i - R, 0), jLo = max(j - R, 0), jCount = min(j + R + 1, tg.n1) - jLo;
+    if (ti.nrows == 0) return;
+    int i0, j0, k0, k1;
+    tileCoords(tg, t, i0, j0, k0, k1);
+    const int span = tg.cols + 2 * R;
     const int kl = max(k0 - R, 0), nk = min(k1 + R, tg.n2) - kl;
-    stageTile(ti, sortedPos, st);
+    stageTile<true>(ti, sortedPos, st);
     for (int x = threadIdx.x; x < ti.nr * (nk + 1); x += blockDim.x)
     {
         const int r = x / (nk + 1), z = x % (nk + 1);
-        int v;
-        if (z < nk)
+        const int ii = i0 - R + r / span, jj = j0 - R + r % span;
+        int v = ti.base[r];
+        if (ii >= 0 && ii < tg.n0 && jj >= 0 && jj < tg.n1)
         {
-            const int ii = iLo + r / jCount, jj = jLo + r % jCount;
-            v = ti.base[r] + (offsets[(ii * tg.n1 + jj) * tg.n2 + kl + z] - ti.begin[r]);
+            if (z < nk)
+                v = ti.base[r] + (offsets[(ii * tg.n1 + jj) * tg.n2 + kl + z] - ti.begin[r]);
+            else
+                v = r + 1 < ti.nr ? ti.base[r + 1] : ti.staged;
         }
-        else
-            v = r + 1 < ti.nr ? ti.base[r + 1] : ti.staged;
         cstart[r * (MAX_SPAN + 1) + z] = v;
     }
     __syncthreads();
-    // the tile's own column is the range with ii == i, jj == j
-    const int self = (i - iLo) * jCount + (j - jLo);
     int localMax = 0;
-    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    for (int row = threadIdx.x; row < ti.nrows; row += blockDim.x)
     {
+        int self;
+        const int s = tileRow(ti, row, self);
         const int32_t p = sorted[s];
         if (p < rowLo || p >= rowHi)
         {
             cnt[s] = 0;
             continue;
         }
-        const int qSelf = ldsSlot(ti, self, s);
-        const double4 xp = st[qSelf];
-        const int ck = cellOf[p] % tg.n2;
+        const int qSelf = ldsSlot(ti, ti.selfRange[self], s);
+        const double4 xp = Staged<true>::get(st, qSelf);
+        const int cell = cellOf[p];
+        const int ck = cell % tg.n2, i = (cell / tg.n2) / tg.n1, j = (cell / tg.n2) % tg.n1;
         int c = 0;
-        int r = 0;
-        for (int ii = iLo; ii < min(i + R + 1, tg.n0); ++ii)
-            for (int jj = jLo; jj < jLo + jCount; ++jj, ++r)
+        for (int ii = max(i - R, 0); ii < min(i + R + 1, tg.n0); ++ii)
+            for (int jj = max(j - R, 0); jj < min(j + R + 1, tg.n1); ++jj)
+            {
+                const int* cs0 = cstart + ((ii - (i0 - R)) * span + (jj - (j0 - R))) * (MAX_SPAN + 1);
                 for (int kk = max(ck - R, 0); kk < min(ck + R + 1, tg.n2); ++kk)
                 {
                     if (g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kk) > rsq) continue;
-                    const int* cs = cstart + r * (MAX_SPAN + 1) + (kk - kl);
+                    const int* cs = cs0 + (kk - kl);
                     const int q1 = cs[1];
-                    // four candidates per pass: independent LDS reads and distances, then
-                    // the appends in ascending slot order
                     for (int q = cs[0]; q < q1; q += 4)
                     {
                         double4 xn[4];
@@ -138,7 +164,7 @@ __global__ void k_verletTiled(const double4* __restrict__ sortedPos, const int32
                         for (int u = 0; u < 4; ++u)
                         {
                             const bool valid = q + u < q1 && q + u != qSelf;
-                            xn[u] = st[valid ? q + u : qSelf];
+                            xn[u] = Staged<true>::get(st, valid ? q + u : qSelf);
                             const double dx = xp.x - xn[u].x;
                             const double dy = xp.y - xn[u].y;
                             const double dz = xp.z - xn[u].z;
@@ -156,6 +182,7 @@ __global__ void k_verletTiled(const double4* __restrict__ sortedPos, const int32
                             }
                     }
                 }
+            }
         // pad the last chunk with the row's own slot: d^2 = 0 entries that the force
         // kernels mask, so their chunk loop needs no bounds test
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
@@ -176,10 +203,7 @@ __global__ void k_toGlobal(const uint16_t* nbr16, const int32_t* cnt16, int64_t 
     const int32_t p = sorted[s];
     if (p < rowLo || p >= rowHi) return;
     const int64_t r = p - rowLo;
-    const int cell = cellOf[p];
-    const int col = cell / tg.n2, kk = cell % tg.n2;
-    const int64_t t = int64_t(col) * tg.tilesPerCol + kk / tg.K;
-    const TileInfo& ti = tiles[t];
+    const TileInfo& ti = tiles[tileOfCell(tg, cellOf[p])];
     const int c = cnt16[s];
     for (int k = 0; k < c; ++k)
     {
@@ -269,7 +293,8 @@ __global__ void k_rowsToHost(const uint32_t* nbr, const int32_t* cnt, int64_t nr
 TileGeom geom(const mrmd_verlet_list* vl)
 {
     const GridD& g = vl->bin.grid;
-    return TileGeom{g.n[0], g.n[1], g.n[2], vl->tileK, vl->tilesPerCol, vl->range};
+    return TileGeom{g.n[0], g.n[1], g.n[2], vl->tileK, vl->tilesPerCol, vl->range, vl->tileCols,
+                    (g.n[1] + vl->tileCols - 1) / vl->tileCols};
 }
 
 void setupTiles(mrmd_verlet_list* vl, int K)
@@ -278,7 +303,8 @@ void setupTiles(mrmd_verlet_list* vl, int K)
     const GridD& g = vl->bin.grid;
     vl->tileK = K;
     vl->tilesPerCol = (g.n[2] + K - 1) / K;
-    vl->ntiles = int64_t(g.n[0]) * g.n[1] * vl->tilesPerCol;
+    const int a = vl->tileCols;
+    vl->ntiles = int64_t((g.n[0] + a - 1) / a) * ((g.n[1] + a - 1) / a) * vl->tilesPerCol;
     vl->tiles.reserve(ctx, std::max<int64_t>(vl->ntiles, 1));
     int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
     hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
@@ -313,24 +339,31 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     vl->bin.build(ctx, pos, 0, nall, g, true);
     vl->globalValid = false;
     vl->halfValid = false;
-    // tile height: aim at ~1500 staged records, shrink until every tile fits in LDS
-    const int R = vl->range, W = 2 * R + 1;
+    // tile shape: a x a columns (a <= 3; 1 when the stencil reaches two cells), height K
+    // chosen to stage ~1650 records; shrink until every tile fits in LDS
+    const int R = vl->range;
+    int a = std::max(1, std::min(vl->tileCols, R == 1 ? 3 : 1));
+    const int W = a + 2 * R;
     int K = vl->tileK > 0 ? std::min(vl->tileK, g.n[2]) : 0;
     if (K <= 0)
     {
         const double perCell = double(std::max<int64_t>(nall, 1)) / double(g.total());
-        K = int(1500.0 / (W * W * std::max(perCell, 1e-3))) - 2 * R;
+        K = int(1650.0 / (W * W * std::max(perCell, 1e-3))) - 2 * R;
         K = std::max(1, std::min(K, g.n[2]));
     }
     K = std::min(K, MAX_SPAN - 2 * R);
     for (;;)
     {
+        vl->tileCols = a;
         setupTiles(vl, K);
         if (vl->maxStaged <= STAGE_MAX) break;
-        if (K == 1)
+        if (K > 1)
+            K = std::max(1, K / 2);
+        else if (a > 1)
+            --a;
+        else
             fail(MRMD_ERR_UNSUP, "a tile neighbourhood holds %d records (limit %d): density too high for the tiled list",
                  vl->maxStaged, STAGE_MAX);
-        K = std::max(1, K / 2);
     }
     vl->strideS = std::max<int64_t>(64, (nall + 63) / 64 * 64);
     vl->cnt16.reserve(ctx, vl->strideS);
@@ -338,7 +371,7 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     if (vl->capFull <= 0) vl->capFull = int32_t(std::max<int64_t>(2 * maxNeigh + 16, 96));
     vl->capFull = (vl->capFull + CHUNK - 1) / CHUNK * CHUNK;
     int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64)) + 1;
-    const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
+    const size_t lds = size_t(std::max(vl->maxStaged, 1)) * Staged<true>::bytes;
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         vl->nbr16.reserve(ctx, int64_t(vl->capFull) * vl->strideS);
@@ -429,20 +462,23 @@ int mrmd_verlet_destroy(mrmd_verlet_list* v)
     return MRMD_OK;
 }
 
-int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int block)
+int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_columns, int block)
 {
     MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
-    REQUIRE(vl && tile_cells >= 0, "invalid tiling");
+    REQUIRE(vl && tile_cells >= 0 && tile_columns >= 1 && tile_columns <= 3, "invalid tiling");
     REQUIRE(block == 64 || block == 128 || block == 192 || block == 256, "workgroup size must be 64, 128, 192 or 256");
     vl->tileK = tile_cells;
+    vl->tileCols = tile_columns;
     vl->block = block;
     MRMD_API_END
 }
 
-int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* block, int* max_staged)
+int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* tile_columns, int* block,
+                           int* max_staged)
 {
     if (!vl) return MRMD_ERR_ARG;
     if (tile_cells) *tile_cells = vl->tileK;
+    if (tile_columns) *tile_columns = vl->tileCols;
     if (block) *block = vl->block;
     if (max_staged) *max_staged = vl->maxStaged;
     return MRMD_OK;

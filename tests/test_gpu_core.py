@@ -95,6 +95,35 @@ def test_espp_verlet_pairs_sets_energy_forces(M, ctx, espp_positions):
     assert np.abs(Fg - Fc).max() <= F_TOL * np.abs(Fc).max()
 
 
+@pytest.mark.parametrize("columns,cells,block", [(1, 1, 64), (1, 5, 256), (2, 2, 128), (2, 4, 256), (3, 1, 192),
+                                                 (3, 2, 256)])
+def test_tilings_give_identical_lists_and_forces(M, ctx, espp_positions, columns, cells, block):
+    """the tile shape and workgroup size change neither the neighbour rows (set and order)
+    nor a single bit of the forces: rows are summed in stencil order whatever the tiling"""
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    ref = M.HalfVerletList(ctx)
+    ref.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+    vl = M.HalfVerletList(ctx)
+    vl.set_tiling(cells, block, columns)
+    vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+    t = vl.tiling()
+    assert (t["tile_columns"], t["tile_cells"], t["block"]) == (columns, cells, block)
+    assert vl.num_pairs() == 1310403
+    a, ac = ref.full_rows(g.nlocal)
+    b, bc = vl.full_rows(g.nlocal)
+    np.testing.assert_array_equal(ac, bc)
+    np.testing.assert_array_equal(a, b)
+    lj = M.LennardJones(ctx, 2.5, 1.0, 1.0)
+    g.set_force(0.0)
+    lj.apply(g, ref)
+    f_ref = g.download(M.FORCE, g.nlocal)
+    g.set_force(0.0)
+    lj.apply(g, vl)
+    np.testing.assert_array_equal(g.download(M.FORCE, g.nlocal), f_ref)
+
+
 def test_espp_integrators_bit_exact(M, ctx, espp_positions):
     rng = np.random.default_rng(5)
     n = espp_positions.shape[0]
