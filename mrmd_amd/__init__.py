@@ -310,10 +310,15 @@ class LinkedCellList:
         ctx.check(_L.mrmd_cell_list_create(ctx.h, C.byref(h)))
         self.h = h
         self.begin, self.end = begin, end
+        self.build(atoms, begin, end, delta, gmin, gmax)
+
+    def build(self, atoms, begin, end, delta, gmin, gmax):
+        """rebuild in place (buffers are reused)"""
+        self.begin, self.end = begin, end
         d = np.ascontiguousarray(np.broadcast_to(delta, 3), dtype=np.float64)
         lo = np.ascontiguousarray(gmin, dtype=np.float64)
         hi = np.ascontiguousarray(gmax, dtype=np.float64)
-        ctx.check(_L.mrmd_cell_list_build(h, atoms.h, begin, end, _p(d), _p(lo), _p(hi)))
+        self.ctx.check(_L.mrmd_cell_list_build(self.h, atoms.h, begin, end, _p(d), _p(lo), _p(hi)))
 
     def __del__(self):
         if getattr(self, "h", None):

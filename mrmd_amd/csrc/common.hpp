@@ -10,6 +10,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mrmd_hip.h"
@@ -128,6 +129,12 @@ struct DBuf
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
+    void swap(DBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(ctx, o.ctx);
+    }
 
     void release()
     {

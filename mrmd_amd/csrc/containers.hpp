@@ -46,10 +46,17 @@ struct mrmd_atoms
     mrmd_hip::DBuf<double4> pos;
     mrmd_hip::DBuf<double> vel, force, mass, charge, relmass;
     mrmd_hip::DBuf<double> stage;  ///< transfer staging
+    /// second set of field buffers: permutation gathers into it and swaps it in
+    mrmd_hip::DBuf<double4> sparePos;
+    mrmd_hip::DBuf<double> spareVel, spareForce, spareMass, spareCharge, spareRelmass;
 
     mrmd_hip::AtomsView view() const
     {
         return {pos.p, vel.p, force.p, mass.p, charge.p, relmass.p, cap};
+    }
+    mrmd_hip::AtomsView spareView() const
+    {
+        return {sparePos.p, spareVel.p, spareForce.p, spareMass.p, spareCharge.p, spareRelmass.p, cap};
     }
     /// grow the capacity to at least `need` records, preserving [0, size)
     void ensureCap(int64_t need);

@@ -153,22 +153,40 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
         for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
     if (threadIdx.x == 0) ti = tl.tiles[blk];
     __syncthreads();
+    // the first row's global loads (count, index, position, first index chunk) are issued
+    // before the staging so that their latency overlaps it
+    int s = -1, c = 0;
+    int32_t i = 0;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint4 cur = {0, 0, 0, 0};
+    if (int(threadIdx.x) < ti.nrows)
+    {
+        int self;
+        s = tileRow(ti, threadIdx.x, self);
+        c = tl.cnt[s];
+        i = tl.sorted[s];
+        pi = tl.sortedPos[s];
+        cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
+    }
     stageTile<ONE>(ti, tl.sortedPos, st);
     __syncthreads();
     double e = 0.0, w = 0.0;
     for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
-        int self;
-        const int s = tileRow(ti, t, self);
-        const int c = tl.cnt[s];
-        const int32_t i = tl.sorted[s];
+        if (t != int(threadIdx.x))
+        {
+            int self;
+            s = tileRow(ti, t, self);
+            c = tl.cnt[s];
+            i = tl.sorted[s];
+            pi = tl.sortedPos[s];
+            cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
+        }
         if (c == 0 || i >= a.rows) continue;
-        const double4 pi = tl.sortedPos[s];
         const int ti0 = ONE ? 0 : int(pi.w) * a.typeStride;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const uint4* chunk = reinterpret_cast<const uint4*>(tl.nbr) + s;
         const int nch = (c + CHUNK - 1) / CHUNK;
-        uint4 cur = chunk[0];
         for (int ch = 0; ch < nch; ++ch)
         {
             uint4 nxt = cur;

@@ -247,46 +247,46 @@ int mrmd_atoms_permute(mrmd_atoms* a, const mrmd_cell_list* cl)
     if (n == 0) return MRMD_OK;
     mrmd_ctx* ctx = a->ctx;
     KTimer t(ctx, "atoms_permute");
-    // gather into scratch buffers of the same capacity, then swap them in
-    mrmd_atoms tmp;
-    tmp.ctx = ctx;
-    tmp.cap = a->cap;
-    tmp.pos.reserve(ctx, a->cap);
-    tmp.vel.reserve(ctx, 3 * a->cap);
-    tmp.force.reserve(ctx, 3 * a->cap);
-    tmp.mass.reserve(ctx, a->cap);
-    tmp.charge.reserve(ctx, a->cap);
-    tmp.relmass.reserve(ctx, a->cap);
-    hipLaunchKernelGGL(k_permuteGather, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(), tmp.view(),
+    // gather into the spare buffers (same capacity), then swap them in; the old buffers
+    // become the spares of the next permutation
+    a->sparePos.reserve(ctx, a->cap);
+    a->spareVel.reserve(ctx, 3 * a->cap);
+    a->spareForce.reserve(ctx, 3 * a->cap);
+    a->spareMass.reserve(ctx, a->cap);
+    a->spareCharge.reserve(ctx, a->cap);
+    a->spareRelmass.reserve(ctx, a->cap);
+    hipLaunchKernelGGL(k_permuteGather, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(), a->spareView(),
                        cl->bin.sorted.p, n, cl->bin.begin);
     // records outside [begin, end) are unchanged: copy them across
     const int64_t b = cl->bin.begin, e = cl->bin.end, s = a->size;
     auto copyRange = [&](int64_t lo, int64_t hi)
     {
         if (hi <= lo) return;
-        HIPCHECK(hipMemcpyAsync(tmp.pos.p + lo, a->pos.p + lo, size_t(hi - lo) * 32, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(a->sparePos.p + lo, a->pos.p + lo, size_t(hi - lo) * 32, hipMemcpyDeviceToDevice,
+                                ctx->stream));
         for (int c = 0; c < 3; ++c)
         {
-            HIPCHECK(hipMemcpyAsync(tmp.vel.p + c * a->cap + lo, a->vel.p + c * a->cap + lo, size_t(hi - lo) * 8,
+            HIPCHECK(hipMemcpyAsync(a->spareVel.p + c * a->cap + lo, a->vel.p + c * a->cap + lo, size_t(hi - lo) * 8,
                                     hipMemcpyDeviceToDevice, ctx->stream));
-            HIPCHECK(hipMemcpyAsync(tmp.force.p + c * a->cap + lo, a->force.p + c * a->cap + lo, size_t(hi - lo) * 8,
-                                    hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHECK(hipMemcpyAsync(a->spareForce.p + c * a->cap + lo, a->force.p + c * a->cap + lo,
+                                    size_t(hi - lo) * 8, hipMemcpyDeviceToDevice, ctx->stream));
         }
-        HIPCHECK(hipMemcpyAsync(tmp.mass.p + lo, a->mass.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHECK(hipMemcpyAsync(tmp.charge.p + lo, a->charge.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHECK(hipMemcpyAsync(tmp.relmass.p + lo, a->relmass.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice,
+        HIPCHECK(hipMemcpyAsync(a->spareMass.p + lo, a->mass.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice,
                                 ctx->stream));
+        HIPCHECK(hipMemcpyAsync(a->spareCharge.p + lo, a->charge.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+        HIPCHECK(hipMemcpyAsync(a->spareRelmass.p + lo, a->relmass.p + lo, size_t(hi - lo) * 8,
+                                hipMemcpyDeviceToDevice, ctx->stream));
     };
     copyRange(0, b);
     copyRange(e, s);
     checkLaunch();
-    sync(ctx);
-    std::swap(a->pos.p, tmp.pos.p);
-    std::swap(a->vel.p, tmp.vel.p);
-    std::swap(a->force.p, tmp.force.p);
-    std::swap(a->mass.p, tmp.mass.p);
-    std::swap(a->charge.p, tmp.charge.p);
-    std::swap(a->relmass.p, tmp.relmass.p);
+    a->pos.swap(a->sparePos);
+    a->vel.swap(a->spareVel);
+    a->force.swap(a->spareForce);
+    a->mass.swap(a->spareMass);
+    a->charge.swap(a->spareCharge);
+    a->relmass.swap(a->spareRelmass);
     MRMD_API_END
 }
 

@@ -115,6 +115,15 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
     tileCoords(tg, t, i0, j0, k0, k1);
     const int span = tg.cols + 2 * R;
     const int kl = max(k0 - R, 0), nk = min(k1 + R, tg.n2) - kl;
+    // the first row's dependent loads (slot -> record -> cell) are issued before the staging
+    int s = -1, self = 0, cell = 0;
+    int32_t p = -1;
+    if (int(threadIdx.x) < ti.nrows)
+    {
+        s = tileRow(ti, threadIdx.x, self);
+        p = sorted[s];
+        cell = cellOf[p];
+    }
     stageTile<true>(ti, sortedPos, st);
     for (int x = threadIdx.x; x < ti.nr * (nk + 1); x += blockDim.x)
     {
@@ -134,9 +143,12 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
     int localMax = 0;
     for (int row = threadIdx.x; row < ti.nrows; row += blockDim.x)
     {
-        int self;
-        const int s = tileRow(ti, row, self);
-        const int32_t p = sorted[s];
+        if (row != int(threadIdx.x))
+        {
+            s = tileRow(ti, row, self);
+            p = sorted[s];
+            cell = cellOf[p];
+        }
         if (p < rowLo || p >= rowHi)
         {
             cnt[s] = 0;
@@ -144,7 +156,6 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
         }
         const int qSelf = ldsSlot(ti, ti.selfRange[self], s);
         const double4 xp = Staged<true>::get(st, qSelf);
-        const int cell = cellOf[p];
         const int ck = cell % tg.n2, i = (cell / tg.n2) / tg.n1, j = (cell / tg.n2) % tg.n1;
         int c = 0;
         for (int ii = max(i - R, 0); ii < min(i + R + 1, tg.n0); ++ii)
@@ -311,10 +322,10 @@ void setupTiles(mrmd_verlet_list* vl, int K)
     hipLaunchKernelGGL(k_tileSetup, dim3(gridFor(vl->ntiles)), dim3(BLOCK), 0, ctx->stream, geom(vl), vl->ntiles,
                        vl->bin.offsets.p, vl->bin.counts.p, vl->tiles.p, dmax);
     checkLaunch();
-    int32_t h = 0;
-    HIPCHECK(hipMemcpyAsync(&h, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+    int32_t* h = reinterpret_cast<int32_t*>(ctx->pinned);
+    HIPCHECK(hipMemcpyAsync(h, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
-    vl->maxStaged = h;
+    vl->maxStaged = h[0];
 }
 
 void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t begin, int64_t end, double rn,
@@ -390,9 +401,10 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                                    vl->capFull, vl->cnt16.p, dmax);
         }
         checkLaunch();
-        int32_t hmax = 0;
-        HIPCHECK(hipMemcpyAsync(&hmax, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+        int32_t* hp = reinterpret_cast<int32_t*>(ctx->pinned);
+        HIPCHECK(hipMemcpyAsync(hp, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
         sync(ctx);
+        const int32_t hmax = hp[0];
         vl->maxFull = hmax;
         if (hmax <= vl->capFull) break;
         // Cabana VerletLayout2D: re-count and refill with the larger width

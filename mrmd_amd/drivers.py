@@ -42,6 +42,7 @@ class LJSystem:
         self.cell_ratio, self.max_neigh = cell_ratio, max_neigh
         self.langevin = VelocityVerletLangevinThermostat(gamma, temperature, seed) if langevin else None
         self.permute = permute
+        self.cells = None
         self.max_disp = np.finfo(np.float64).max
         self.rebuilds = 0
         self.step_count = 0
@@ -50,8 +51,11 @@ class LJSystem:
         a = self.atoms
         self.ghosts.exchange_real_atoms(a, self.box)
         if self.permute:
-            cl = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
-            a.permute(cl)
+            if self.cells is None:
+                self.cells = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
+            else:
+                self.cells.build(a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
+            a.permute(self.cells)
         self.ghosts.create_ghost_atoms(a, self.box)
         self.vl.build(a, 0, a.nlocal, self.rn, self.cell_ratio, np.full(3, -self.rn), self.L + self.rn,
                       self.max_neigh)
