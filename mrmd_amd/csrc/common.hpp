@@ -129,6 +129,12 @@ struct DBuf
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
+    /// capacity for at least `need` elements, growing by a quarter beyond it so that a
+    /// slowly varying size does not reallocate on every call
+    void ensure(mrmd_ctx* c, int64_t need, int64_t keep = 0)
+    {
+        if (need > n) reserve(c, need + need / 4 + 256, keep);
+    }
     void swap(DBuf& o)
     {
         std::swap(p, o.p);

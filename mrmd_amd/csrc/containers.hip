@@ -473,7 +473,7 @@ int mrmd_atoms_upload(mrmd_atoms* a, int field, const void* host, int64_t off, i
     mrmd_ctx* ctx = a->ctx;
     const bool vec = field <= MRMD_FORCE;
     const int64_t bytes = (vec ? 3 : 1) * n * 8;
-    a->stage.reserve(ctx, (vec ? 3 : 1) * n);
+    a->stage.ensure(ctx, (vec ? 3 : 1) * n);
     HIPCHECK(hipMemcpyAsync(a->stage.p, host, size_t(bytes), hipMemcpyHostToDevice, ctx->stream));
     const dim3 g(gridFor(n)), b(BLOCK);
     switch (field)
@@ -516,7 +516,7 @@ int mrmd_atoms_download(const mrmd_atoms* ac, int field, void* host, int64_t off
     if (n == 0) return MRMD_OK;
     mrmd_ctx* ctx = a->ctx;
     const bool vec = field <= MRMD_FORCE;
-    a->stage.reserve(ctx, (vec ? 3 : 1) * n);
+    a->stage.ensure(ctx, (vec ? 3 : 1) * n);
     const dim3 g(gridFor(n)), b(BLOCK);
     const double* src = nullptr;
     switch (field)
@@ -691,7 +691,7 @@ int mrmd_molecules_upload(mrmd_molecules* m, int field, const void* host, int64_
     if (off + n > m->size) m->setSize(off + n);
     mrmd_ctx* ctx = m->ctx;
     const bool vec = field == MRMD_MOL_POS || field == MRMD_MOL_FORCE || field == MRMD_MOL_GRADLAMBDA;
-    m->stage.reserve(ctx, (vec ? 3 : 1) * n);
+    m->stage.ensure(ctx, (vec ? 3 : 1) * n);
     HIPCHECK(hipMemcpyAsync(m->stage.p, host, size_t((vec ? 3 : 1) * n) * 8, hipMemcpyHostToDevice, ctx->stream));
     const dim3 g(gridFor(n)), b(BLOCK);
     switch (field)
@@ -732,7 +732,7 @@ int mrmd_molecules_download(const mrmd_molecules* mc, int field, void* host, int
     if (n == 0) return MRMD_OK;
     mrmd_ctx* ctx = m->ctx;
     const bool vec = field == MRMD_MOL_POS || field == MRMD_MOL_FORCE || field == MRMD_MOL_GRADLAMBDA;
-    m->stage.reserve(ctx, (vec ? 3 : 1) * n);
+    m->stage.ensure(ctx, (vec ? 3 : 1) * n);
     const dim3 g(gridFor(n)), b(BLOCK);
     const void* src = nullptr;
     switch (field)

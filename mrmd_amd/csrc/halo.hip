@@ -407,8 +407,8 @@ void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int ax
     const int64_t total = a->nlocal + a->nghost;
     if (total >= (int64_t(1) << 31) - 1) fail(MRMD_ERR_UNSUP, "more than 2^31 records");
     const int64_t tiles = std::max<int64_t>(1, (total + TILE - 1) / TILE);
-    gl->tileCnt.reserve(ctx, 2 * tiles);
-    gl->tileOff.reserve(ctx, 2 * tiles);
+    gl->tileCnt.ensure(ctx, 2 * tiles);
+    gl->tileOff.ensure(ctx, 2 * tiles);
     int32_t* tot = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
     hipLaunchKernelGGL(k_ghostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->pos.p, total, axis,
                        s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, gl->tileCnt.p + tiles);
@@ -443,8 +443,8 @@ void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const
     const int64_t totM = m->nlocal + m->nghost;
     const int64_t totA = a->nlocal + a->nghost;
     const int64_t tiles = std::max<int64_t>(1, (totM + TILE - 1) / TILE);
-    gl->tileCnt.reserve(ctx, 4 * tiles);
-    gl->tileOff.reserve(ctx, 4 * tiles);
+    gl->tileCnt.ensure(ctx, 4 * tiles);
+    gl->tileOff.ensure(ctx, 4 * tiles);
     int32_t* tot = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
     hipLaunchKernelGGL(k_molGhostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(), totM, axis,
                        s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, tiles);

@@ -88,16 +88,32 @@ struct TiledList
     const double4* sortedPos;
 };
 
+// 1 / x from the hardware reciprocal estimate refined by two Newton-Raphson steps
+// (e = 1 - x r, r <- r + r e): each step squares the relative error, so the result is
+// within an ulp or two of the correctly rounded quotient.  x = 0 gives a NaN, which the
+// callers mask.
+__device__ inline double recipNewton(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Branch-free pair terms of one chunk of eight entries, written stage by stage over the
 // eight pairs so that the eight dependency chains (LDS read, distance, reciprocal, force)
 // are independent and interleave.  Padding entries (the row itself, d^2 = 0) and pairs
 // beyond rc select zero.  CAP: the table has a capping distance, so the capped branch
-// (LennardJones.hpp:68-75) is evaluated too.
+// (LennardJones.hpp:68-75) is evaluated too.  Products and sums may contract to FMAs here
+// (the neighbour decision of the list is unaffected: it is made, unfused, at build time);
+// the forces then differ from the unfused CPU arithmetic by a few ulps per term.
 template <bool PRED, bool ONE, bool CAP>
 __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, int ti0, const double4& pi,
                                const void* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
                                double& w)
 {
+#pragma clang fp contract(fast)
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
     double dx[CHUNK], dy[CHUNK], dz[CHUNK], d2[CHUNK], ty[CHUNK];
 #pragma unroll
@@ -115,7 +131,7 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
     for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
     double frac2[CHUNK];
 #pragma unroll
-    for (int u = 0; u < CHUNK; ++u) frac2[u] = 1.0 / d2[u];
+    for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u)
     {
@@ -567,7 +583,7 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
     mrmd_ctx* ctx = lj->ctx;
     const bool full = lj->symmetric && predIsSymmetric(pred);
     const unsigned grid = full ? unsigned(std::max<int64_t>(vl->ntiles, 1)) : gridFor(rows);
-    lj->partials.reserve(ctx, int64_t(grid) * 2);
+    lj->partials.ensure(ctx, int64_t(grid) * 2);
     double* partials = lj->partials.p;
     LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
              int(lj->typeStride), lj->rcSqr, pred ? *pred : mrmd_pred2{}, pred && pred->op != MRMD_PAIR_ALWAYS};

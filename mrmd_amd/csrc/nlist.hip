@@ -123,11 +123,11 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
     end = e;
     const int64_t n = e - b;
     const int ncell = g.total();
-    cellOf.reserve(ctx, std::max<int64_t>(growCap(n), 1));
-    sorted.reserve(ctx, std::max<int64_t>(growCap(n), 1));
-    counts.reserve(ctx, ncell + 1);
-    offsets.reserve(ctx, ncell + 1);
-    fill.reserve(ctx, ncell + 1);
+    cellOf.ensure(ctx, std::max<int64_t>(n, 1));
+    sorted.ensure(ctx, std::max<int64_t>(n, 1));
+    counts.ensure(ctx, ncell + 1);
+    offsets.ensure(ctx, ncell + 1);
+    fill.ensure(ctx, ncell + 1);
     hipLaunchKernelGGL(k_zeroI32, dim3(gridFor(ncell)), dim3(BLOCK), 0, ctx->stream, counts.p, int64_t(ncell));
     hipLaunchKernelGGL(k_zeroI32, dim3(gridFor(ncell)), dim3(BLOCK), 0, ctx->stream, fill.p, int64_t(ncell));
     if (n > 0)
@@ -143,7 +143,7 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
     }
     if (withPos && n > 0)
     {
-        sortedPos.reserve(ctx, growCap(n));
+        sortedPos.ensure(ctx, n);
         hipLaunchKernelGGL(k_gatherPos, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, sorted.p, n,
                            sortedPos.p);
     }

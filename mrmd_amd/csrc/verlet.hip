@@ -167,30 +167,37 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     if (g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kk) > rsq) continue;
                     const int* cs = cs0 + (kk - kl);
                     const int q1 = cs[1];
-                    for (int q = cs[0]; q < q1; q += 4)
+                    for (int qb = cs[0]; qb < q1; qb += 32)
                     {
-                        double4 xn[4];
-                        bool hit[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
+                        // branch-free pass over up to 32 candidates: hit and half-list bits
+                        uint32_t hits = 0, halves = 0;
+                        const int nq = min(32, q1 - qb);
+                        for (int u0 = 0; u0 < nq; u0 += 4)
                         {
-                            const bool valid = q + u < q1 && q + u != qSelf;
-                            xn[u] = Staged<true>::get(st, valid ? q + u : qSelf);
-                            const double dx = xp.x - xn[u].x;
-                            const double dy = xp.y - xn[u].y;
-                            const double dz = xp.z - xn[u].z;
-                            hit[u] = valid && dx * dx + dy * dy + dz * dz <= rsq;
-                        }
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (hit[u])
+                            for (int u = u0; u < u0 + 4; ++u)
                             {
-                                const double4 n = xn[u];
-                                const bool half =
-                                    (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
-                                if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((q + u) | (half ? 0x8000 : 0));
-                                ++c;
+                                const bool valid = u < nq && qb + u != qSelf;
+                                const double4 n = Staged<true>::get(st, valid ? qb + u : qSelf);
+                                const double dx = xp.x - n.x;
+                                const double dy = xp.y - n.y;
+                                const double dz = xp.z - n.z;
+                                const bool hit = valid && dx * dx + dy * dy + dz * dz <= rsq;
+                                const bool half = (n.x > xp.x) ||
+                                                  ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
+                                hits |= uint32_t(hit) << u;
+                                halves |= uint32_t(half) << u;
                             }
+                        }
+                        // append the hits in slot order
+                        while (hits)
+                        {
+                            const int u = __builtin_ctz(hits);
+                            hits &= hits - 1;
+                            if (c < cap)
+                                nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (((halves >> u) & 1u) << 15));
+                            ++c;
+                        }
                     }
                 }
             }
@@ -316,7 +323,7 @@ void setupTiles(mrmd_verlet_list* vl, int K)
     vl->tilesPerCol = (g.n[2] + K - 1) / K;
     const int a = vl->tileCols;
     vl->ntiles = int64_t((g.n[0] + a - 1) / a) * ((g.n[1] + a - 1) / a) * vl->tilesPerCol;
-    vl->tiles.reserve(ctx, std::max<int64_t>(vl->ntiles, 1));
+    vl->tiles.ensure(ctx, std::max<int64_t>(vl->ntiles, 1));
     int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64));
     hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
     hipLaunchKernelGGL(k_tileSetup, dim3(gridFor(vl->ntiles)), dim3(BLOCK), 0, ctx->stream, geom(vl), vl->ntiles,
@@ -377,7 +384,7 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                  vl->maxStaged, STAGE_MAX);
     }
     vl->strideS = std::max<int64_t>(64, (nall + 63) / 64 * 64);
-    vl->cnt16.reserve(ctx, vl->strideS);
+    vl->cnt16.ensure(ctx, vl->strideS);
     hipLaunchKernelGGL(k_zeroI32v, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, vl->cnt16.p, nall);
     if (vl->capFull <= 0) vl->capFull = int32_t(std::max<int64_t>(2 * maxNeigh + 16, 96));
     vl->capFull = (vl->capFull + CHUNK - 1) / CHUNK * CHUNK;
@@ -385,7 +392,7 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     const size_t lds = size_t(std::max(vl->maxStaged, 1)) * Staged<true>::bytes;
     for (int attempt = 0; attempt < 2; ++attempt)
     {
-        vl->nbr16.reserve(ctx, int64_t(vl->capFull) * vl->strideS);
+        vl->nbr16.ensure(ctx, int64_t(vl->capFull) * vl->strideS);
         hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
         if (vl->ntiles > 0 && end > begin)
         {
@@ -427,8 +434,8 @@ void mrmd_verlet_list::ensureGlobal()
     if (globalValid) return;
     const int64_t rows = end - begin;
     stride = std::max<int64_t>(64, (rows + 63) / 64 * 64);
-    cnt.reserve(ctx, stride);
-    nbr.reserve(ctx, int64_t(std::max(maxFull, 1)) * stride);
+    cnt.ensure(ctx, stride);
+    nbr.ensure(ctx, int64_t(std::max(maxFull, 1)) * stride);
     if (rows > 0)
     {
         hipLaunchKernelGGL(k_zeroI32v, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, cnt.p, rows);
@@ -445,8 +452,8 @@ void mrmd_verlet_list::ensureHalf()
     ensureGlobal();
     const int64_t rows = end - begin;
     capHalf = std::max(maxFull, 1);
-    hnbr.reserve(ctx, int64_t(capHalf) * stride);
-    hcnt.reserve(ctx, stride);
+    hnbr.ensure(ctx, int64_t(capHalf) * stride);
+    hcnt.ensure(ctx, stride);
     if (rows > 0)
         hipLaunchKernelGGL(k_halfExtract, dim3(gridFor(rows)), dim3(BLOCK), 0, ctx->stream, nbr.p, cnt.p, rows, stride,
                            hnbr.p, hcnt.p);
