@@ -31,24 +31,58 @@ GridD makeGrid(const double* gmin, const double* gmax, const double* delta)
 
 namespace
 {
+// Consecutive records of a wave that fall into the same cell form a run (after a cell
+// permutation nearly every wave is a few runs); the run's first lane does the run's
+// atomic, so one atomic per run replaces one per record.
+struct Run
+{
+    int head;  ///< lane of the run's first record
+    int len;   ///< records in the run (valid on the head lane)
+};
+
+__device__ inline Run laneRun(int c, bool valid)
+{
+    const int lane = int(threadIdx.x) % WAVE;
+    const int prev = __shfl_up(c, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long valids = __ballot(valid);
+    const unsigned long long upTo = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);
+    Run r;
+    r.head = heads & upTo ? 63 - __clzll(heads & upTo) : lane;
+    const unsigned long long above = heads & ~upTo;
+    const int end = above ? __builtin_ctzll(above) : __popcll(valids);
+    r.len = end - lane;
+    return r;
+}
+
 __global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g, int32_t* cellOf, int32_t* counts)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double4 p = pos[begin + k];
-    const int c = g.cell(p.x, p.y, p.z);
-    cellOf[k] = c;
-    atomicAdd(&counts[c], 1);
+    const bool valid = k < n;
+    int c = -1;
+    if (valid)
+    {
+        const double4 p = pos[begin + k];
+        c = g.cell(p.x, p.y, p.z);
+        cellOf[k] = c;
+    }
+    const Run r = laneRun(c, valid);
+    if (valid && r.head == int(threadIdx.x) % WAVE) atomicAdd(&counts[c], r.len);
 }
 
 __global__ void k_binFill(const int32_t* cellOf, int64_t begin, int64_t n, const int32_t* offsets, int32_t* fill,
                           int32_t* sorted)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int c = cellOf[k];
-    const int slot = offsets[c] + atomicAdd(&fill[c], 1);
-    sorted[slot] = int32_t(begin + k);
+    const bool valid = k < n;
+    const int c = valid ? cellOf[k] : -1;
+    const Run r = laneRun(c, valid);
+    const int lane = int(threadIdx.x) % WAVE;
+    int base = 0;
+    if (valid && r.head == lane) base = atomicAdd(&fill[c], r.len);
+    base = __shfl(base, r.head, WAVE);
+    if (valid) sorted[offsets[c] + base + (lane - r.head)] = int32_t(begin + k);
 }
 
 // restore ascending record order inside each cell (insertion sort; cells hold tens of
