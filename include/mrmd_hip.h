@@ -299,8 +299,8 @@ int mrmd_lj_set_params(mrmd_lj* lj, const double* sigma, const double* epsilon);
 /* =========================================================================================
  * integrators  (action::VelocityVerlet, VelocityVerletLangevinThermostat)
  * ========================================================================================= */
-/* VelocityVerlet::preForceIntegrate (VelocityVerlet.cpp:26-67); max_disp may be NULL (the
- * value then stays on the device and accumulates for mrmd_rebuild_check) */
+/* VelocityVerlet::preForceIntegrate (VelocityVerlet.cpp:26-67); returns sqrt(max |dx|^2) in
+ * *max_disp (synchronises); max_disp may be NULL (no synchronisation) */
 int mrmd_vv_pre(mrmd_atoms* atoms, double dt, double* max_disp);
 /* VelocityVerlet::postForceIntegrate (VelocityVerlet.cpp:69-90) */
 int mrmd_vv_post(mrmd_atoms* atoms, double dt);

@@ -239,6 +239,7 @@ void mrmd_atoms::setSize(int64_t n)
     {
         hipLaunchKernelGGL(k_zeroAtoms, dim3(gridFor(n - size)), dim3(BLOCK), 0, ctx->stream, view(), size, n);
         checkLaunch();
+        if (massValue != 0.0) massUniform = false;
     }
     size = n;
 }
@@ -492,8 +493,20 @@ int mrmd_atoms_upload(mrmd_atoms* a, int field, const void* host, int64_t off, i
                                reinterpret_cast<const int64_t*>(a->stage.p), off, n);
             break;
         case MRMD_MASS:
+        {
             HIPCHECK(hipMemcpyAsync(a->mass.p + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            const double* m = static_cast<const double*>(host);
+            bool same = true;
+            for (int64_t k = 1; k < n && same; ++k) same = m[k] == m[0];
+            if (off == 0 && n >= a->size)
+            {
+                a->massUniform = same;
+                a->massValue = m[0];
+            }
+            else
+                a->massUniform = a->massUniform && same && m[0] == a->massValue;
             break;
+        }
         case MRMD_CHARGE:
             HIPCHECK(hipMemcpyAsync(a->charge.p + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
             break;
@@ -580,6 +593,8 @@ int mrmd_atoms_fill(mrmd_atoms* a, int field, double value)
         }
         case MRMD_MASS:
             hipLaunchKernelGGL(k_fill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->mass.p, n, value);
+            a->massUniform = true;
+            a->massValue = value;
             break;
         case MRMD_CHARGE:
             hipLaunchKernelGGL(k_fill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->charge.p, n, value);
@@ -613,6 +628,7 @@ int mrmd_atoms_device_ptr(mrmd_atoms* a, int field, void** ptr, int64_t* stride)
             break;
         case MRMD_MASS:
             p = a->mass.p;
+            a->massUniform = false;  // the caller may write masses directly
             break;
         case MRMD_CHARGE:
             p = a->charge.p;

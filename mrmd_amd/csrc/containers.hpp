@@ -43,6 +43,10 @@ struct mrmd_atoms
     mrmd_hip::CtxPtr ctx;
     int64_t nlocal = 0, nghost = 0, size = 0, cap = 0;
     int64_t maxType = 0;  ///< largest type ever uploaded (kernels index tables with it)
+    /// every record [0, size) has mass massValue (tracked by the host-side writers; lets the
+    /// Langevin step compute its mass-dependent factors once)
+    bool massUniform = true;
+    double massValue = 0.0;
     mrmd_hip::DBuf<double4> pos;
     mrmd_hip::DBuf<double> vel, force, mass, charge, relmass;
     mrmd_hip::DBuf<double> stage;  ///< transfer staging

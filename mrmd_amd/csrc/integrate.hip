@@ -1,6 +1,7 @@
 // Velocity-Verlet and Langevin (BAOAB-style) integration and the kinetic energy.
-// Streaming kernels: one lane per local atom, coalesced SoA reads/writes, the maximum
-// displacement reduced with wave shuffles and one 64-bit atomic max per workgroup.
+// Streaming kernels: one lane per local atom, coalesced SoA reads/writes; the maximum
+// displacement is reduced with wave shuffles to one value per workgroup and then by one
+// small ordered pass (no atomics on a single address).
 #include <algorithm>
 
 #include "containers.hpp"
@@ -19,11 +20,12 @@ struct DispScalars
 __global__ void k_zero1(double* p) { *p = 0.0; }
 
 // VelocityVerlet::preForceIntegrate (VelocityVerlet.cpp:26-67; UpdateSteps.hpp:25-54)
-__global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double dt, double* maxSq)
+__global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double dt, double* partials)
 {
     __shared__ double red[BLOCK / WAVE];
     double d2 = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
     {
         const double dtHalf = 0.5 * dt;
         double4 p = a.pos[i];
@@ -46,7 +48,7 @@ __global__ __launch_bounds__(BLOCK) void k_vvPre(AtomsView a, int64_t n, double 
         d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
     }
     const double m = blockMax(d2, red);
-    if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, m);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
 // VelocityVerlet::postForceIntegrate (VelocityVerlet.cpp:69-90)
@@ -62,19 +64,39 @@ __global__ __launch_bounds__(BLOCK) void k_vvPost(AtomsView a, int64_t n, double
 
 // VelocityVerletLangevinThermostat::preForceIntegrate_apply_if
 // (VelocityVerletLangevinThermostat.hpp:63-136; UpdateSteps.hpp:56-79)
-template <bool PRED>
+// mass-dependent factors of the O-step for one mass (the expressions of the per-atom path)
+struct LangevinFactors
+{
+    double m, dtfm, damping, sigma;
+};
+
+__host__ __device__ inline LangevinFactors langevinFactors(double m, double dt, double zeta, double temperature)
+{
+    LangevinFactors f;
+    f.m = m;
+    f.dtfm = (0.5 * dt) / m;
+    const double dtm = dt / m;
+    f.damping = detExp(-zeta * dtm);
+    f.sigma = sqrt(temperature / m * (1.0 - detExp(-2.0 * zeta * dtm)));
+    return f;
+}
+
+// UNI: every atom has the mass uf.m, whose factors the host computed with the same code
+template <bool PRED, bool UNI>
 __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
-                                                       uint64_t seed, uint64_t call, mrmd_pred1 pred, double* maxSq)
+                                                       uint64_t seed, uint64_t call, mrmd_pred1 pred, LangevinFactors uf,
+                                                       double* partials)
 {
     __shared__ double red[BLOCK / WAVE];
     double d2 = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
     {
         const double dtHalf = 0.5 * dt;
         double4 p = a.pos[i];
-        const double m = a.mass[i];
         double vx = a.vel[i], vy = a.vel[a.cap + i], vz = a.vel[2 * a.cap + i];
-        const double dtfm = dtHalf / m;
+        const LangevinFactors f = UNI ? uf : langevinFactors(a.mass[i], dt, zeta, temperature);
+        const double dtfm = f.dtfm;
         vx += dtfm * a.force[i];
         vy += dtfm * a.force[a.cap + i];
         vz += dtfm * a.force[2 * a.cap + i];
@@ -86,9 +108,8 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
         {
             double xi[3];
             normals3(seed, uint64_t(i), call, xi);
-            const double dtm = dt / m;
-            const double damping = detExp(-zeta * dtm);
-            const double sigma = sqrt(temperature / m * (1.0 - detExp(-2.0 * zeta * dtm)));
+            const double damping = f.damping;
+            const double sigma = f.sigma;
             vx *= damping;
             vy *= damping;
             vz *= damping;
@@ -107,7 +128,7 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
         d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
     }
     const double mx = blockMax(d2, red);
-    if (threadIdx.x == 0) atomicMaxNonNeg(maxSq, mx);
+    if (threadIdx.x == 0) partials[blockIdx.x] = mx;
 }
 
 // analysis::getKineticEnergy (KineticEnergy.hpp:26-39): per-workgroup partials
@@ -134,11 +155,22 @@ __global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
     if (threadIdx.x == 0) *out = s;
 }
 
-double* dispScalar(mrmd_atoms* a) { return static_cast<double*>(ctxScratch(a->ctx, 3, 64)); }
+// per-workgroup maxima of the squared displacement, then their maximum (one workgroup)
+__global__ void k_maxPartials(const double* partials, int64_t n, double* out)
+{
+    __shared__ double red[BLOCK / WAVE];
+    double m = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, partials[k]);
+    m = blockMax(m, red);
+    if (threadIdx.x == 0) *out = m;
+}
 
-// grid of the displacement-reducing integrators: enough workgroups to fill the chip, few
-// enough that the per-workgroup atomic maximum on one address stays cheap
-unsigned strideGrid(int64_t n) { return std::min<unsigned>(gridFor(n), 2048u); }
+// scratch slot 3: [0] maximum squared displacement of the last pre-step, [1 ..] partials
+double* dispScalar(mrmd_atoms* a) { return static_cast<double*>(ctxScratch(a->ctx, 3, 64)); }
+double* dispPartials(mrmd_atoms* a, unsigned grid)
+{
+    return static_cast<double*>(ctxScratch(a->ctx, 3, (size_t(grid) + 1) * sizeof(double))) + 1;
+}
 
 void finishDisp(mrmd_atoms* a, double* maxDisp)
 {
@@ -156,13 +188,17 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a, "null atoms");
     mrmd_ctx* ctx = a->ctx;
+    const unsigned grid = gridFor(a->nlocal);
+    double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
-    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
     {
         KTimer t(ctx, "vv_pre");
-        hipLaunchKernelGGL(k_vvPre, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, ms);
+        hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms);
     }
+    else
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     checkLaunch();
     finishDisp(a, maxDisp);
     MRMD_API_END
@@ -186,19 +222,26 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
     REQUIRE(a, "null atoms");
     validatePred1(pred);
     mrmd_ctx* ctx = a->ctx;
+    const unsigned grid = gridFor(a->nlocal);
+    double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
-    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
+    if (a->nlocal == 0) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
     {
         KTimer t(ctx, "langevin_pre");
         const bool usePred = pred && pred->kind != MRMD_PRED_ALWAYS;
         const mrmd_pred1 p = pred ? *pred : mrmd_pred1{};
+        const bool uni = a->massUniform && a->massValue > 0.0;
+        const LangevinFactors uf = uni ? langevinFactors(a->massValue, dt, zeta, temperature) : LangevinFactors{};
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, zeta,
+                               temperature, seed, call, p, uf, part);
+        };
         if (usePred)
-            hipLaunchKernelGGL(k_langevinPre<true>, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                               a->nlocal, dt, zeta, temperature, seed, call, p, ms);
+            uni ? launch(k_langevinPre<true, true>) : launch(k_langevinPre<true, false>);
         else
-            hipLaunchKernelGGL(k_langevinPre<false>, dim3(strideGrid(a->nlocal)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                               a->nlocal, dt, zeta, temperature, seed, call, p, ms);
+            uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms);
     }
     checkLaunch();
     finishDisp(a, maxDisp);

@@ -169,8 +169,8 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     const int q1 = cs[1];
                     for (int qb = cs[0]; qb < q1; qb += 32)
                     {
-                        // branch-free pass over up to 32 candidates: hit and half-list bits
-                        uint32_t hits = 0, halves = 0;
+                        // branch-free pass over up to 32 candidates: hit bits
+                        uint32_t hits = 0;
                         const int nq = min(32, q1 - qb);
                         for (int u0 = 0; u0 < nq; u0 += 4)
                         {
@@ -182,20 +182,18 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                                 const double dx = xp.x - n.x;
                                 const double dy = xp.y - n.y;
                                 const double dz = xp.z - n.z;
-                                const bool hit = valid && dx * dx + dy * dy + dz * dz <= rsq;
-                                const bool half = (n.x > xp.x) ||
-                                                  ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
-                                hits |= uint32_t(hit) << u;
-                                halves |= uint32_t(half) << u;
+                                hits |= uint32_t(valid && dx * dx + dy * dy + dz * dz <= rsq) << u;
                             }
                         }
-                        // append the hits in slot order
+                        // append the hits in slot order with their half-list bit
                         while (hits)
                         {
                             const int u = __builtin_ctz(hits);
                             hits &= hits - 1;
-                            if (c < cap)
-                                nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (((halves >> u) & 1u) << 15));
+                            const double4 n = Staged<true>::get(st, qb + u);
+                            const bool half =
+                                (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
+                            if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
                             ++c;
                         }
                     }
