@@ -329,7 +329,14 @@ int mrmd_ljig_create(mrmd_ctx* ctx, const double* cap, const double* rc, const d
 int mrmd_ljig_destroy(mrmd_lj_idealgas* s);
 /* setCompensationEnergySamplingInterval / setCompensationEnergyUpdateInterval (LJ_IdealGas.hpp:73-80) */
 int mrmd_ljig_set_intervals(mrmd_lj_idealgas* s, int64_t sampling, int64_t update);
-/* LJ_IdealGas::run (LJ_IdealGas.cpp:227-260); synchronises for the energy */
+/* Tuning of the tiled form (no counterpart in the reference): pairs whose partner is not a
+ * row (a ghost image) deposit the partner side from a per-workgroup queue of at most
+ * `capacity` pairs, 0..4096 (default 4096); rows whose pairs do not fit deposit their own.
+ * Results do not depend on it beyond the order of FP64 atomic sums. */
+int mrmd_ljig_set_queue(mrmd_lj_idealgas* s, int capacity);
+/* LJ_IdealGas::run (LJ_IdealGas.cpp:227-260); synchronises for the energy.  Lists built
+ * over molecules of one atom each with a single-type table run the tiled full-row form;
+ * otherwise the half-list form. */
 int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* mols, mrmd_verlet_list* vl, mrmd_atoms* atoms,
                   double* energy);
 /* which: 0 compensation energy, 1 counter, 2 mean (getMeanCompensationEnergy), 200 x numTypes */

@@ -545,6 +545,10 @@ class LJ_IdealGas:
     def set_intervals(self, sampling, update):
         self.ctx.check(_L.mrmd_ljig_set_intervals(self.h, sampling, update))
 
+    def set_queue(self, capacity):
+        """tiled form: per-workgroup queue of pairs with a non-row partner (0..4096)"""
+        self.ctx.check(_L.mrmd_ljig_set_queue(self.h, int(capacity)))
+
     def run(self, molecules, verlet_list, atoms):
         e = C.c_double()
         self.ctx.check(_L.mrmd_ljig_run(self.h, molecules.h, verlet_list.h, atoms.h, C.byref(e)))

@@ -3,12 +3,15 @@
 // distribution, and the thermodynamic force (density sampling, smoothing, gradient,
 // application).
 //
-// LJ_IdealGas keeps the reference's half-list algorithm (LJ_IdealGas.cpp:52-225): the
-// weighting function is evaluated on ghost images, whose lambda need not equal that of
-// their real molecule, so the pair must be evaluated once with the ghost's lambda exactly
-// as the reference does; partner forces go through FP64 atomics.  Only molecule pairs
-// with at least one non-CG partner do pair work.  Density sampling accumulates into a
-// workgroup-private LDS histogram before one global atomic per touched bin.
+// LJ_IdealGas (LJ_IdealGas.cpp:52-225) has two forms.  The weighting function is
+// evaluated on ghost images, whose lambda need not equal that of their real molecule, so
+// a pair must be evaluated with the lambda of the two copies the reference's half list
+// pairs up.  The general form keeps the half-list algorithm (partner forces by FP64
+// atomics) for molecules of several atoms and multi-type tables.  The tiled form, for
+// lists whose every molecule is one atom and a single-type table, runs full rows from
+// LDS-staged tiles with atomics only for partners that are not rows.  Only molecule
+// pairs with at least one non-CG partner do pair work.  Density sampling accumulates into
+// a workgroup-private LDS histogram before one global atomic per touched bin.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -193,6 +196,338 @@ __global__ __launch_bounds__(BLOCK) void k_ljIdealGas(IGArgs g, double* partials
     if (threadIdx.x == 0) partials[blockIdx.x] = energy;
 }
 
+// ---- tiled form for molecule lists whose every record is one atom ----
+//
+// The half list holds each physical pair once: either as (row, partner) in the row's list
+// or, when the partner is a non-row record (a ghost image), possibly only as the mirror
+// pair (partner's real, row's image) in the real's list -- the position-lexicographic
+// discriminator picks exactly one of the two images.  A row of the full tiled list
+// therefore takes
+//   * every row partner (that partner's own row adds the other side), and
+//   * a non-row partner only when the entry carries the half-list flag; the partner's
+//     side (atom force, molecule drift force, compensation sample) then goes to the
+//     non-row record through FP64 atomics, exactly as the reference deposits it before
+//     the ghost fold.
+// Unflagged non-row partners are skipped: the mirror pair deposits both sides.  Every
+// pair is thus evaluated with the lambda of the very copies the reference pairs up, for
+// any slab placement, and the row's own sums need no atomics.
+struct IGTiled
+{
+    const uint16_t* nbr;
+    const int32_t* cnt;
+    int64_t strideS;
+    const TileInfo* tiles;
+    const int32_t* sorted;
+    const double4* igPos;  ///< sorted slot -> {atom x, y, z, lambda_mod}, sign bit set on non-rows
+    int64_t rowsEnd;       ///< molecules [0, rowsEnd) are rows
+    int queueCap;          ///< queued non-row partner pairs per workgroup (<= IG_QUEUE)
+};
+
+// sorted-slot record of each molecule's single atom and its modulated lambda (in [0, 1];
+// stored negated, -0.0 included, for molecules that are not rows)
+__global__ void k_igGather(MolView m, const double4* __restrict__ atomPos, const int32_t* __restrict__ sorted,
+                           int64_t n, int64_t rowsEnd, double4* __restrict__ out)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t k = sorted[s];
+    const double4 p = atomPos[m.offset[k]];
+    const double l = fabs(m.modLambda[k]);
+    out[s] = make_double4(p.x, p.y, p.z, k < rowsEnd ? l : -l);
+}
+
+__device__ inline int stagedSlot(const TileInfo& ti, int q)
+{
+    int r = 0;
+    while (r + 1 < ti.nr && ti.base[r + 1] <= q) ++r;
+    return ti.begin[r] + (q - ti.base[r]);
+}
+
+// pair terms of one chunk of eight entries (single-type table).  Returns the mask of
+// entries that pair the row with a flagged non-row partner.
+template <bool CAP>
+__device__ inline uint32_t igChunk(const IGArgs& g, const Pre& p, const double4& pi, const double4* st,
+                                   const uint4 cur, double& fx, double& fy, double& fz, double& ew, double& v)
+{
+#pragma clang fp contract(fast)
+    const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+    double dx[CHUNK], dy[CHUNK], dz[CHUNK], d2[CHUNK], lb[CHUNK];
+    uint32_t flag = 0;
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const uint32_t e = (wd[u >> 1] >> ((u & 1) * 16)) & 0xffffu;
+        const double4 pj = st[e & 0x7fffu];
+        dx[u] = pi.x - pj.x;
+        dy[u] = pi.y - pj.y;
+        dz[u] = pi.z - pj.z;
+        lb[u] = pj.w;
+        flag |= (e >> 15) << u;
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+    double frac2[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
+    const bool hyA = isHY(pi.w);
+    uint32_t remote = 0;
+    auto accumulate = [&](int u, double ff, double en) {
+        const bool nonRow = signbit(lb[u]);
+        const double lam = fabs(lb[u]);
+        // beyond rc, padding (d^2 = 0), both CG (LJ_IdealGas.cpp:95), unflagged non-row
+        const bool in = d2[u] <= g.rcSqr && d2[u] > 0.0 && !(isCG(pi.w) && isCG(lam)) &&
+                        (!nonRow || ((flag >> u) & 1u));
+        const double w = 0.5 * (pi.w + lam);
+        const double ffw = in ? ff * w : 0.0;
+        fx += dx[u] * ffw;
+        fy += dy[u] * ffw;
+        fz += dz[u] * ffw;
+        const double ewu = en * w;
+        ew += in ? (nonRow ? ewu : 0.5 * ewu) : 0.0;
+        v += (in && (hyA || isHY(lam))) ? 0.5 * en : 0.0;
+        remote |= uint32_t(in && nonRow) << u;
+    };
+    if constexpr (!CAP)
+    {
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const double frac6 = frac2[u] * frac2[u] * frac2[u];
+            accumulate(u, frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u], frac6 * (p.ef1 * frac6 - p.ef2) - p.shift);
+        }
+        return remote;
+    }
+    double ffv[CHUNK], env[CHUNK];
+    bool capped = false;
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const double frac6 = frac2[u] * frac2[u] * frac2[u];
+        ffv[u] = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+        env[u] = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+        capped = capped || d2[u] < p.capDistSqr;
+    }
+    // pairs inside the capping distance are rare: the capped form (LennardJones.hpp:68-75)
+    // runs, by selects over the chunk, only where some lane of the wave has one
+    if (__any(capped))
+    {
+        // keeps the block a branch: the compiler would otherwise evaluate it for every chunk
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const double dist = sqrt(d2[u]);
+            const bool c = d2[u] < p.capDistSqr;
+            ffv[u] = c ? p.capCoeff / dist : ffv[u];
+            env[u] = c ? p.eCap - (dist - p.capDist) * p.capCoeff - p.shift : env[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) accumulate(u, ffv[u], env[u]);
+    return remote;
+}
+
+// the partner side of one (row, flagged non-row) pair: atom force, drift on the partner
+// molecule and its compensation sample (LJ_IdealGas.cpp:120-190), by atomics
+template <bool CAP>
+__device__ inline void igRemotePair(const IGArgs& g, const IGTiled& tl, const TileInfo& ti, const Pre& p,
+                                    const double4& pi, const double4* st, int q)
+{
+    const MolView& M = g.m;
+    const AtomsView& A = g.a;
+    const double4 pj = st[q];
+    const double lam = fabs(pj.w);
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    double ff, en;
+    if (CAP && d2 < p.capDistSqr)
+    {
+        const double dist = sqrt(d2);
+        ff = p.capCoeff / dist;
+        en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
+    }
+    else
+    {
+        const double frac2 = 1.0 / d2;
+        const double frac6 = frac2 * frac2 * frac2;
+        ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2;
+        en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+    }
+    const double ffw = ff * (0.5 * (pi.w + lam));
+    const int32_t k = tl.sorted[stagedSlot(ti, q)];
+    const int64_t b = M.offset[k];
+    atomicAddD(&A.force[b], -(dx * ffw));
+    atomicAddD(&A.force[A.cap + b], -(dy * ffw));
+    atomicAddD(&A.force[2 * A.cap + b], -(dz * ffw));
+    if (isHY(pi.w) || isHY(lam))
+    {
+        const double Vij = 0.5 * en;
+        for (int d = 0; d < 3; ++d)
+        {
+            const double gB = M.grad[d * M.cap + k];
+            if (gB != 0.0) atomicAddD(&M.force[d * M.cap + k], -Vij * gB);
+        }
+        if (g.sampling && isHY(lam))
+        {
+            const int64_t binB = getBin(M.lambda[k], g.compMin, g.compInvBin, COMP_BINS);
+            if (binB != -1) atomicAddD(&g.comp[binB], Vij);
+        }
+    }
+}
+
+// the (row, flagged non-row) pairs of one row's chunks from ch0 on, by a second pass over
+// its entries (used once the workgroup's queue of such pairs is full)
+template <bool CAP>
+__device__ void igRemoteRow(const IGArgs& g, const IGTiled& tl, const TileInfo& ti, const Pre& p,
+                            const double4& pi, const double4* st, int s, int c, int ch0)
+{
+    const uint4* chunk = reinterpret_cast<const uint4*>(tl.nbr) + s;
+    for (int ch = ch0; ch * CHUNK < c; ++ch)
+    {
+        const uint4 cur = chunk[int64_t(ch) * tl.strideS];
+        const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+        for (int u = 0; u < CHUNK && ch * CHUNK + u < c; ++u)
+        {
+            const uint32_t e = (wd[u >> 1] >> ((u & 1) * 16)) & 0xffffu;
+            const int q = int(e & 0x7fffu);
+            const double4 pj = st[q];
+            if (!(e & 0x8000u) || !signbit(pj.w)) continue;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= g.rcSqr && d2 > 0.0 && !(isCG(pi.w) && isCG(fabs(pj.w))))
+                igRemotePair<CAP>(g, tl, ti, p, pi, st, q);
+        }
+    }
+}
+
+constexpr int IG_QUEUE = 4096;  // (row, partner) pairs with a non-row partner, per workgroup
+constexpr uint32_t IG_SKIP = 0xffffffffu;  // a queue slot reserved by a reservation that failed
+
+template <bool CAP>
+__global__ __launch_bounds__(BLOCK) void k_igTiled(IGArgs g, IGTiled tl, Pre p, double* partials)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileInfo ti;
+    __shared__ double red[BLOCK / WAVE];
+    __shared__ uint32_t queue[IG_QUEUE];  // LDS slot of the row << 16 | LDS slot of the partner
+    __shared__ int queued;
+    // consecutive tiles go to different XCDs: the pair work sits in the tiles of the
+    // atomistic and hybrid slab, one contiguous range of tile indices, which an
+    // XCD-contiguous remap would hand to two or three XCDs only
+    const unsigned blk = blockIdx.x;
+    if (threadIdx.x == 0)
+    {
+        ti = tl.tiles[blk];
+        queued = 0;
+    }
+    __syncthreads();
+    // the first row's loads overlap the staging
+    int s = -1, c = 0;
+    int32_t i = 0;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint4 cur = {0, 0, 0, 0};
+    if (int(threadIdx.x) < ti.nrows)
+    {
+        int self;
+        s = tileRow(ti, threadIdx.x, self);
+        c = tl.cnt[s];
+        i = tl.sorted[s];
+        pi = tl.igPos[s];
+        cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
+    }
+    stageTile<false>(ti, tl.igPos, st);
+    __syncthreads();
+    // a neighbourhood of coarse-grained molecules only (lambda_mod = 0 everywhere) has no
+    // pair work (LJ_IdealGas.cpp:95), no drift and no compensation term
+    int busy = 0;
+    for (int q = threadIdx.x; q < ti.staged; q += blockDim.x) busy |= st[q].w != 0.0;
+    if (!__syncthreads_or(busy))
+    {
+        if (threadIdx.x == 0) partials[blk] = 0.0;
+        return;
+    }
+    const MolView& M = g.m;
+    const AtomsView& A = g.a;
+    double eTile = 0.0;
+    for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
+    {
+        int self = 0;
+        if (t != int(threadIdx.x))
+        {
+            s = tileRow(ti, t, self);
+            c = tl.cnt[s];
+            i = tl.sorted[s];
+            pi = tl.igPos[s];
+            cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
+        }
+        else
+            tileRow(ti, t, self);
+        if (c == 0 || i >= tl.rowsEnd) continue;
+        // the row's own record sits in its column's staged range
+        const uint32_t ownSlot = uint32_t(ldsSlot(ti, ti.selfRange[self], s));
+        double fx = 0.0, fy = 0.0, fz = 0.0, v = 0.0;
+        const uint4* chunk = reinterpret_cast<const uint4*>(tl.nbr) + s;
+        const int nch = (c + CHUNK - 1) / CHUNK;
+        int overflow = -1;  // first chunk whose partner sides did not fit into the queue
+        for (int ch = 0; ch < nch; ++ch)
+        {
+            uint4 nxt = cur;
+            if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * tl.strideS];
+            uint32_t remote = igChunk<CAP>(g, p, pi, st, cur, fx, fy, fz, eTile, v);
+            if (remote && overflow < 0)
+            {
+                // the partner sides go to the workgroup's queue, shared out after the rows
+                const int n = __builtin_popcount(remote);
+                int at = atomicAdd(&queued, n);
+                if (at + n > tl.queueCap)
+                {
+                    // every later reservation fails too; slots of this one that exist are
+                    // marked so that the queue pass skips them
+                    overflow = ch;
+                    for (int k = at; k < tl.queueCap; ++k) queue[k] = IG_SKIP;
+                }
+                else
+                    while (remote)
+                    {
+                        const int b = __builtin_ctz(remote);
+                        remote &= remote - 1;
+                        const uint32_t w2 = b < 4 ? (b < 2 ? cur.x : cur.y) : (b < 6 ? cur.z : cur.w);
+                        queue[at++] = (ownSlot << 16) | ((w2 >> ((b & 1) * 16)) & 0x7fffu);
+                    }
+            }
+            cur = nxt;
+        }
+        // partner sides that did not fit are deposited by the row itself
+        if (overflow >= 0) igRemoteRow<CAP>(g, tl, ti, p, pi, st, s, c, overflow);
+        const int64_t a = M.offset[i];
+        A.force[a] += fx;
+        A.force[A.cap + a] += fy;
+        A.force[2 * A.cap + a] += fz;
+        // drift -1/2 sum_j V_ij grad(lambda_i) over hybrid pairs and the mean compensation
+        // energy (LJ_IdealGas.cpp:150-170, 208-224)
+        const double gA[3] = {M.grad[i], M.grad[M.cap + i], M.grad[2 * M.cap + i]};
+        int64_t binA = -1;
+        if (isHY(pi.w)) binA = getBin(M.lambda[i], g.compMin, g.compInvBin, COMP_BINS);
+        const double mc = binA != -1 ? g.mean[binA] : 0.0;
+        for (int d = 0; d < 3; ++d) M.force[d * M.cap + i] += -v * gA[d] + mc * gA[d];
+        if (g.sampling && binA != -1)
+        {
+            atomicAddD(&g.comp[binA], v);
+            atomicAddD(&g.compCnt[binA], 1.0);
+        }
+    }
+    __syncthreads();
+    const int nq = min(queued, tl.queueCap);
+    for (int k = threadIdx.x; k < nq; k += blockDim.x)
+    {
+        const uint32_t item = queue[k];
+        if (item == IG_SKIP) continue;
+        igRemotePair<CAP>(g, tl, ti, p, st[item >> 16], st, int(item & 0x7fffu));
+    }
+    eTile = blockSum(eTile, red);
+    if (threadIdx.x == 0) partials[blk] = eTile;
+}
+
 // updateMeanCompensationEnergy (LJ_IdealGas.cpp:21-50)
 __global__ void k_updateMeanComp(double* comp, double* cnt, double* mean, int64_t n, double factor)
 {
@@ -360,6 +695,8 @@ struct mrmd_lj_idealgas
     double rcSqr = 0.0;
     DBuf<Pre> dtable;
     DBuf<double> hist;  ///< comp | count | mean, each COMP_BINS x numTypes
+    DBuf<double4> igPos;  ///< tiled form: sorted-slot atom records
+    int queueCap = IG_QUEUE;
     int64_t runCounter = 0;
     int64_t samplingInterval = 200;
     int64_t updateInterval = 20000;
@@ -471,6 +808,14 @@ int mrmd_ljig_set_intervals(mrmd_lj_idealgas* s, int64_t sampling, int64_t updat
     MRMD_API_END
 }
 
+int mrmd_ljig_set_queue(mrmd_lj_idealgas* s, int capacity)
+{
+    MRMD_API_BEGIN(s ? s->ctx : nullptr)
+    REQUIRE(s && capacity >= 0 && capacity <= IG_QUEUE, "queue capacity must be in [0, %d]", IG_QUEUE);
+    s->queueCap = capacity;
+    MRMD_API_END
+}
+
 int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, mrmd_atoms* a, double* energy)
 {
     MRMD_API_BEGIN(s ? s->ctx : nullptr)
@@ -480,16 +825,36 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
             (long long)s->numTypes);
     REQUIRE(vl->nrows <= m->nlocal + m->nghost, "the list was built for more molecules than exist");
     mrmd_ctx* ctx = s->ctx;
-    vl->ensureHalf();
     const int64_t rows = std::min<int64_t>(m->nlocal, vl->end);
-    const unsigned grid = gridFor(rows);
+    // the tiled form serves single-type tables over lists whose molecules are single atoms
+    // (checked when the list was built, valid while the molecules' layout is unchanged)
+    const bool tiled = s->numTypes == 1 && vl->monoOf == m && vl->monoVersion == m->layoutVersion &&
+                       vl->ntiles > 0 && vl->nrows == m->nlocal + m->nghost;
+    if (!tiled) vl->ensureHalf();
+    const unsigned grid = tiled ? unsigned(vl->ntiles) : gridFor(rows);
     double* part = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid + 1) * sizeof(double)));
     const int64_t nh = COMP_BINS * s->numTypes;
     const double compBin = (1.0 - 0.0) / double(COMP_BINS);
     IGArgs g{m->view(), a->view(), vl->hnbr.p, vl->hcnt.p, vl->stride, rows, s->dtable.p, int(s->numTypes), s->rcSqr,
              (s->runCounter % s->samplingInterval) == 0 ? 1 : 0, 0.0, 1.0 / compBin, s->hist.p, s->hist.p + nh,
              s->hist.p + 2 * nh};
-    if (rows > 0)
+    if (rows > 0 && tiled)
+    {
+        KTimer t(ctx, "lj_idealgas_tiled");
+        s->igPos.ensure(ctx, vl->nrows);
+        hipLaunchKernelGGL(k_igGather, dim3(gridFor(vl->nrows)), dim3(BLOCK), 0, ctx->stream, m->view(), a->pos.p,
+                           vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
+        const IGTiled tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p, s->igPos.p, rows,
+                         s->queueCap};
+        const Pre p = s->table[0];
+        const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
+        if (p.capDistSqr > 0.0)
+            hipLaunchKernelGGL(k_igTiled<true>, dim3(grid), dim3(vl->block), lds, ctx->stream, g, tl, p, part);
+        else
+            hipLaunchKernelGGL(k_igTiled<false>, dim3(grid), dim3(vl->block), lds, ctx->stream, g, tl, p, part);
+        hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), part + grid);
+    }
+    else if (rows > 0)
     {
         KTimer t(ctx, "lj_idealgas");
         hipLaunchKernelGGL(k_ljIdealGas, dim3(grid), dim3(BLOCK), 0, ctx->stream, g, part);

@@ -500,19 +500,8 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
     for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u) frac2[u] = 1.0 / d2[u];
-#pragma unroll
-    for (int u = 0; u < CHUNK; ++u)
-    {
-        const Pre& p = b.lj;
-        const double frac6 = frac2[u] * frac2[u] * frac2[u];
-        double ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
-        double en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
-        if (CAP && d2[u] < p.capDistSqr)
-        {
-            const double dist = sqrt(d2[u]);
-            ff = p.capCoeff / dist;
-            en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
-        }
+    const Pre& p = b.lj;
+    auto accumulate = [&](int u, double ff, double en) {
         // both partners coarse-grained: no interaction (LJ_IdealGas.cpp:95)
         const bool in = d2[u] <= b.rcSqr && d2[u] > 0.0 && !(pi.w <= 0.0 && lj[u] <= 0.0);
         const double w = 0.5 * (pi.w + lj[u]);
@@ -522,7 +511,44 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
         fz += dz[u] * ffw;
         ew += in ? en * w : 0.0;
         e0 += in ? en : 0.0;
+    };
+    if constexpr (!CAP)
+    {
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const double frac6 = frac2[u] * frac2[u] * frac2[u];
+            accumulate(u, frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u], frac6 * (p.ef1 * frac6 - p.ef2) - p.shift);
+        }
+        return;
     }
+    double ffv[CHUNK], env[CHUNK];
+    bool capped = false;
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u)
+    {
+        const double frac6 = frac2[u] * frac2[u] * frac2[u];
+        ffv[u] = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+        env[u] = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+        capped = capped || d2[u] < p.capDistSqr;
+    }
+    // the capped form (LennardJones.hpp:68-75) only for chunks where some lane of the wave
+    // needs it
+    if (__any(capped))
+    {
+        // keeps the block a branch: the compiler would otherwise evaluate it for every chunk
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const double dist = sqrt(d2[u]);
+            const bool c = d2[u] < p.capDistSqr;
+            ffv[u] = c ? p.capCoeff / dist : ffv[u];
+            env[u] = c ? p.eCap - (dist - p.capDist) * p.capCoeff - p.shift : env[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; ++u) accumulate(u, ffv[u], env[u]);
 }
 
 template <bool CAP>

@@ -260,6 +260,7 @@ void mrmd_molecules::ensureCap(int64_t need)
 
 void mrmd_molecules::setSize(int64_t n)
 {
+    ++layoutVersion;
     ensureCap(n);
     if (n > size)
     {
@@ -696,6 +697,7 @@ int mrmd_molecules_set_counts(mrmd_molecules* m, int64_t nl, int64_t ng)
     if (nl + ng > m->size) m->setSize(nl + ng);
     m->nlocal = nl;
     m->nghost = ng;
+    ++m->layoutVersion;
     MRMD_API_END
 }
 
@@ -705,6 +707,7 @@ int mrmd_molecules_upload(mrmd_molecules* m, int field, const void* host, int64_
     REQUIRE(m && host && off >= 0 && n >= 0 && field >= 0 && field <= MRMD_MOL_NUMATOMS, "invalid arguments");
     if (n == 0) return MRMD_OK;
     if (off + n > m->size) m->setSize(off + n);
+    if (field == MRMD_MOL_OFFSET || field == MRMD_MOL_NUMATOMS) ++m->layoutVersion;
     mrmd_ctx* ctx = m->ctx;
     const bool vec = field == MRMD_MOL_POS || field == MRMD_MOL_FORCE || field == MRMD_MOL_GRADLAMBDA;
     m->stage.ensure(ctx, (vec ? 3 : 1) * n);
@@ -791,6 +794,7 @@ int mrmd_molecules_fill(mrmd_molecules* m, int field, double value)
     mrmd_ctx* ctx = m->ctx;
     const int64_t n = m->size;
     if (n == 0) return MRMD_OK;
+    if (field == MRMD_MOL_OFFSET || field == MRMD_MOL_NUMATOMS) ++m->layoutVersion;
     const dim3 g(gridFor(n)), b(BLOCK);
     switch (field)
     {
@@ -849,5 +853,6 @@ extern "C" int mrmd_molecules_for_each_atom(mrmd_molecules* m, const mrmd_atoms*
     }
     m->nlocal = a->nlocal;
     m->nghost = a->nghost;
+    ++m->layoutVersion;
     MRMD_API_END
 }

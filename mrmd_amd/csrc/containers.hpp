@@ -76,6 +76,9 @@ struct mrmd_molecules
     mrmd_hip::DBuf<double> force, lambda, modLambda, grad;
     mrmd_hip::DBuf<int64_t> offset, numAtoms;
     mrmd_hip::DBuf<double> stage;
+    /// bumped by every call that may change the record count or an offset / atom count
+    /// (lists built over the molecules remember the value they saw)
+    uint64_t layoutVersion = 0;
 
     mrmd_hip::MolView view() const
     {

@@ -468,6 +468,7 @@ void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const
     }
     gl->corrSize = newA;
     m->nghost += h[0] + h[2];
+    ++m->layoutVersion;
     a->nghost += h[1] + h[3];
 }
 
@@ -616,6 +617,7 @@ int mrmd_multires_halo_create(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atom
     REQUIRE(gl && m && a, "invalid arguments");
     checkBox(box);
     m->nghost = 0;
+    ++m->layoutVersion;
     resetCorr(gl, a->nlocal);
     a->nghost = 0;
     for (int d = 0; d < 3; ++d) molCreateAxis(gl, m, a, box, d);

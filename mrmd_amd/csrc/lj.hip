@@ -88,24 +88,12 @@ struct TiledList
     const double4* sortedPos;
 };
 
-// 1 / x from the hardware reciprocal estimate refined by two Newton-Raphson steps
-// (e = 1 - x r, r <- r + r e): each step squares the relative error, so the result is
-// within an ulp or two of the correctly rounded quotient.  x = 0 gives a NaN, which the
-// callers mask.
-__device__ inline double recipNewton(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-
 // Branch-free pair terms of one chunk of eight entries, written stage by stage over the
 // eight pairs so that the eight dependency chains (LDS read, distance, reciprocal, force)
 // are independent and interleave.  Padding entries (the row itself, d^2 = 0) and pairs
 // beyond rc select zero.  CAP: the table has a capping distance, so the capped branch
-// (LennardJones.hpp:68-75) is evaluated too.  Products and sums may contract to FMAs here
+// (LennardJones.hpp:68-75) is evaluated too (single-type tables: only for the chunks where
+// a lane of the wave needs it).  Products and sums may contract to FMAs here
 // (the neighbour decision of the list is unaffected: it is made, unfused, at build time);
 // the forces then differ from the unfused CPU arithmetic by a few ulps per term.
 template <bool PRED, bool ONE, bool CAP>
@@ -132,19 +120,7 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
     double frac2[CHUNK];
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
-#pragma unroll
-    for (int u = 0; u < CHUNK; ++u)
-    {
-        const Pre& p = ONE ? p1 : tab[ti0 + max(int(ty[u]), 0)];
-        const double frac6 = frac2[u] * frac2[u] * frac2[u];
-        double ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
-        double en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
-        if (CAP && d2[u] < p.capDistSqr)
-        {
-            const double dist = sqrt(d2[u]);
-            ff = p.capCoeff / dist;
-            en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
-        }
+    auto accumulate = [&](int u, double ff, double en) {
         bool in = d2[u] <= a.rcSqr && d2[u] > 0.0;
         if (PRED) in = in && ty[u] >= 0.0;
         ff = in ? ff : 0.0;
@@ -154,6 +130,58 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
         fz += dz[u] * ff;
         e += en;
         w += ff * d2[u];
+    };
+    if constexpr (!CAP || !ONE)
+    {
+        // multi-type tables keep the capped branch per pair (the split form below would hold
+        // two table records per pair and cost occupancy)
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const Pre& p = ONE ? p1 : tab[ti0 + max(int(ty[u]), 0)];
+            const double frac6 = frac2[u] * frac2[u] * frac2[u];
+            double ff = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+            double en = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+            if (CAP && d2[u] < p.capDistSqr)
+            {
+                const double dist = sqrt(d2[u]);
+                ff = p.capCoeff / dist;
+                en = p.eCap - (dist - p.capDist) * p.capCoeff - p.shift;
+            }
+            accumulate(u, ff, en);
+        }
+    }
+    else
+    {
+        double ffv[CHUNK], env[CHUNK];
+        bool capped = false;
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            const Pre& p = ONE ? p1 : tab[ti0 + max(int(ty[u]), 0)];
+            const double frac6 = frac2[u] * frac2[u] * frac2[u];
+            ffv[u] = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
+            env[u] = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
+            capped = capped || d2[u] < p.capDistSqr;
+        }
+        // pairs inside the capping distance are rare: the capped form runs, by selects over
+        // the chunk, only where some lane of the wave has one
+        if (__any(capped))
+        {
+            // keeps the block a branch: the compiler would otherwise evaluate it every chunk
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < CHUNK; ++u)
+            {
+                const Pre& p = ONE ? p1 : tab[ti0 + max(int(ty[u]), 0)];
+                const double dist = sqrt(d2[u]);
+                const bool c = d2[u] < p.capDistSqr;
+                ffv[u] = c ? p.capCoeff / dist : ffv[u];
+                env[u] = c ? p.eCap - (dist - p.capDist) * p.capCoeff - p.shift : env[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) accumulate(u, ffv[u], env[u]);
     }
 }
 

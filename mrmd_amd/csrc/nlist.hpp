@@ -97,6 +97,9 @@ __device__ inline int tileRow(const TileInfo& ti, int t, int& self)
     return ti.selfBegin[r] + (t - ti.selfPrefix[r]);
 }
 
+// LDS slot of sorted slot s that lies in staged range r of the tile
+__device__ inline int ldsSlot(const TileInfo& ti, int r, int s) { return ti.base[r] + (s - ti.begin[r]); }
+
 // Neighbour entries are 16-bit LDS slots stored in chunks of 8 per row: entry k of sorted
 // slot s sits at ((k / 8) * strideS + s) * 8 + k % 8, so one 16-byte load per lane brings
 // 8 entries and a wave's load is 1 KB contiguous.
@@ -201,6 +204,10 @@ struct mrmd_verlet_list
     mrmd_hip::DBuf<int32_t> hnbr;  ///< half rows (Cabana discriminator), same layout
     mrmd_hip::DBuf<int32_t> hcnt;
     mrmd_hip::Binning bin;
+
+    // ---- molecule lists whose every record is one atom (checked at build time) ----
+    const mrmd_molecules* monoOf = nullptr;  ///< the molecules, when all records were monatomic
+    uint64_t monoVersion = 0;                ///< their layout version at that build
 
     /// copy the current positions into sorted-slot order (bin.sortedPos) for the tiled kernels
     void refreshSorted(const double4* pos);

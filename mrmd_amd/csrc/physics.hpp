@@ -12,6 +12,19 @@ namespace mrmd_hip
 {
 constexpr double PI = 3.14159265358979323846;
 
+// 1 / x from the hardware reciprocal estimate refined by two Newton-Raphson steps
+// (e = 1 - x r, r <- r + r e): each step squares the relative error, so the result is
+// within an ulp or two of the correctly rounded quotient.  x = 0 gives a NaN, which the
+// callers mask.
+__device__ inline double recipNewton(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // PrecomputedValues (LennardJones.hpp:27-39)
 struct Pre
 {

@@ -84,9 +84,6 @@ __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets,
     atomicMax(maxStaged, staged);
 }
 
-// LDS slot of sorted slot s that lies in column range r of the tile
-__device__ inline int ldsSlot(const TileInfo& ti, int r, int s) { return ti.base[r] + (s - ti.begin[r]); }
-
 constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
 
 // One lane per row.  Every row scans its stencil cells in Cabana's i, j, k order with the
@@ -333,8 +330,17 @@ void setupTiles(mrmd_verlet_list* vl, int K)
     vl->maxStaged = h[0];
 }
 
-void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t begin, int64_t end, double rn,
-                 double ratio, const double* gmin, const double* gmax, int64_t maxNeigh)
+// records whose atom count is not 1 (molecule lists only)
+__global__ void k_countNotOne(const int64_t* numAtoms, int64_t n, int32_t* out)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n && numAtoms[k] != 1) atomicAdd(out, 1);
+}
+
+// numAtoms (molecule lists) is checked for records of one atom; returns whether all are
+bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t begin, int64_t end, double rn,
+                 double ratio, const double* gmin, const double* gmax, int64_t maxNeigh,
+                 const int64_t* numAtoms = nullptr)
 {
     mrmd_ctx* ctx = vl->ctx;
     REQUIRE(begin >= 0 && begin <= end && end <= nall, "invalid row range [%lld, %lld) of %lld", (long long)begin,
@@ -387,6 +393,11 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     if (vl->capFull <= 0) vl->capFull = int32_t(std::max<int64_t>(2 * maxNeigh + 16, 96));
     vl->capFull = (vl->capFull + CHUNK - 1) / CHUNK * CHUNK;
     int32_t* dmax = static_cast<int32_t*>(ctxScratch(ctx, 2, 64)) + 1;
+    int32_t* dmulti = dmax + 1;
+    hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmulti, int64_t(1));
+    if (numAtoms && nall > 0)
+        hipLaunchKernelGGL(k_countNotOne, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, numAtoms, nall, dmulti);
+    bool mono = false;
     const size_t lds = size_t(std::max(vl->maxStaged, 1)) * Staged<true>::bytes;
     for (int attempt = 0; attempt < 2; ++attempt)
     {
@@ -407,15 +418,17 @@ void verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
         }
         checkLaunch();
         int32_t* hp = reinterpret_cast<int32_t*>(ctx->pinned);
-        HIPCHECK(hipMemcpyAsync(hp, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(hp, dmax, 8, hipMemcpyDeviceToHost, ctx->stream));
         sync(ctx);
         const int32_t hmax = hp[0];
+        mono = hp[1] == 0;
         vl->maxFull = hmax;
         if (hmax <= vl->capFull) break;
         // Cabana VerletLayout2D: re-count and refill with the larger width
         vl->capFull = (hmax + hmax / 8 + 8 + CHUNK - 1) / CHUNK * CHUNK;
     }
     if (vl->maxFull > vl->capFull) fail(MRMD_ERR_STATE, "neighbour rows did not fit after a re-count");
+    return mono;
 }
 }  // namespace
 
@@ -506,6 +519,7 @@ int mrmd_verlet_build(mrmd_verlet_list* vl, const mrmd_atoms* a, int64_t begin, 
 {
     MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
     REQUIRE(vl && a && gmin && gmax, "invalid arguments");
+    vl->monoOf = nullptr;
     verletBuild(vl, a->pos.p, a->nlocal + a->nghost, begin, end, rn, ratio, gmin, gmax, maxNeigh);
     MRMD_API_END
 }
@@ -515,7 +529,12 @@ int mrmd_verlet_build_molecules(mrmd_verlet_list* vl, const mrmd_molecules* m, i
 {
     MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
     REQUIRE(vl && m && gmin && gmax, "invalid arguments");
-    verletBuild(vl, m->pos.p, m->nlocal + m->nghost, begin, end, rn, ratio, gmin, gmax, maxNeigh);
+    vl->monoOf = nullptr;
+    if (verletBuild(vl, m->pos.p, m->nlocal + m->nghost, begin, end, rn, ratio, gmin, gmax, maxNeigh, m->numAtoms.p))
+    {
+        vl->monoOf = m;
+        vl->monoVersion = m->layoutVersion;
+    }
     MRMD_API_END
 }
 

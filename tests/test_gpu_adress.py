@@ -123,11 +123,25 @@ def _slab_system(M, ctx, n=6000, Lx=40.0, Lyz=12.0, seed=11):
     return pos, types
 
 
-def test_adress_pipeline_matches_oracle(M, ctx):
+@pytest.mark.parametrize("num_types,centre,queue", [(2, 20.0, None), (1, 20.0, None), (1, 3.0, None),
+                                                   (1, 3.0, 0), (1, 3.0, 5)],
+                         ids=["two-types-half-list", "one-type-tiled", "one-type-tiled-slab-across-boundary",
+                              "one-type-tiled-no-queue", "one-type-tiled-queue-overflow"])
+def test_adress_pipeline_matches_oracle(M, ctx, num_types, centre, queue):
     """one full AdResS force evaluation: multi-res ghosts, molecules, molecule list,
-    LJ_IdealGas (sampling run), molecule -> atom forces, ghost fold, thermodynamic force"""
+    LJ_IdealGas (sampling run), molecule -> atom forces, ghost fold, thermodynamic force.
+
+    Two types run the half-list kernel; one type runs the tiled kernel.  With the slab
+    centred at x = 3 the atomistic and hybrid regions cross the periodic boundary, so ghost
+    images carry a lambda different from their real molecule's (ghosts at x < 0 are hybrid
+    or atomistic, their reals near x = 40 coarse-grained): the tiled kernel must pair the
+    same copies as the reference's half list.  Queue capacities 0 and 5 send the partner
+    sides of ghost pairs through the rows' own second pass (all of them, or those past the
+    first few)."""
     Lx, Lyz, rn = 40.0, 12.0, 2.8
     pos, types = _slab_system(M, ctx, Lx=Lx, Lyz=Lyz)
+    if num_types == 1:
+        types = np.zeros_like(types)
     n = pos.shape[0]
     gbox = M.make_box([0, 0, 0], [Lx, Lyz, Lyz], rn)
     obox = O.make_box([0, 0, 0], [Lx, Lyz, Lyz], rn)
@@ -135,8 +149,8 @@ def test_adress_pipeline_matches_oracle(M, ctx):
     oa = O.Atoms.from_arrays(pos, type=types)
     gm = M.Molecules.for_each_atom(ctx, ga)
     om = O.Molecules.for_each_atom(oa)
-    slab = M.Slab([Lx / 2, 0, 0], 8.0, 5.0, 1)
-    os_ = O.make_slab([Lx / 2, 0, 0], 8.0, 5.0, 1)
+    slab = M.Slab([centre, 0, 0], 8.0, 5.0, 1)
+    os_ = O.make_slab([centre, 0, 0], 8.0, 5.0, 1)
     gm.update(ga, None)
     O.lib().or_update_molecules(om.h, oa.h, None)
     gl = M.GhostLayer(ctx)
@@ -158,17 +172,31 @@ def test_adress_pipeline_matches_oracle(M, ctx):
     ov = O.Verlet()
     ov.build(om, nm, 0, om.nlocal, rn, 1.0, lo, hi, molecules=True)
     assert vl.num_pairs() == ov.total()
-    cap, rc, sig, eps = [0.7] * 4, [2.5] * 4, [1.0, 0.95, 0.95, 0.9], [1.0, 0.8, 0.8, 0.6]
-    glj = M.LJ_IdealGas(ctx, cap, rc, sig, eps, True, num_types=2)
-    olj = O.LJIdealGas(cap, rc, sig, eps, 2, shifted=True)
+    if num_types == 2:
+        cap, rc, sig, eps = [0.7] * 4, [2.5] * 4, [1.0, 0.95, 0.95, 0.9], [1.0, 0.8, 0.8, 0.6]
+    else:
+        cap, rc, sig, eps = [0.7], [2.5], [1.0], [1.0]
+    glj = M.LJ_IdealGas(ctx, cap, rc, sig, eps, True, num_types=num_types)
+    olj = O.LJIdealGas(cap, rc, sig, eps, num_types, shifted=True)
+    if queue is not None:
+        glj.set_queue(queue)
     ga.set_force(0.0)
     oa.set(O.FORCE, np.zeros((na, 3)))
     gm.set_force(0.0)
     O.lib().or_mol_set_force(om.h, 0.0)
     O.lib().or_set_threads(4)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
     Eg = glj.run(gm, vl, ga)
+    tiled_runs = ctx.kernel_stats("lj_idealgas_tiled")[0]
+    ctx.set_profiling(False)
+    assert tiled_runs == (1 if num_types == 1 else 0)
     Ec = olj.run(om, ov, oa)
     assert Eg == pytest.approx(Ec, rel=1e-12)
+    if centre < 10.0:
+        # the case is meant: some ghost images are non-CG while their reals are CG
+        lam = gm.download(M.M_MODLAMBDA, nm)
+        assert (lam[gm.nlocal:] > 0.0).any()
     _close(ga.download(M.FORCE, na), oa.get(O.FORCE, na))
     _close(gm.download(M.M_FORCE, nm), om.get(O.M_FORCE, nm))
     for which in (0, 1, 2):
