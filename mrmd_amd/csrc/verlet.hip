@@ -87,8 +87,8 @@ __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets,
 constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
 
 // One lane per row.  Every row scans its stencil cells in Cabana's i, j, k order with the
-// min-distance prune, four candidates per pass (independent LDS reads and distances, then
-// the appends in slot order).
+// min-distance prune (the surviving cells of a column as one slot range), eight candidates
+// per pass (independent LDS reads and distances, then the appends in slot order).
 template <int R>
 __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict__ sortedPos,
                                                        const int32_t* __restrict__ sorted,
@@ -96,14 +96,18 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                                                        const int32_t* __restrict__ offsets,
                                                        const TileInfo* __restrict__ tiles, TileGeom tg, GridD g,
                                                        int64_t rowLo, int64_t rowHi, double rsq, uint16_t* nbr,
-                                                       int64_t strideS, int cap, int32_t* cnt, int32_t* maxCnt)
+                                                       int64_t strideS, int cap, int32_t* cnt, int32_t* maxCnt,
+                                                       int stagedCap)
 {
     extern __shared__ double4 stRaw[];
     void* st = stRaw;
     __shared__ TileInfo ti;
-    // LDS slot of the first record of cell kl + z of range r: cstart[r * (MAX_SPAN + 1) + z];
-    // entry z = nk closes the range (ranges outside the grid are empty)
-    __shared__ int cstart[MAX_RANGES * (MAX_SPAN + 1)];
+    // after the staged records: the LDS slot of the first record of cell kl + z of range r,
+    // cstart[r * cz + z] with cz = K + 2R + 1; entry z = nk closes the range (ranges outside
+    // the grid are empty).  Sized to the tile height, so that short tiles leave LDS for more
+    // workgroups.
+    const int cz = tg.K + 2 * R + 1;
+    int* cstart = reinterpret_cast<int*>(static_cast<double*>(st) + 3 * int64_t(stagedCap));
     const int64_t t = blockIdx.x;
     if (threadIdx.x == 0) ti = tiles[t];
     __syncthreads();
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
             else
                 v = r + 1 < ti.nr ? ti.base[r + 1] : ti.staged;
         }
-        cstart[r * (MAX_SPAN + 1) + z] = v;
+        cstart[r * cz + z] = v;
     }
     __syncthreads();
     int localMax = 0;
@@ -158,41 +162,43 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
         for (int ii = max(i - R, 0); ii < min(i + R + 1, tg.n0); ++ii)
             for (int jj = max(j - R, 0); jj < min(j + R + 1, tg.n1); ++jj)
             {
-                const int* cs0 = cstart + ((ii - (i0 - R)) * span + (jj - (j0 - R))) * (MAX_SPAN + 1);
-                for (int kk = max(ck - R, 0); kk < min(ck + R + 1, tg.n2); ++kk)
+                const int* cs0 = cstart + ((ii - (i0 - R)) * span + (jj - (j0 - R))) * cz;
+                // Cabana's min-distance prune per cell: along z the distance to a cell is
+                // unimodal in k (cell centres are monotone, the x and y terms fixed), so the
+                // cells that survive form one run [kLo, kHi] and the candidates one slot range
+                int kLo = max(ck - R, 0), kHi = min(ck + R + 1, tg.n2) - 1;
+                while (kLo <= kHi && g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kLo) > rsq) ++kLo;
+                while (kHi >= kLo && g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kHi) > rsq) --kHi;
+                if (kLo > kHi) continue;
+                const int q1 = cs0[kHi + 1 - kl];
+                for (int qb = cs0[kLo - kl]; qb < q1; qb += 32)
                 {
-                    if (g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kk) > rsq) continue;
-                    const int* cs = cs0 + (kk - kl);
-                    const int q1 = cs[1];
-                    for (int qb = cs[0]; qb < q1; qb += 32)
+                    // branch-free passes over up to 32 candidates: hit bits
+                    uint32_t hits = 0;
+                    const int nq = min(32, q1 - qb);
+                    for (int u0 = 0; u0 < nq; u0 += 8)
                     {
-                        // branch-free pass over up to 32 candidates: hit bits
-                        uint32_t hits = 0;
-                        const int nq = min(32, q1 - qb);
-                        for (int u0 = 0; u0 < nq; u0 += 4)
-                        {
 #pragma unroll
-                            for (int u = u0; u < u0 + 4; ++u)
-                            {
-                                const bool valid = u < nq && qb + u != qSelf;
-                                const double4 n = Staged<true>::get(st, valid ? qb + u : qSelf);
-                                const double dx = xp.x - n.x;
-                                const double dy = xp.y - n.y;
-                                const double dz = xp.z - n.z;
-                                hits |= uint32_t(valid && dx * dx + dy * dy + dz * dz <= rsq) << u;
-                            }
-                        }
-                        // append the hits in slot order with their half-list bit
-                        while (hits)
+                        for (int u = u0; u < u0 + 8; ++u)
                         {
-                            const int u = __builtin_ctz(hits);
-                            hits &= hits - 1;
-                            const double4 n = Staged<true>::get(st, qb + u);
-                            const bool half =
-                                (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
-                            if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
-                            ++c;
+                            const bool valid = u < nq && qb + u != qSelf;
+                            const double4 n = Staged<true>::get(st, valid ? qb + u : qSelf);
+                            const double dx = xp.x - n.x;
+                            const double dy = xp.y - n.y;
+                            const double dz = xp.z - n.z;
+                            hits |= uint32_t(valid && dx * dx + dy * dy + dz * dz <= rsq) << u;
                         }
+                    }
+                    // append the hits in slot order with their half-list bit
+                    while (hits)
+                    {
+                        const int u = __builtin_ctz(hits);
+                        hits &= hits - 1;
+                        const double4 n = Staged<true>::get(st, qb + u);
+                        const bool half =
+                            (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
+                        if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
+                        ++c;
                     }
                 }
             }
@@ -398,7 +404,10 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     if (numAtoms && nall > 0)
         hipLaunchKernelGGL(k_countNotOne, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, numAtoms, nall, dmulti);
     bool mono = false;
-    const size_t lds = size_t(std::max(vl->maxStaged, 1)) * Staged<true>::bytes;
+    // staged records, then the per-cell start table of (a + 2R)^2 ranges x (K + 2R + 1)
+    const int stagedCap = std::max(vl->maxStaged, 1);
+    const size_t lds = size_t(stagedCap) * Staged<true>::bytes +
+                       size_t((vl->tileCols + 2 * R) * (vl->tileCols + 2 * R) * (vl->tileK + 2 * R + 1)) * sizeof(int);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         vl->nbr16.ensure(ctx, int64_t(vl->capFull) * vl->strideS);
@@ -409,12 +418,12 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                 hipLaunchKernelGGL(k_verletTiled<1>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
                                    vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
                                    vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax);
+                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
             else
                 hipLaunchKernelGGL(k_verletTiled<2>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
                                    vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
                                    vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax);
+                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
         }
         checkLaunch();
         int32_t* hp = reinterpret_cast<int32_t*>(ctx->pinned);
