@@ -85,21 +85,68 @@ __global__ void k_binFill(const int32_t* cellOf, int64_t begin, int64_t n, const
     if (valid) sorted[offsets[c] + base + (lane - r.head)] = int32_t(begin + k);
 }
 
-// restore ascending record order inside each cell (insertion sort; cells hold tens of
-// records, a heap sort covers pathological crowding)
-__global__ void k_binSortCells(const int32_t* offsets, const int32_t* counts, int32_t* sorted, int ncell)
+// order the records inside each cell: ascending record index, or (Z) ascending z with the
+// record index breaking ties.  A workgroup takes 128 consecutive cells, whose records are
+// one contiguous run of the sorted array: the run's keys are loaded into LDS once, every
+// lane insertion-sorts its cell there (cells hold tens of records), and the run is written
+// back.  A run longer than the LDS buffer (pathological crowding) sorts in place in global
+// memory, with a heap sort for crowded cells.
+constexpr int SORT_RUN = 4096;
+constexpr int SORT_CELLS = 128;  // cells per workgroup: ~20 records each fill a run of ~2500
+
+template <bool Z>
+__global__ __launch_bounds__(SORT_CELLS) void k_binSortCells(const int32_t* offsets, const int32_t* counts, int32_t* sorted,
+                                                        int ncell, const double4* __restrict__ pos)
 {
-    const int c = int(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    __shared__ int32_t idx[SORT_RUN];
+    __shared__ double key[Z ? SORT_RUN : 1];
+    const int c0 = int(blockIdx.x) * SORT_CELLS;
+    const int c1 = min(c0 + SORT_CELLS, ncell);
+    const int r0 = offsets[c0], r1 = offsets[c1 - 1] + counts[c1 - 1];
+    const int c = c0 + int(threadIdx.x);
+    auto less = [](double zx, int32_t x, double zy, int32_t y) { return Z ? (zx < zy || (zx == zy && x < y)) : x < y; };
+    if (r1 - r0 <= SORT_RUN)
+    {
+        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS)
+        {
+            const int32_t v = sorted[r0 + k];
+            idx[k] = v;
+            if (Z) key[k] = pos[v].z;
+        }
+        __syncthreads();
+        if (c < c1)
+        {
+            const int o = offsets[c] - r0, n = counts[c];
+            for (int i = 1; i < n; ++i)
+            {
+                const int32_t v = idx[o + i];
+                const double zv = Z ? key[o + i] : 0.0;
+                int j = i - 1;
+                while (j >= 0 && less(zv, v, Z ? key[o + j] : 0.0, idx[o + j]))
+                {
+                    idx[o + j + 1] = idx[o + j];
+                    if (Z) key[o + j + 1] = key[o + j];
+                    --j;
+                }
+                idx[o + j + 1] = v;
+                if (Z) key[o + j + 1] = zv;
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS) sorted[r0 + k] = idx[k];
+        return;
+    }
+    if (c >= c1) return;
     int32_t* a = sorted + offsets[c];
     const int n = counts[c];
+    auto lessRec = [&](int32_t x, int32_t y) { return less(Z ? pos[x].z : 0.0, x, Z ? pos[y].z : 0.0, y); };
     if (n <= 96)
     {
         for (int i = 1; i < n; ++i)
         {
             const int32_t v = a[i];
             int j = i - 1;
-            while (j >= 0 && a[j] > v)
+            while (j >= 0 && lessRec(v, a[j]))
             {
                 a[j + 1] = a[j];
                 --j;
@@ -115,8 +162,8 @@ __global__ void k_binSortCells(const int32_t* offsets, const int32_t* counts, in
         while (2 * root + 1 < end)
         {
             int child = 2 * root + 1;
-            if (child + 1 < end && a[child] < a[child + 1]) ++child;
-            if (a[root] < a[child])
+            if (child + 1 < end && lessRec(a[child], a[child + 1])) ++child;
+            if (lessRec(a[root], a[child]))
             {
                 const int32_t t = a[root];
                 a[root] = a[child];
@@ -150,7 +197,8 @@ __global__ void k_zeroI32(int32_t* p, int64_t n)
 }
 }  // namespace
 
-void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos)
+void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
+                    bool zOrder)
 {
     grid = g;
     begin = b;
@@ -172,8 +220,12 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
     {
         hipLaunchKernelGGL(k_binFill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, cellOf.p, b, n, offsets.p,
                            fill.p, sorted.p);
-        hipLaunchKernelGGL(k_binSortCells, dim3(gridFor(ncell)), dim3(BLOCK), 0, ctx->stream, offsets.p, counts.p,
-                           sorted.p, ncell);
+        if (zOrder)
+            hipLaunchKernelGGL(k_binSortCells<true>, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
+                               ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
+        else
+            hipLaunchKernelGGL(k_binSortCells<false>, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
+                               ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
     }
     if (withPos && n > 0)
     {

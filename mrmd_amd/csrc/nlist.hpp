@@ -48,7 +48,9 @@ struct GridD
 GridD makeGrid(const double* gmin, const double* gmax, const double* delta);
 
 // binning of records [begin, end) of a double4 position array into a grid; the sorted
-// order is ascending record index inside every cell (a stable counting sort)
+// order is ascending record index inside every cell (a stable counting sort), or with
+// zOrder ascending z (record index breaking ties), which makes every column of cells one
+// z-sorted run
 struct Binning
 {
     GridD grid{};
@@ -60,7 +62,8 @@ struct Binning
     DBuf<int32_t> sorted;   ///< slot -> record index
     DBuf<double4> sortedPos;
 
-    void build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos);
+    void build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
+               bool zOrder = false);
 };
 }  // namespace mrmd_hip
 

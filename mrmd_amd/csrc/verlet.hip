@@ -87,8 +87,9 @@ __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets,
 constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
 
 // One lane per row.  Every row scans its stencil cells in Cabana's i, j, k order with the
-// min-distance prune (the surviving cells of a column as one slot range), eight candidates
-// per pass (independent LDS reads and distances, then the appends in slot order).
+// min-distance prune (the surviving cells of a column as one slot range, cut by binary
+// search to the z window of the row, the records being z-sorted), eight candidates per pass
+// (independent LDS reads and distances, then the appends in slot order).
 template <int R>
 __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict__ sortedPos,
                                                        const int32_t* __restrict__ sorted,
@@ -170,8 +171,32 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                 while (kLo <= kHi && g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kLo) > rsq) ++kLo;
                 while (kHi >= kLo && g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kHi) > rsq) --kHi;
                 if (kLo > kHi) continue;
-                const int q1 = cs0[kHi + 1 - kl];
-                for (int qb = cs0[kLo - kl]; qb < q1; qb += 32)
+                // records inside a column run are z-sorted: the candidates that can pass are
+                // those whose (dz)^2, as the test computes it, stays within rsq; d^2 >= (dz)^2
+                // for the rounded sums, so the cut changes no decision
+                int lo = cs0[kLo - kl], hi = cs0[kHi + 1 - kl];
+                {
+                    int a = lo, b = hi;  // first slot not below the window
+                    while (a < b)
+                    {
+                        const int m = (a + b) >> 1;
+                        const double dz = xp.z - Staged<true>::get(st, m).z;
+                        if (dz > 0.0 && dz * dz > rsq) a = m + 1;
+                        else b = m;
+                    }
+                    lo = a;
+                    b = hi;  // first slot above the window
+                    while (a < b)
+                    {
+                        const int m = (a + b) >> 1;
+                        const double dz = xp.z - Staged<true>::get(st, m).z;
+                        if (dz < 0.0 && dz * dz > rsq) b = m;
+                        else a = m + 1;
+                    }
+                    hi = a;
+                }
+                const int q1 = hi;
+                for (int qb = lo; qb < q1; qb += 32)
                 {
                     // branch-free passes over up to 32 candidates: hit bits
                     uint32_t hits = 0;
@@ -364,7 +389,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     const double gs = ratio * rn;
     const double delta[3] = {gs, gs, gs};
     const GridD g = makeGrid(gmin, gmax, delta);
-    vl->bin.build(ctx, pos, 0, nall, g, true);
+    vl->bin.build(ctx, pos, 0, nall, g, true, true);
     vl->globalValid = false;
     vl->halfValid = false;
     // tile shape: a x a columns (a <= 3; 1 when the stencil reaches two cells), height K
