@@ -16,7 +16,10 @@ divided by the kernel's mean duration from HIP events on the library stream).
 atomic F_j, OpenMP) on a bounded sample of the same system.  `batched_vs_serial_speedup`
 compares the ensemble engine on a 256-replica ensemble with the same replicas run one after
 another through the single-system path; `adjoint_over_forward` is the adjoint LJ kernel's
-time over the forward force kernel's on a 32,000-atom two-type mixture.
+time over the forward force kernel's on a 32,000-atom two-type mixture.  `extras.adress_slab`
+times the AdResS LJ / ideal-gas slab (160,000 molecules in a 200 x 40 x 40 box) through the
+single-system path, beside the CPU restatement on a bounded sample of the same run
+(`--adress-only` runs just that workload).
 """
 import argparse
 import json
@@ -162,6 +165,100 @@ def adjoint_ratio(ctx, reps=5):
     return {"atoms": int(pos.shape[0]), "forward_ms": tf / nf, "adjoint_ms": ta / na, "ratio": (ta / na) / (tf / nf)}
 
 
+ADRESS_L = (200.0, 40.0, 40.0)
+ADRESS_RHO = 0.5
+ADRESS_AT, ADRESS_HY = 30.0, 10.0  # atomistic diameter |x - 100| < 15, hybrid width 10
+ADRESS_CAP = 0.7                   # capping radius of the reference's AdResS example
+
+
+def adress_state(ctx):
+    """the AdResS slab's start: 160,000 monatomic molecules uniform-random in the periodic
+    200 x 40 x 40 box (density 0.5, seeded), relaxed 200 steps with the capped force"""
+    from mrmd_amd.ensemble import relax, replica
+    from mrmd_amd.lattice import rng
+    L = np.asarray(ADRESS_L)
+    n = int(round(ADRESS_RHO * float(np.prod(L))))
+    pos = rng(7).uniform(0.0, 1.0, (n, 3)) * L
+    rep = replica(n, list(L), 1.0, 7, adress=True, tf_every=100)
+    rep.update(atomistic_diameter=ADRESS_AT, hybrid_diameter=ADRESS_HY)
+    P, V = relax(ctx, [rep], [pos])
+    return P[0], V[0], list(L)
+
+
+def adress_slab(ctx, steps, warm=50, profile_steps=100):
+    """AdResS LJ / ideal-gas slab through the single-system path (AdResSSystem: multi-
+    resolution ghosts, molecule list, LJ_IdealGas, thermodynamic force on 1-sigma x-bins
+    updated every 100 steps over the hybrid region, Langevin T = 1, gamma = 1, dt = 0.005).
+    Timed steps run without per-kernel events; a separate pass of profile_steps steps gives
+    the kernel breakdown."""
+    from mrmd_amd.drivers import AdResSSystem
+    P, V, L = adress_state(ctx)
+    n = P.shape[0]
+    s = AdResSSystem(ctx, P, V, L, ADRESS_AT, ADRESS_HY, 1, cap=ADRESS_CAP, seed=7)
+    s.run(warm, 0.005)
+    ctx.synchronize()
+    r0 = s.rebuilds
+    t0 = time.perf_counter()
+    s.run(steps, 0.005)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    rebuilds = s.rebuilds - r0
+    names = ("lj_idealgas_tiled", "lj_idealgas", "verlet_build", "langevin_pre", "vv_post", "halo_update",
+             "halo_fold", "update_molecules", "contribute_molecule_force", "tf_sample", "tf_apply")
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    s.run(profile_steps, 0.005)
+    ctx.synchronize()
+    kernels = {k: {"launches": ctx.kernel_stats(k)[0], "total_ms": ctx.kernel_stats(k)[1]} for k in names}
+    ctx.set_profiling(False)
+    return {"atoms": n, "box": L, "steps": steps, "atom_steps_per_s": n * steps / elapsed,
+            "ms_per_step": elapsed / steps * 1e3, "rebuilds_in_timed_steps": rebuilds, "cap": ADRESS_CAP,
+            "tiling": s.vl.tiling(), "kernel_ms_per_100_steps": kernels}, (P, V, L)
+
+
+def cpu_adress_sample(P, V, L, steps, threads):
+    """the same AdResS step loop through the CPU restatement (OpenMP), started from the
+    same relaxed state: returns (atom-steps/s, seconds, rebuilds)"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    lib = O.lib()
+    lib.or_set_threads(threads)
+    rn, dt = 2.8, 0.005
+    a = O.Atoms.from_arrays(P, vel=V)
+    m = O.Molecules.for_each_atom(a)
+    box = O.make_box([0, 0, 0], L, rn)
+    slab = O.make_slab([0.5 * L[0], 0.5 * L[1], 0.5 * L[2]], ADRESS_AT, ADRESS_HY, 1)
+    g, v = O.Ghost(), O.Verlet()
+    lj = O.LJIdealGas([ADRESS_CAP], [2.5], [1.0], [1.0], 1, shifted=True)
+    tf = O.ThermodynamicForce([P.shape[0] / float(np.prod(L))], box, 1.0, [1.0], sym=True, periodic=True)
+    lo, hi = [-rn] * 3, [L[d] + rn for d in range(3)]
+    disp, rebuilds = np.inf, 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        disp += lib.or_langevin_pre(a.h, dt, 1.0, 1.0, 7, k, None)
+        if disp >= 0.15:
+            disp = 0.0
+            rebuilds += 1
+            lib.or_update_molecules(m.h, a.h, None)
+            lib.or_multires_exchange_real(O.C.byref(box), m.h, a.h)
+            lib.or_multires_ghost_xyz(g.h, m.h, a.h, O.C.byref(box))
+            lib.or_update_molecules(m.h, a.h, None)
+            v.build(m, m.nlocal + m.nghost, 0, m.nlocal, rn, 1.0, lo, hi, molecules=True)
+        else:
+            g.update(a, box)
+        lib.or_update_molecules(m.h, a.h, O.C.byref(slab))
+        lib.or_atoms_set_force(a.h, 0.0)
+        lib.or_mol_set_force(m.h, 0.0)
+        lj.run(m, v, a)
+        lib.or_contribute_molecule_force(m.h, a.h)
+        g.fold(a)
+        tf.sample(a)
+        tf.apply(a)
+        lib.or_vv_post(a.h, dt)
+    seconds = time.perf_counter() - t0
+    return P.shape[0] * steps / seconds, seconds, rebuilds
+
+
 def copy_calibration(mib):
     """a plain device-to-device copy of known size, for scaling rocprofv3's FETCH_SIZE /
     WRITE_SIZE counters to bytes"""
@@ -191,9 +288,18 @@ def main():
     ap.add_argument("--serial-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--adress-steps", type=int, default=300, help="timed steps of the AdResS slab extra")
+    ap.add_argument("--adress-cpu-steps", type=int, default=20, help="CPU sample of the AdResS slab")
+    ap.add_argument("--adress-only", action="store_true",
+                    help="only run the AdResS slab (no CPU sample) and print its JSON line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.adress_only:
+        import mrmd_amd as M
+        res, _ = adress_slab(M.Context(0), args.adress_steps)
+        print(json.dumps({"adress_slab": res}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -264,6 +370,14 @@ def main():
         extras = {"ensemble": ensemble_bench(ctx, args.ensemble_replicas, args.ensemble_steps, args.serial_steps,
                                              tune=not args.no_tune),
                   "adjoint": adjoint_ratio(ctx)}
+        ad, (P, V, L2) = adress_slab(ctx, args.adress_steps)
+        if not args.no_cpu:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            cv, ct, creb = cpu_adress_sample(P, V, L2, args.adress_cpu_steps, threads)
+            ad["cpu_baseline"] = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+                                  "sample": f"{args.adress_cpu_steps} steps from the same relaxed state, "
+                                            f"{creb} list rebuild(s) inside, {ct:.1f} s wall"}
+        extras["adress_slab"] = ad
 
     out = {
         "metric": "atom-steps/s at 4M-atom LJ",
