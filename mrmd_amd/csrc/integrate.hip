@@ -155,14 +155,22 @@ __global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
     if (threadIdx.x == 0) *out = s;
 }
 
-// per-workgroup maxima of the squared displacement, then their maximum (one workgroup)
-__global__ void k_maxPartials(const double* partials, int64_t n, double* out)
+// per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
+// eight independent loads per lane in flight)
+__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out)
 {
     __shared__ double red[BLOCK / WAVE];
-    double m = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, partials[k]);
-    m = blockMax(m, red);
-    if (threadIdx.x == 0) *out = m;
+    double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int64_t k = threadIdx.x;
+    for (; k + 7 * BLOCK < n; k += 8 * BLOCK)
+    {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], partials[k + u * BLOCK]);
+    }
+    for (; k < n; k += BLOCK) m[0] = fmax(m[0], partials[k]);
+    double r = fmax(fmax(fmax(m[0], m[1]), fmax(m[2], m[3])), fmax(fmax(m[4], m[5]), fmax(m[6], m[7])));
+    r = blockMax(r, red);
+    if (threadIdx.x == 0) *out = r;
 }
 
 // scratch slot 3: [0] maximum squared displacement of the last pre-step, [1 ..] partials
