@@ -104,6 +104,15 @@ typedef struct mrmd_slab {
     int32_t pad;
 } mrmd_slab;
 
+/* weighting_function::Spherical(center, atomisticRadius, hybridRegionDiameter, exponent)
+ * (mrmd/weighting_function/Spherical.hpp:25-99) */
+typedef struct mrmd_spherical {
+    double center[3];
+    double atomistic_radius;
+    double hybrid_diameter;
+    int64_t exponent;
+} mrmd_spherical;
+
 /* ---- field identifiers ------------------------------------------------------------------ */
 /* data::Atoms members (mrmd/data/Atoms.hpp:37-46).  Host-side layout of transfers:
  * POS/VEL/FORCE n x 3 doubles (row-major), TYPE int64, MASS/CHARGE/RELMASS doubles. */
@@ -309,6 +318,10 @@ int mrmd_vv_post(mrmd_atoms* atoms, double dt);
  * counter (call, atom index) - replaces Random_XorShift1024_Pool(1234). pred NULL = always. */
 int mrmd_langevin_pre(mrmd_atoms* atoms, double dt, double zeta, double temperature, uint64_t seed,
                       uint64_t call, const mrmd_pred1* pred, double* max_disp);
+/* BerendsenThermostat::apply (mrmd/action/BerendsenThermostat.cpp:27-50): local velocities
+ * scaled by sqrt(1 + gamma (target / current - 1)); nothing happens for current <= 0, and a
+ * target <= 0 is an error */
+int mrmd_berendsen_apply(mrmd_atoms* atoms, double current_temperature, double target_temperature, double gamma);
 /* analysis::getKineticEnergy (mrmd/analysis/KineticEnergy.hpp:26-39); synchronises */
 int mrmd_kinetic_energy(const mrmd_atoms* atoms, double* energy);
 
@@ -318,6 +331,9 @@ int mrmd_kinetic_energy(const mrmd_atoms* atoms, double* energy);
  * ========================================================================================= */
 /* Slab::operator() (Slab.hpp:161-187) on n host x values: out5[i] = lambda, modLambda, grad */
 int mrmd_slab_eval(mrmd_ctx* ctx, const mrmd_slab* slab, const double* x, int64_t n, double* out5);
+/* Spherical::operator() (Spherical.hpp:37-76) on n host points (n x 3, row-major):
+ * out4[4 i ..] = lambda, d lambda / dx, dy, dz */
+int mrmd_spherical_eval(mrmd_ctx* ctx, const mrmd_spherical* s, const double* xyz, int64_t n, double* out4);
 /* UpdateMolecules::update (mrmd/action/UpdateMolecules.hpp:24-70); slab NULL = COM only */
 int mrmd_molecules_update(mrmd_molecules* mols, const mrmd_atoms* atoms, const mrmd_slab* slab);
 /* ContributeMoleculeForceToAtoms::update (ContributeMoleculeForceToAtoms.cpp:23-48) */

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import capi
-from .capi import Box, MrmdError, Pred1, Pred2, Slab as SlabParams
+from .capi import Box, MrmdError, Pred1, Pred2, Slab as SlabParams, Spherical as SphericalParams
 
 _L = capi.lib
 
@@ -122,6 +122,35 @@ class Slab:
         out = np.zeros((x.size, 5))
         ctx.check(_L.mrmd_slab_eval(ctx.h, C.byref(self.params), _p(x), x.size, _p(out)))
         return out
+
+
+class Spherical:
+    """weighting_function::Spherical(center, atomisticRadius, hybridRegionDiameter, exponent)
+    (Spherical.hpp:25-99): lambda = cos^exponent over the hybrid shell around the centre"""
+
+    def __init__(self, center, atomistic_radius, hybrid_diameter, exponent):
+        self.params = SphericalParams()
+        for d in range(3):
+            self.params.center[d] = center[d]
+        self.params.atomistic_radius = atomistic_radius
+        self.params.hybrid_diameter = hybrid_diameter
+        self.params.exponent = int(exponent)
+
+    def __call__(self, ctx, xyz):
+        """(n, 4): lambda and its gradient at n points"""
+        x = np.ascontiguousarray(np.atleast_2d(xyz), dtype=np.float64)
+        out = np.zeros((x.shape[0], 4))
+        ctx.check(_L.mrmd_spherical_eval(ctx.h, C.byref(self.params), _p(x), x.shape[0], _p(out)))
+        return out
+
+
+class BerendsenThermostat:
+    """action::BerendsenThermostat (BerendsenThermostat.hpp:22-36)"""
+
+    @staticmethod
+    def apply(atoms, current_temperature, target_temperature, gamma):
+        atoms.ctx.check(_L.mrmd_berendsen_apply(atoms.h, float(current_temperature), float(target_temperature),
+                                                float(gamma)))
 
 
 class _Container:
@@ -573,6 +602,7 @@ class ThermodynamicForce:
         t = np.ascontiguousarray(np.atleast_1d(target_density), dtype=np.float64)
         m = np.ascontiguousarray(np.atleast_1d(modulation), dtype=np.float64)
         self.num_types = t.size
+        self.lo, self.hi = float(box.min[0]), float(box.max[0])  # the bins span x of the subdomain
         h = C.c_void_p()
         ctx.check(_L.mrmd_tf_create(ctx.h, _p(t), _p(m), t.size, C.byref(box), bin_width,
                                     int(enforce_symmetry), int(use_periodicity), C.byref(h)))
@@ -621,6 +651,12 @@ class ThermodynamicForce:
     def set_force(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64)
         self.ctx.check(_L.mrmd_tf_set_force(self.h, _p(a)))
+
+    def grid(self):
+        """bin centres, MultiHistogram::getBinPosition (MultiHistogram.hpp:68-74)"""
+        nb = self.num_bins
+        size = (self.hi - self.lo) / nb
+        return self.lo + (np.arange(nb) + 0.5) * size
 
     def mu(self):
         l, r = np.zeros(self.num_types), np.zeros(self.num_types)

@@ -42,7 +42,13 @@ class Replica(C.Structure):
                 ("tf_symmetric", C.c_int32), ("tf_periodic", C.c_int32)]
 
 
-STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab, "mrmd_replica": Replica}
+class Spherical(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("atomistic_radius", C.c_double), ("hybrid_diameter", C.c_double),
+                ("exponent", C.c_int64)]
+
+
+STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab, "mrmd_replica": Replica,
+           "mrmd_spherical": Spherical}
 
 _SCALARS = {
     "int": C.c_int, "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64,

@@ -39,6 +39,18 @@ __global__ void k_slabEval(mrmd_slab s, const double* x, int64_t n, double* out)
     out[5 * k + 4] = 0.0;
 }
 
+__global__ void k_sphericalEval(mrmd_spherical s, const double* xyz, int64_t n, double* out)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double l, gx, gy, gz;
+    sphericalEval(s, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], l, gx, gy, gz);
+    out[4 * k] = l;
+    out[4 * k + 1] = gx;
+    out[4 * k + 2] = gy;
+    out[4 * k + 3] = gz;
+}
+
 // UpdateMolecules::update (UpdateMolecules.hpp:24-70)
 __global__ void k_updateMolecules(MolView m, AtomsView a, int64_t n, mrmd_slab s, int useSlab)
 {
@@ -728,6 +740,23 @@ int mrmd_slab_eval(mrmd_ctx* c, const mrmd_slab* s, const double* x, int64_t n, 
     hipLaunchKernelGGL(k_slabEval, dim3(gridFor(n)), dim3(BLOCK), 0, c->stream, *s, dx.p, n, dout.p);
     checkLaunch();
     HIPCHECK(hipMemcpyAsync(out5, dout.p, size_t(5 * n) * 8, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    MRMD_API_END
+}
+
+int mrmd_spherical_eval(mrmd_ctx* c, const mrmd_spherical* s, const double* xyz, int64_t n, double* out4)
+{
+    MRMD_API_BEGIN(c)
+    REQUIRE(s && xyz && out4 && n >= 0, "invalid arguments");
+    REQUIRE(s->hybrid_diameter > 0.0, "the hybrid shell needs a positive width");
+    if (n == 0) return MRMD_OK;
+    DBuf<double> dx, dout;
+    dx.reserve(c, 3 * n);
+    dout.reserve(c, 4 * n);
+    HIPCHECK(hipMemcpyAsync(dx.p, xyz, size_t(3 * n) * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_sphericalEval, dim3(gridFor(n)), dim3(BLOCK), 0, c->stream, *s, dx.p, n, dout.p);
+    checkLaunch();
+    HIPCHECK(hipMemcpyAsync(out4, dout.p, size_t(4 * n) * 8, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     MRMD_API_END
 }

@@ -155,6 +155,16 @@ __global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
     if (threadIdx.x == 0) *out = s;
 }
 
+// BerendsenThermostat::apply (BerendsenThermostat.cpp:27-50)
+__global__ void k_scaleVel(AtomsView a, int64_t n, double beta)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a.vel[i] *= beta;
+    a.vel[a.cap + i] *= beta;
+    a.vel[2 * a.cap + i] *= beta;
+}
+
 // per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
 // eight independent loads per lane in flight)
 __global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out)
@@ -253,6 +263,20 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
     }
     checkLaunch();
     finishDisp(a, maxDisp);
+    MRMD_API_END
+}
+
+int mrmd_berendsen_apply(mrmd_atoms* a, double current, double target, double gamma)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a, "null atoms");
+    if (current <= 0.0) return MRMD_OK;
+    REQUIRE(target > 0.0, "the target temperature must be positive");
+    const double beta = sqrt(1.0 + gamma * (target / current - 1.0));
+    if (a->nlocal > 0)
+        hipLaunchKernelGGL(k_scaleVel, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, a->ctx->stream, a->view(), a->nlocal,
+                           beta);
+    checkLaunch();
     MRMD_API_END
 }
 

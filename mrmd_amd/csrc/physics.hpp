@@ -133,6 +133,38 @@ __host__ __device__ inline void slabEval(const mrmd_slab& s, double x, double& l
     gx = factor * dx;
 }
 
+// weighting_function::Spherical::operator() (Spherical.hpp:37-76): lambda = cos^n of the
+// radial position inside the hybrid shell, its gradient along the radius
+__host__ __device__ inline void sphericalEval(const mrmd_spherical& s, double x, double y, double z, double& lambda,
+                                              double& gx, double& gy, double& gz)
+{
+    const double dx[3] = {x - s.center[0], y - s.center[1], z - s.center[2]};
+    const double dxSqr = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    const double atSqr = s.atomistic_radius * s.atomistic_radius;
+    const double cgR = s.atomistic_radius + s.hybrid_diameter;
+    if (dxSqr < atSqr)
+    {
+        lambda = 1.0;
+        gx = gy = gz = 0.0;
+        return;
+    }
+    if (dxSqr > cgR * cgR)
+    {
+        lambda = 0.0;
+        gx = gy = gz = 0.0;
+        return;
+    }
+    const double r = sqrt(dxSqr);
+    const double arg = PI / (2.0 * s.hybrid_diameter) * (r - s.atomistic_radius);
+    const double base = cos(arg);
+    lambda = powInt(base, s.exponent);
+    const double factor =
+        -PI / (2.0 * s.hybrid_diameter) * double(s.exponent) * sin(arg) * powInt(base, s.exponent - 1) / r;
+    gx = factor * dx[0];
+    gy = factor * dx[1];
+    gz = factor * dx[2];
+}
+
 // MultiHistogram::getBin (MultiHistogram.hpp:60-66)
 __host__ __device__ inline int64_t getBin(double v, double min, double invBinSize, int64_t numBins)
 {

@@ -1,0 +1,4 @@
+// reference include path action/BerendsenThermostat.hpp of the mrmd action API, served by the gfx950 library
+#pragma once
+
+#include "mrmd_facade/core.hpp"

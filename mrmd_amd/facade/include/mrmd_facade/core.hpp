@@ -742,6 +742,44 @@ private:
 };
 }  // namespace weighting_function
 
+namespace weighting_function
+{
+/// weighting_function::Spherical (Spherical.hpp:25-99), evaluated by the device library
+class Spherical
+{
+public:
+    Spherical(const Point3D& center, const real_t atomisticRadius, const real_t hybridRegionDiameter,
+              const int exponent)
+    {
+        for (int d = 0; d < 3; ++d) p_.center[d] = center[d];
+        p_.atomistic_radius = atomisticRadius;
+        p_.hybrid_diameter = hybridRegionDiameter;
+        p_.exponent = exponent;
+    }
+    void operator()(const real_t x, const real_t y, const real_t z, real_t& lambda, real_t& gradLambdaX,
+                    real_t& gradLambdaY, real_t& gradLambdaZ) const
+    {
+        const real_t p[3] = {x, y, z};
+        real_t out[4];
+        impl::check(mrmd_spherical_eval(context(), &p_, p, 1, out), "Spherical");
+        lambda = out[0];
+        gradLambdaX = out[1];
+        gradLambdaY = out[2];
+        gradLambdaZ = out[3];
+    }
+    real_t operator()(const real_t x, const real_t y, const real_t z) const
+    {
+        real_t lambda, tmp;
+        operator()(x, y, z, lambda, tmp, tmp, tmp);
+        return lambda;
+    }
+    const mrmd_spherical& params() const { return p_; }
+
+private:
+    mrmd_spherical p_{};
+};
+}  // namespace weighting_function
+
 namespace pred
 {
 /// region of a slab as a one-position predicate
@@ -1012,6 +1050,10 @@ public:
     {
     }
     ThermodynamicForce(const ThermodynamicForce&) = delete;
+    ThermodynamicForce(ThermodynamicForce&& o) noexcept : h_(o.h_), numTypes_(o.numTypes_), min_(o.min_), max_(o.max_)
+    {
+        o.h_ = nullptr;
+    }
     ~ThermodynamicForce() { mrmd_tf_destroy(h_); }
     void sample(data::Atoms& a) { impl::check(mrmd_tf_sample(h_, a.sync()), "sample"); }
     void update(const real_t& sigma, const real_t& range) { impl::check(mrmd_tf_update(h_, sigma, range, nullptr), "update"); }
@@ -1035,6 +1077,14 @@ public:
         impl::check(mrmd_tf_apply_interpolated(h_, a.sync(), &d), "applyInterpolated_if");
     }
     data::MultiHistogram getForce() const { return get(0, "thermodynamic-force"); }
+    /// the force of one atom type over the bins (ThermodynamicForce.hpp:49-52)
+    std::vector<real_t> getForce(const idx_t& typeId) const
+    {
+        const auto f = getForce();
+        std::vector<real_t> out(static_cast<size_t>(f.numBins));
+        for (idx_t b = 0; b < f.numBins; ++b) out[size_t(b)] = f(b, typeId);
+        return out;
+    }
     data::MultiHistogram getDensityProfile() const { return get(1, "density-profile"); }
     void setForce(const data::MultiHistogram& f) const { impl::check(mrmd_tf_set_force(h_, f.data.data()), "setForce"); }
     idx_t getNumberOfDensityProfileSamples() const
@@ -1065,6 +1115,31 @@ private:
     idx_t numTypes_;
     real_t min_, max_;
 };
+}  // namespace action
+
+namespace data
+{
+/// createGrid (MultiHistogram.cpp:214-227): the bin centres
+inline std::vector<real_t> createGrid(const MultiHistogram& h)
+{
+    std::vector<real_t> g(static_cast<size_t>(h.numBins));
+    for (idx_t b = 0; b < h.numBins; ++b) g[size_t(b)] = h.getBinPosition(b);
+    return g;
+}
+}  // namespace data
+
+namespace action
+{
+/// action::BerendsenThermostat (BerendsenThermostat.hpp:22-36)
+namespace BerendsenThermostat
+{
+inline void apply(data::Atoms& atoms, const real_t& currentTemperature, const real_t& targetTemperature,
+                  const real_t& gamma)
+{
+    impl::check(mrmd_berendsen_apply(atoms.sync(), currentTemperature, targetTemperature, gamma),
+                "BerendsenThermostat::apply");
+}
+}  // namespace BerendsenThermostat
 }  // namespace action
 
 // ---- analysis / io ----------------------------------------------------------------------

@@ -1257,6 +1257,42 @@ void or_slab_eval(const or_slab* s, double x, double* out5)
     out5[4] = g[2];
 }
 
+// weighting_function/Spherical.hpp:37-76 (center, atomistic radius, hybrid shell width,
+// exponent); out4 = lambda, gradient x, y, z
+void or_spherical_eval(const double* center, double atomisticRadius, double hybridDiameter, int64_t exponent,
+                       double x, double y, double z, double* out4)
+{
+    const double dx[3] = {x - center[0], y - center[1], z - center[2]};
+    const double dxSqr = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    const double atSqr = atomisticRadius * atomisticRadius;
+    const double cgSqr = (atomisticRadius + hybridDiameter) * (atomisticRadius + hybridDiameter);
+    if (dxSqr < atSqr || dxSqr > cgSqr)
+    {
+        out4[0] = dxSqr < atSqr ? 1.0 : 0.0;
+        out4[1] = out4[2] = out4[3] = 0.0;
+        return;
+    }
+    const double r = std::sqrt(dxSqr);
+    const double arg = PI / (2.0 * hybridDiameter) * (r - atomisticRadius);
+    const double base = std::cos(arg);
+    out4[0] = powInt(base, exponent);
+    const double factor =
+        -PI / (2.0 * hybridDiameter) * double(exponent) * std::sin(arg) * powInt(base, exponent - 1) / r;
+    for (int d = 0; d < 3; ++d) out4[1 + d] = factor * dx[d];
+}
+
+// action/BerendsenThermostat.cpp:27-50; returns 0, or -1 for a non-positive target
+int or_berendsen_apply(OAtoms* at, double current, double target, double gamma)
+{
+    if (current <= 0.0) return 0;
+    if (!(target > 0.0)) return -1;
+    const double beta = std::sqrt(1.0 + gamma * (target / current - 1.0));
+#pragma omp parallel for
+    for (int64_t i = 0; i < at->nlocal; ++i)
+        for (int d = 0; d < 3; ++d) at->a[size_t(i)].vel[d] *= beta;
+    return 0;
+}
+
 // action/UpdateMolecules.hpp:24-70
 void or_update_molecules(OMolecules* m, const OAtoms* at, const or_slab* s)
 {

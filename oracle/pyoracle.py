@@ -147,6 +147,8 @@ def lib():
         "or_langevin_pre": (D, [P, D, D, D, C.c_uint64, C.c_uint64, P]),
         "or_kinetic_energy": (D, [P]),
         "or_slab_eval": (None, [P, D, P]),
+        "or_spherical_eval": (None, [P, D, D, I64, D, D, D, P]),
+        "or_berendsen_apply": (I32, [P, D, D, D]),
         "or_update_molecules": (None, [P, P, P]),
         "or_contribute_molecule_force": (None, [P, P]),
         "or_multires_exchange_real": (None, [P, P, P]),

@@ -31,6 +31,7 @@ inline bool floatEq(double a, double b)
         }                                                                \
     } while (0)
 #define EXPECT_FALSE(c) EXPECT_TRUE(!(c))
+#define EXPECT_LE(a, b) EXPECT_TRUE((a) <= (b))
 #define EXPECT_EQ(a, b)                                                                           \
     do                                                                                            \
     {                                                                                             \

@@ -2,9 +2,11 @@
 // VelocityVerlet.test.cpp, VelocityVerletLangevinThermostat.test.cpp, LJ_IdealGas.test.cpp,
 // ThermodynamicForce.test.cpp, Slab.test.cpp, CheckRegion.test.cpp, UpdateMolecules.test.cpp,
 // ContributeMoleculeForceToAtoms.test.cpp, GhostExchange.test.cpp,
-// MultiResPeriodicGhostExchange.test.cpp.
+// MultiResPeriodicGhostExchange.test.cpp, io/ThermoForce.test.cpp, Spherical.test.cpp,
+// BerendsenThermostat.test.cpp.
 #include <cstdio>
 
+#include "action/BerendsenThermostat.hpp"
 #include "action/ContributeMoleculeForceToAtoms.hpp"
 #include "action/LJ_IdealGas.hpp"
 #include "action/ThermodynamicForce.hpp"
@@ -15,8 +17,12 @@
 #include "communication/GhostLayer.hpp"
 #include "communication/MultiResGhostLayer.hpp"
 #include "data/MoleculesFromAtoms.hpp"
+#include "io/DumpThermoForce.hpp"
+#include "io/RestoreThermoForce.hpp"
 #include "weighting_function/CheckRegion.hpp"
 #include "weighting_function/Slab.hpp"
+#include "weighting_function/Spherical.hpp"
+#include "analysis/KineticEnergy.hpp"
 
 using namespace mrmd;
 
@@ -133,6 +139,92 @@ static void thermodynamicForce()
     EXPECT_EQ(tf.getNumberOfDensityProfileSamples(), 0);
 }
 
+// io/ThermoForce.test.cpp:25-120: forces (bin + 1)(type + 1) on 100 bins of the x extent of
+// subdomain (1, 2, 3)-(4, 6, 8), dumped and restored; single type and five types
+static action::ThermodynamicForce createThermoForce(idx_t numBins, idx_t numForces, const data::Subdomain& subdomain)
+{
+    const real_t binWidth = (subdomain.maxCorner[0] - subdomain.minCorner[0]) / real_c(numBins);
+    action::ThermodynamicForce tf(std::vector<real_t>(size_t(numForces), 1_r), subdomain, binWidth,
+                                  std::vector<real_t>(size_t(numForces), 1_r));
+    auto f = tf.getForce();
+    for (idx_t b = 0; b < numBins; ++b)
+        for (idx_t t = 0; t < numForces; ++t) f(b, t) = 1_r * (real_c(b) + 1) * (real_c(t) + 1);
+    tf.setForce(f);
+    return tf;
+}
+
+static void thermoForceDumpRestore()
+{
+    const data::Subdomain subdomain({1_r, 2_r, 3_r}, {4_r, 6_r, 8_r}, 0.5_r);
+    {
+        auto tf1 = createThermoForce(100, 1, subdomain);
+        io::dumpThermoForce("dummySingleForce.txt", tf1, 0);
+        auto tf2 = io::restoreThermoForce("dummySingleForce.txt", subdomain);
+        const auto g1 = data::createGrid(tf1.getForce()), g2 = data::createGrid(tf2.getForce());
+        const auto f1 = tf1.getForce(0), f2 = tf2.getForce(0);
+        EXPECT_EQ(idx_c(g2.size()), 100);
+        for (size_t b = 0; b < 100; ++b)
+        {
+            EXPECT_FLOAT_EQ(g1[b], g2[b]);
+            EXPECT_FLOAT_EQ(f1[b], f2[b]);
+        }
+    }
+    {
+        auto tf1 = createThermoForce(100, 5, subdomain);
+        io::dumpThermoForce("dummyMultipleForces.txt", tf1);
+        const std::vector<real_t> ones(5, 1_r);
+        auto tf2 = io::restoreThermoForce("dummyMultipleForces.txt", subdomain, ones, ones);
+        const auto h1 = tf1.getForce(), h2 = tf2.getForce();
+        const auto g1 = data::createGrid(h1), g2 = data::createGrid(h2);
+        for (idx_t b = 0; b < 100; ++b)
+        {
+            EXPECT_FLOAT_EQ(g1[size_t(b)], g2[size_t(b)]);
+            for (idx_t t = 0; t < 5; ++t) EXPECT_FLOAT_EQ(h1(b, t), h2(b, t));
+        }
+    }
+}
+
+// weighting_function/Spherical.test.cpp:22-85
+static void spherical()
+{
+    const Point3D center = {2_r, 3_r, 4_r};
+    const auto weight = weighting_function::Spherical(center, 2_r, 2_r, 2);
+    Point3D pos = center;
+    auto w = weight(pos[0], pos[1], pos[2]);
+    for (auto i = 0; i < 60; ++i)
+    {
+        const auto old = w;
+        w = weight(pos[0], pos[1], pos[2]);
+        EXPECT_LE(w, old);
+        for (int d = 0; d < 3; ++d) pos[d] += 0.1_r;
+    }
+    EXPECT_FLOAT_EQ(weight(2.1_r, 3.1_r, 4.1_r), 1_r);
+    EXPECT_FLOAT_EQ(weight(6.1_r, 7.1_r, 8.1_r), 0_r);
+    const auto eps = 1e-10_r;
+    pos = center;
+    for (auto i = 0; i < 60; ++i)
+    {
+        real_t l0, gx0, gy0, gz0, l1, gx1, gy1, gz1;
+        weight(pos[0], 0_r, 0_r, l0, gx0, gy0, gz0);
+        weight(pos[0] + eps, 0_r, 0_r, l1, gx1, gy1, gz1);
+        EXPECT_FLOAT_EQ((l1 - l0) / eps, 0.5_r * (gx0 + gx1));
+        pos[0] += 0.1_r;
+    }
+}
+
+// action/BerendsenThermostat.test.cpp:24-45 on the single-atom fixture
+static void berendsen()
+{
+    for (const real_t gamma : {0_r, 1_r})
+    {
+        auto atoms = singleAtom();
+        auto T = analysis::getMeanKineticEnergy(atoms) * (2_r / 3_r);
+        action::BerendsenThermostat::apply(atoms, T, 3.8_r, gamma);
+        T = analysis::getMeanKineticEnergy(atoms) * (2_r / 3_r);
+        EXPECT_FLOAT_EQ(T, gamma == 0_r ? 41.5_r : 3.8_r);
+    }
+}
+
 static void slabAndRegions()
 {
     auto weight = weighting_function::Slab({2_r, 3_r, 4_r}, 2_r, 2_r, 2);
@@ -238,6 +330,9 @@ int main(int argc, char** argv)
     ljIdealGas(0.5_r, 0.5_r);
     ljIdealGas(0_r, 0_r);
     thermodynamicForce();
+    thermoForceDumpRestore();
+    spherical();
+    berendsen();
     slabAndRegions();
     moleculesUpdateAndContribute();
     ghostExchange();

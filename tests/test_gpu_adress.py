@@ -246,3 +246,38 @@ def test_thermodynamic_force_reference_cases(M, ctx):
     x = u.download(M.POS)[:, 0]
     inside = (x > 0.5) & (x < 4.5)
     np.testing.assert_allclose(u.download(M.FORCE)[:, 0], np.where(inside, x - 0.5, 0.0), atol=1e-12)
+
+
+def test_thermo_force_dump_restore(M, ctx, tmp_path):
+    """io/ThermoForce.test.cpp:25-120 through the Python mirror: forces (bin + 1)(type + 1)
+    on 100 bins of subdomain (1, 2, 3)-(4, 6, 8), one type and five types"""
+    from mrmd_amd.io import dump_thermo_force, restore_thermo_force
+    box = M.make_box([1, 2, 3], [4, 6, 8], 0.5)
+    for nt in (1, 5):
+        tf1 = M.ThermodynamicForce(ctx, [1.0] * nt, box, 3.0 / 100, [1.0] * nt)
+        assert tf1.num_bins == 100
+        f = np.outer(np.arange(1, 101), np.arange(1, nt + 1)).astype(float)
+        tf1.set_force(f)
+        path = str(tmp_path / f"force{nt}.txt")
+        if nt == 1:
+            dump_thermo_force(path, tf1, 0)
+        else:
+            dump_thermo_force(path, tf1)
+        tf2 = restore_thermo_force(ctx, path, box, [1.0] * nt, [1.0] * nt)
+        np.testing.assert_allclose(tf2.grid(), tf1.grid(), rtol=1e-6)
+        np.testing.assert_array_equal(tf2.get_force(), f)
+
+
+def test_spherical_matches_oracle(M, ctx):
+    """weighting_function::Spherical (Spherical.hpp:37-76) on the device against the oracle
+    on points spread over the atomistic core, the hybrid shell and beyond (exponents 1-3)"""
+    g = np.random.default_rng(9)
+    c = np.array([2.0, 3.0, 4.0])
+    pts = c + g.normal(size=(3000, 3)) * 2.5
+    for n in (1, 2, 3):
+        out = M.Spherical(c, 2.0, 2.0, n)(ctx, pts)
+        ref = np.zeros_like(out)
+        for k, p in enumerate(pts):
+            O.lib().or_spherical_eval(O.ptr(c), 2.0, 2.0, n, p[0], p[1], p[2], O.ptr(ref[k]))
+        np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-15)
+        assert (out[:, 0] == 1.0).any() and (out[:, 0] == 0.0).any() and ((out[:, 0] > 0) & (out[:, 0] < 1)).any()

@@ -287,3 +287,26 @@ def test_invalid_arguments_fail_loudly(M, ctx):
     bad = M.pred1(kind=99)
     with pytest.raises(M.MrmdError):
         M.VelocityVerletLangevinThermostat(1.0, 1.0).pre_force_integrate(a, 0.1, pred=bad)
+
+
+def test_berendsen_matches_oracle(M, ctx):
+    """BerendsenThermostat::apply (BerendsenThermostat.cpp:27-50): bit-exact velocity scaling
+    against the oracle; the reference's single-atom cases (T = 41.5 -> 41.5 at gamma 0,
+    -> 3.8 at gamma 1); T <= 0 leaves velocities alone; a target <= 0 is an error"""
+    g = np.random.default_rng(8)
+    pos, vel = g.uniform(0, 10, (5000, 3)), g.normal(size=(5000, 3))
+    ga, ca = _both(M, ctx, pos, vel=vel)
+    M.BerendsenThermostat.apply(ga, 1.3, 0.9, 0.2)
+    assert O.lib().or_berendsen_apply(ca.h, 1.3, 0.9, 0.2) == 0
+    np.testing.assert_array_equal(ga.download(M.VEL), ca.get(O.VEL))
+    M.BerendsenThermostat.apply(ga, 0.0, 0.9, 0.2)
+    np.testing.assert_array_equal(ga.download(M.VEL), ca.get(O.VEL))
+    with pytest.raises(M.MrmdError):
+        M.BerendsenThermostat.apply(ga, 1.0, 0.0, 0.2)
+    for gamma, expected in ((0.0, 41.5), (1.0, 3.8)):
+        a = M.Atoms.from_arrays(ctx, np.array([[2.0, 3.0, 4.0]]), vel=np.array([[7.0, 5.0, 3.0]]),
+                                mass=np.array([1.5]))
+        t = a.kinetic_energy() * (2.0 / 3.0)
+        assert t == pytest.approx(41.5, rel=1e-15)
+        M.BerendsenThermostat.apply(a, t, 3.8, gamma)
+        assert a.kinetic_energy() * (2.0 / 3.0) == pytest.approx(expected, rel=1e-12)

@@ -14,10 +14,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "mrmd_amd", "facade", "bin")
 
 
-def _run(name, *args):
+def _run(name, *args, cwd=None):
     exe = os.path.join(BIN, name)
     assert os.path.exists(exe), f"{exe} missing: run __graft_entry__.build()"
-    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120, cwd=cwd)
     print(r.stdout[-4000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     return r.stdout
@@ -31,5 +31,5 @@ def test_espp_comparison_dropin(tmp_path):
     assert "ghosts 22104 pairs 1310403" in out
 
 
-def test_unit_actions_dropin():
-    _run("unit_actions")
+def test_unit_actions_dropin(tmp_path):
+    _run("unit_actions", cwd=tmp_path)  # the thermodynamic-force files go to tmp_path

@@ -594,3 +594,76 @@ def test_multires_real_atoms_exchange(apm):
     assert np.all(mp >= 1) and np.all(mp < 2)
     ap = atoms.get(O.POS)[: 27 * apm]
     assert np.all(ap >= 1) and np.all(ap < 2)
+
+
+# ---------------------------------------------------------------- spherical weighting
+# mrmd/weighting_function/Spherical.test.cpp:22-85 (centre (2, 3, 4), atomistic radius 2,
+# hybrid shell 2, exponent 2)
+def _sph(x, y, z, center=(2.0, 3.0, 4.0), rat=2.0, hy=2.0, n=2):
+    out = np.zeros(4)
+    c = np.asarray(center, dtype=np.float64)
+    O.lib().or_spherical_eval(O.ptr(c), rat, hy, n, x, y, z, O.ptr(out))
+    return out
+
+
+def test_spherical_reference_cases():
+    # monotonous along the diagonal from the centre
+    w = [_sph(2.0 + 0.1 * i, 3.0 + 0.1 * i, 4.0 + 0.1 * i)[0] for i in range(60)]
+    assert all(w[i + 1] <= w[i] for i in range(59))
+    # boundary values
+    assert _sph(2.1, 3.1, 4.1)[0] == 1.0
+    assert _sph(6.1, 7.1, 8.1)[0] == 0.0
+    # derivative as the reference test evaluates it (x varied at y = z = 0)
+    eps = 1e-10
+    for i in range(60):
+        x = 2.0 + 0.1 * i
+        l0, l1 = _sph(x, 0.0, 0.0), _sph(x + eps, 0.0, 0.0)
+        assert (l1[0] - l0[0]) / eps == pytest.approx(0.5 * (l0[1] + l1[1]), abs=1e-6)
+
+
+def test_spherical_closed_form_and_gradient():
+    """inside the shell lambda = cos^n(pi/(2 d_HY) (r - r_AT)); the gradient is the radial
+    derivative times the unit vector (central differences along every axis)"""
+    g = np.random.default_rng(3)
+    c = np.array([2.0, 3.0, 4.0])
+    for _ in range(200):
+        u = g.normal(size=3)
+        u /= np.linalg.norm(u)
+        r = g.uniform(2.0, 4.0)
+        p = c + r * u
+        out = _sph(*p, n=3)
+        assert out[0] == pytest.approx(math.cos(math.pi / 4.0 * (r - 2.0)) ** 3, rel=1e-12, abs=1e-15)
+        h = 1e-6
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            fd = (_sph(*(p + e), n=3)[0] - _sph(*(p - e), n=3)[0]) / (2 * h)
+            assert out[1 + d] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    # a different exponent changes lambda as cos^n
+    p = c + np.array([3.0, 0.0, 0.0])
+    assert _sph(*p, n=1)[0] ** 2 == pytest.approx(_sph(*p, n=2)[0], rel=1e-14)
+
+
+# ---------------------------------------------------------------- Berendsen thermostat
+# mrmd/action/BerendsenThermostat.test.cpp:24-45 on the single-atom fixture (T = 41.5)
+@pytest.mark.parametrize("gamma,expected", [(0.0, 41.5), (1.0, 3.8)])
+def test_berendsen_reference_cases(gamma, expected):
+    a = _single_atom()
+    t = O.lib().or_kinetic_energy(a.h) * (2.0 / 3.0)
+    assert t == pytest.approx(41.5, rel=1e-15)
+    assert O.lib().or_berendsen_apply(a.h, t, 3.8, gamma) == 0
+    assert O.lib().or_kinetic_energy(a.h) * (2.0 / 3.0) == pytest.approx(expected, rel=1e-12)
+
+
+def test_berendsen_scaling_law():
+    """v scales by sqrt(1 + gamma (T0/T - 1)), so T moves a fraction gamma towards T0; no
+    change for T <= 0; a non-positive target is rejected"""
+    g = np.random.default_rng(4)
+    v = g.normal(size=(1000, 3))
+    a = O.Atoms.from_arrays(g.uniform(0, 10, (1000, 3)), vel=v)
+    O.lib().or_berendsen_apply(a.h, 2.0, 1.0, 0.25)
+    np.testing.assert_allclose(a.get(O.VEL), v * math.sqrt(1.0 + 0.25 * (1.0 / 2.0 - 1.0)), rtol=1e-15)
+    before = a.get(O.VEL)
+    assert O.lib().or_berendsen_apply(a.h, 0.0, 1.0, 0.5) == 0
+    np.testing.assert_array_equal(a.get(O.VEL), before)
+    assert O.lib().or_berendsen_apply(a.h, 1.0, 0.0, 0.5) == -1
