@@ -221,42 +221,13 @@ def cpu_adress_sample(P, V, L, steps, threads):
     same relaxed state: returns (atom-steps/s, seconds, rebuilds)"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-    lib = O.lib()
-    lib.or_set_threads(threads)
-    rn, dt = 2.8, 0.005
-    a = O.Atoms.from_arrays(P, vel=V)
-    m = O.Molecules.for_each_atom(a)
-    box = O.make_box([0, 0, 0], L, rn)
-    slab = O.make_slab([0.5 * L[0], 0.5 * L[1], 0.5 * L[2]], ADRESS_AT, ADRESS_HY, 1)
-    g, v = O.Ghost(), O.Verlet()
-    lj = O.LJIdealGas([ADRESS_CAP], [2.5], [1.0], [1.0], 1, shifted=True)
-    tf = O.ThermodynamicForce([P.shape[0] / float(np.prod(L))], box, 1.0, [1.0], sym=True, periodic=True)
-    lo, hi = [-rn] * 3, [L[d] + rn for d in range(3)]
-    disp, rebuilds = np.inf, 0
+    O.lib().or_set_threads(threads)
+    run = O.AdResSRun(P, V, L, ADRESS_AT, ADRESS_HY, 1, cap=ADRESS_CAP, seed=7)
     t0 = time.perf_counter()
-    for k in range(steps):
-        disp += lib.or_langevin_pre(a.h, dt, 1.0, 1.0, 7, k, None)
-        if disp >= 0.15:
-            disp = 0.0
-            rebuilds += 1
-            lib.or_update_molecules(m.h, a.h, None)
-            lib.or_multires_exchange_real(O.C.byref(box), m.h, a.h)
-            lib.or_multires_ghost_xyz(g.h, m.h, a.h, O.C.byref(box))
-            lib.or_update_molecules(m.h, a.h, None)
-            v.build(m, m.nlocal + m.nghost, 0, m.nlocal, rn, 1.0, lo, hi, molecules=True)
-        else:
-            g.update(a, box)
-        lib.or_update_molecules(m.h, a.h, O.C.byref(slab))
-        lib.or_atoms_set_force(a.h, 0.0)
-        lib.or_mol_set_force(m.h, 0.0)
-        lj.run(m, v, a)
-        lib.or_contribute_molecule_force(m.h, a.h)
-        g.fold(a)
-        tf.sample(a)
-        tf.apply(a)
-        lib.or_vv_post(a.h, dt)
+    for _ in range(steps):
+        run.step(0.005)
     seconds = time.perf_counter() - t0
-    return P.shape[0] * steps / seconds, seconds, rebuilds
+    return P.shape[0] * steps / seconds, seconds, run.rebuilds
 
 
 def copy_calibration(mib):

@@ -571,3 +571,66 @@ def md_params(**kw):
     for k, v in defaults.items():
         setattr(p, k, v)
     return p
+
+
+PRED_SLAB_HY = 5
+
+
+class AdResSRun:
+    """the AdResS step of a monatomic LJ / ideal-gas slab through the oracle, in the order of
+    the single-system driver (Langevin pre-step with the skin criterion, multi-resolution
+    exchange + ghosts + molecule list on rebuild steps, ghost update otherwise, molecule
+    update with the slab, LJ_IdealGas, molecule forces to atoms, ghost fold, density sample,
+    thermodynamic-force update every tf_every steps over the hybrid region, its application,
+    post-step) -- the loop of examples/04_LennardJones_IdealGas_LocalCap"""
+
+    def __init__(self, pos, vel, L, atomistic_diameter, hybrid_diameter, nu=1, rc=2.5, skin=0.3, cap=0.0,
+                 gamma=1.0, temperature=1.0, seed=1234, tf_every=100, tf_sigma=2.0, tf_range=3.0, shifted=True):
+        self.L = [float(x) for x in L]
+        self.rn, self.skin = rc + skin, skin
+        self.gamma, self.temperature, self.seed = gamma, temperature, seed
+        self.tf_every, self.tf_sigma, self.tf_range = tf_every, tf_sigma, tf_range
+        self.atoms = Atoms.from_arrays(np.asarray(pos, dtype=np.float64), vel=np.asarray(vel, dtype=np.float64))
+        self.mols = Molecules.for_each_atom(self.atoms)
+        self.box = make_box([0, 0, 0], self.L, self.rn)
+        c = [0.5 * x for x in self.L]
+        self.slab = make_slab(c, atomistic_diameter, hybrid_diameter, nu)
+        self.ghost, self.verlet = Ghost(), Verlet()
+        self.lj = LJIdealGas([cap], [rc], [1.0], [1.0], 1, shifted=shifted)
+        rho = pos.shape[0] / float(np.prod(self.L))
+        self.tf = ThermodynamicForce([rho], self.box, 1.0, [1.0], sym=True, periodic=True)
+        self.hy = pred1(kind=PRED_SLAB_HY, axis=0, a=c[0], b=0.5 * atomistic_diameter, c=hybrid_diameter)
+        self.lo, self.hi = [-self.rn] * 3, [x + self.rn for x in self.L]
+        self.disp, self.rebuilds, self.calls, self.steps = np.inf, 0, 0, 0
+
+    def _rebuild(self):
+        a, m = self.atoms, self.mols
+        lib().or_update_molecules(m.h, a.h, None)
+        lib().or_multires_exchange_real(C.byref(self.box), m.h, a.h)
+        lib().or_multires_ghost_xyz(self.ghost.h, m.h, a.h, C.byref(self.box))
+        lib().or_update_molecules(m.h, a.h, None)
+        self.verlet.build(m, m.nlocal + m.nghost, 0, m.nlocal, self.rn, 1.0, self.lo, self.hi, molecules=True)
+        self.rebuilds += 1
+
+    def step(self, dt):
+        a, m = self.atoms, self.mols
+        self.disp += lib().or_langevin_pre(a.h, dt, self.gamma, self.temperature, self.seed, self.calls, None)
+        self.calls += 1
+        if self.disp >= 0.5 * self.skin:
+            self.disp = 0.0
+            self._rebuild()
+        else:
+            self.ghost.update(a, self.box)
+        lib().or_update_molecules(m.h, a.h, C.byref(self.slab))
+        lib().or_atoms_set_force(a.h, 0.0)
+        lib().or_mol_set_force(m.h, 0.0)
+        energy = self.lj.run(m, self.verlet, a)
+        lib().or_contribute_molecule_force(m.h, a.h)
+        self.ghost.fold(a)
+        self.tf.sample(a)
+        self.steps += 1
+        if self.tf_every and self.steps % self.tf_every == 0:
+            self.tf.update(self.tf_sigma, self.tf_range, self.hy)
+        self.tf.apply(a)
+        lib().or_vv_post(a.h, dt)
+        return energy

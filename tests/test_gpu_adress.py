@@ -281,3 +281,35 @@ def test_spherical_matches_oracle(M, ctx):
             O.lib().or_spherical_eval(O.ptr(c), 2.0, 2.0, n, p[0], p[1], p[2], O.ptr(ref[k]))
         np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-15)
         assert (out[:, 0] == 1.0).any() and (out[:, 0] == 0.0).any() and ((out[:, 0] > 0) & (out[:, 0] < 1)).any()
+
+
+def test_adress_trajectory_matches_oracle(M, ctx):
+    """40 steps of the single-system AdResS driver (drivers.AdResSSystem) against the oracle's
+    step loop (pyoracle.AdResSRun) from the same start: Langevin noise (the same Philox
+    stream), rebuild steps, multi-resolution exchange and ghosts, the tiled LJ_IdealGas,
+    molecule forces, ghost fold, thermodynamic force sampled every step and updated every 10
+    steps over the hybrid region.  Positions agree to 1e-9 after 40 steps (the pair sums
+    differ in order, so trajectories separate slowly from the ~1e-15 level)."""
+    from mrmd_amd.drivers import AdResSSystem
+    from mrmd_amd.lattice import maxwell_boltzmann
+    L = [40.0, 12.0, 12.0]
+    g = np.random.default_rng(21)
+    grid = np.stack(np.meshgrid(np.arange(32) * 1.25, np.arange(10) * 1.2, np.arange(10) * 1.2, indexing="ij"), -1)
+    pos = np.mod(grid.reshape(-1, 3) + 0.3 + g.uniform(-0.05, 0.05, (3200, 3)), L)
+    vel = maxwell_boltzmann(pos.shape[0], 1.0, 5)
+    kw = dict(cap=0.7, seed=11, tf_every=10)
+    gsys = AdResSSystem(ctx, pos, vel, L, 8.0, 5.0, 1, **kw)
+    osys = O.AdResSRun(pos, vel, L, 8.0, 5.0, 1, **kw)
+    for _ in range(40):
+        gsys.step(0.005)
+        osys.step(0.005)
+    assert gsys.rebuilds == osys.rebuilds and gsys.rebuilds >= 2
+    n = gsys.atoms.nlocal
+    assert n == osys.atoms.nlocal
+    d = gsys.local(M.POS) - osys.atoms.get(O.POS, n)
+    d -= np.asarray(L) * np.round(d / np.asarray(L))
+    assert np.abs(d).max() < 1e-9, np.abs(d).max()
+    dv = gsys.local(M.VEL) - osys.atoms.get(O.VEL, n)
+    assert np.abs(dv).max() < 1e-7, np.abs(dv).max()
+    np.testing.assert_allclose(gsys.tf.get_force(), osys.tf.force(), rtol=1e-9, atol=1e-10)
+    assert np.abs(osys.tf.force()).max() > 0.0
