@@ -233,6 +233,7 @@ struct IGTiled
     const double4* igPos;  ///< sorted slot -> {atom x, y, z, lambda_mod}, sign bit set on non-rows
     int64_t rowsEnd;       ///< molecules [0, rowsEnd) are rows
     int queueCap;          ///< queued non-row partner pairs per workgroup (<= IG_QUEUE)
+    int32_t maxStaged;     ///< the sentinel record sits at sentinelSlot(maxStaged)
 };
 
 // sorted-slot record of each molecule's single atom and its modulated lambda (in [0, 1];
@@ -448,6 +449,7 @@ __global__ __launch_bounds__(BLOCK) void k_igTiled(IGArgs g, IGTiled tl, Pre p, 
         cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
     }
     stageTile<false>(ti, tl.igPos, st);
+    putSentinel<false>(st, tl.maxStaged, 0.0);
     __syncthreads();
     // a neighbourhood of coarse-grained molecules only (lambda_mod = 0 everywhere) has no
     // pair work (LJ_IdealGas.cpp:95), no drift and no compensation term
@@ -874,9 +876,9 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
         hipLaunchKernelGGL(k_igGather, dim3(gridFor(vl->nrows)), dim3(BLOCK), 0, ctx->stream, m->view(), a->pos.p,
                            vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
         const IGTiled tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p, s->igPos.p, rows,
-                         s->queueCap};
+                         s->queueCap, vl->maxStaged};
         const Pre p = s->table[0];
-        const size_t lds = size_t(std::max(vl->maxStaged, 1)) * sizeof(double4);
+        const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) * sizeof(double4);
         if (p.capDistSqr > 0.0)
             hipLaunchKernelGGL(k_igTiled<true>, dim3(grid), dim3(vl->block), lds, ctx->stream, g, tl, p, part);
         else

@@ -86,12 +86,13 @@ struct TiledList
     const TileInfo* tiles;
     const int32_t* sorted;
     const double4* sortedPos;
+    int32_t maxStaged;  ///< the sentinel record sits at sentinelSlot(maxStaged)
 };
 
 // Branch-free pair terms of one chunk of eight entries, written stage by stage over the
 // eight pairs so that the eight dependency chains (LDS read, distance, reciprocal, force)
-// are independent and interleave.  Padding entries (the row itself, d^2 = 0) and pairs
-// beyond rc select zero.  CAP: the table has a capping distance, so the capped branch
+// are independent and interleave.  Padding entries (the sentinel record) and pairs beyond
+// rc get a zero factor.  CAP: the table has a capping distance, so the capped branch
 // (LennardJones.hpp:68-75) is evaluated too (single-type tables: only for the chunks where
 // a lane of the wave needs it).  Products and sums may contract to FMAs here
 // (the neighbour decision of the list is unaffected: it is made, unfused, at build time);
@@ -121,10 +122,13 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
     auto accumulate = [&](int u, double ff, double en) {
-        bool in = d2[u] <= a.rcSqr && d2[u] > 0.0;
+        // padding entries point at the sentinel record beyond every cutoff, so the cutoff test
+        // alone decides; the terms are finite for every entry and a 0 / 1 factor keeps them
+        bool in = d2[u] <= a.rcSqr;
         if (PRED) in = in && ty[u] >= 0.0;
-        ff = in ? ff : 0.0;
-        en = in ? en : 0.0;
+        const double m = in ? 1.0 : 0.0;
+        ff *= m;
+        en *= m;
         fx += dx[u] * ff;
         fy += dy[u] * ff;
         fz += dz[u] * ff;
@@ -213,6 +217,7 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
         cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
     }
     stageTile<ONE>(ti, tl.sortedPos, st);
+    putSentinel<ONE>(st, tl.maxStaged, 0.0);
     __syncthreads();
     double e = 0.0, w = 0.0;
     for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
@@ -623,11 +628,11 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
             vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
-                               vl->bin.sortedPos.p};
+                               vl->bin.sortedPos.p, vl->maxStaged};
             const Pre p1 = lj->table[0];
             const bool one = lj->table.size() == 1;
-            const size_t lds =
-                size_t(std::max(vl->maxStaged, 1)) * (one ? Staged<true>::bytes : Staged<false>::bytes);
+            const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) *
+                               (one ? Staged<true>::bytes : Staged<false>::bytes);
             const dim3 g(grid), b(vl->block);
             bool cap = false;
             for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;

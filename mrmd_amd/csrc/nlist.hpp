@@ -142,6 +142,18 @@ struct Staged<false>
     __device__ static double4 get(const void* st, int q) { return static_cast<const double4*>(st)[q]; }
 };
 
+// Rows are padded to whole chunks with a sentinel entry: the LDS slot just past the largest
+// staged neighbourhood of the list, which every tiled force kernel fills with a record far
+// from the box, so a padding pair lies beyond any cutoff and needs no test of its own.
+constexpr double SENTINEL_COORD = 1.0e6;
+__host__ __device__ inline int sentinelSlot(int32_t maxStaged) { return maxStaged > 1 ? maxStaged : 1; }
+template <bool XYZ>
+__device__ inline void putSentinel(void* st, int32_t maxStaged, double w)
+{
+    if (threadIdx.x == 0)
+        Staged<XYZ>::put(st, sentinelSlot(maxStaged), make_double4(SENTINEL_COORD, SENTINEL_COORD, SENTINEL_COORD, w));
+}
+
 // copy the tile's neighbourhood (contiguous runs of sorted slots) into LDS; the copy is
 // unrolled so that four independent loads per lane are in flight
 template <bool XYZ>

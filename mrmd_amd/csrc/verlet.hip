@@ -227,10 +227,10 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     }
                 }
             }
-        // pad the last chunk with the row's own slot: d^2 = 0 entries that the force
-        // kernels mask, so their chunk loop needs no bounds test
+        // pad the last chunk with the sentinel slot (a record beyond every cutoff), so the
+        // force kernels' chunk loop needs no bounds test
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
-            nbr[entryIndex(k, s, strideS)] = uint16_t(qSelf);
+            nbr[entryIndex(k, s, strideS)] = uint16_t(sentinelSlot(stagedCap));
         cnt[s] = min(c, cap);  // the true width goes to maxCnt; reads stay inside the list
         localMax = max(localMax, c);
     }
