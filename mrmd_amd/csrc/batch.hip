@@ -499,7 +499,7 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u) d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
 #pragma unroll
-    for (int u = 0; u < CHUNK; ++u) frac2[u] = 1.0 / d2[u];
+    for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
     const Pre& p = b.lj;
     auto accumulate = [&](int u, double ff, double en) {
         // both partners coarse-grained: no interaction (LJ_IdealGas.cpp:95)
@@ -530,7 +530,7 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
         const double frac6 = frac2[u] * frac2[u] * frac2[u];
         ffv[u] = frac6 * (p.ff1 * frac6 - p.ff2) * frac2[u];
         env[u] = frac6 * (p.ef1 * frac6 - p.ef2) - p.shift;
-        capped = capped || d2[u] < p.capDistSqr;
+        capped = capped || (d2[u] < p.capDistSqr && d2[u] > 0.0);  // padding (the row itself) is not a pair
     }
     // the capped form (LennardJones.hpp:68-75) only for chunks where some lane of the wave
     // needs it
