@@ -355,20 +355,28 @@ __global__ void k_bTiles(BS b)
 __global__ void k_bGather(BS b)
 {
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= b.natoms) return;
-    const int32_t i = b.sorted[s];
-    const double4 p = b.pos[i];
-    const RepD& R = b.reps[b.atomRep[i]];
-    double lam = 1.0, mod = 1.0, g = 0.0;
-    if (R.adress)
+    const bool valid = s < b.natoms;
+    int64_t key = -1;  // density bin of the atom's replica, -1 for none
+    if (valid)
     {
-        slabEval(R.slab, p.x, lam, mod, g);
-        const int64_t bin = getBin(p.x, 0.0, R.invBinWidth, R.nbins);
-        if (bin != -1) atomicAddD(&b.tfDensity[R.h0 + bin], 1.0);
+        const int32_t i = b.sorted[s];
+        const double4 p = b.pos[i];
+        const RepD& R = b.reps[b.atomRep[i]];
+        double lam = 1.0, mod = 1.0, g = 0.0;
+        if (R.adress)
+        {
+            slabEval(R.slab, p.x, lam, mod, g);
+            const int64_t bin = getBin(p.x, 0.0, R.invBinWidth, R.nbins);
+            if (bin != -1) key = R.h0 + bin;
+        }
+        b.sortedPos[s] = make_double4(p.x, p.y, p.z, mod);
+        b.slotLam[s] = lam;
+        b.slotGrad[s] = g;
     }
-    b.sortedPos[s] = make_double4(p.x, p.y, p.z, mod);
-    b.slotLam[s] = lam;
-    b.slotGrad[s] = g;
+    // atoms in cell order share their x-bin over long runs: one atomic per run of a wave
+    // (counts are integers, so the sums are exact in any order)
+    const Run r = laneRun(int(key), valid);
+    if (valid && key >= 0 && r.head == int(threadIdx.x) % WAVE) atomicAddD(&b.tfDensity[key], double(r.len));
 }
 
 // copy a tile's neighbourhood into LDS, shifting each piece by its periodic image

@@ -415,6 +415,32 @@ __device__ inline void atomicMaxNonNeg(double* addr, double v)
 
 __device__ inline void atomicAddD(double* addr, double v) { unsafeAtomicAdd(addr, v); }
 
+// Consecutive lanes of a wave with the same key (a cell, a histogram bin) form a run
+// (records in cell order give a few runs per wave); the run's first lane does the run's
+// atomic, so one atomic per run replaces one per record.  The valid lanes must be a prefix
+// of the wave.
+struct Run
+{
+    int head;  ///< lane of the run's first record
+    int len;   ///< records in the run (valid on the head lane)
+};
+
+__device__ inline Run laneRun(int c, bool valid)
+{
+    const int lane = int(threadIdx.x) % WAVE;
+    const int prev = __shfl_up(c, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long valids = __ballot(valid);
+    const unsigned long long upTo = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);
+    Run r;
+    r.head = heads & upTo ? 63 - __clzll(heads & upTo) : lane;
+    const unsigned long long above = heads & ~upTo;
+    const int end = above ? __builtin_ctzll(above) : __popcll(valids);
+    r.len = end - lane;
+    return r;
+}
+
 // XCD-aware block remap: gfx950 dispatches consecutive workgroups round-robin over 8 XCDs,
 // each with its own L2.  Remapping gives every XCD a contiguous range of workgroups, so
 // spatially neighbouring atoms (cell-sorted) share one L2.

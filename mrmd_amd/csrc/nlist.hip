@@ -31,31 +31,6 @@ GridD makeGrid(const double* gmin, const double* gmax, const double* delta)
 
 namespace
 {
-// Consecutive records of a wave that fall into the same cell form a run (after a cell
-// permutation nearly every wave is a few runs); the run's first lane does the run's
-// atomic, so one atomic per run replaces one per record.
-struct Run
-{
-    int head;  ///< lane of the run's first record
-    int len;   ///< records in the run (valid on the head lane)
-};
-
-__device__ inline Run laneRun(int c, bool valid)
-{
-    const int lane = int(threadIdx.x) % WAVE;
-    const int prev = __shfl_up(c, 1, WAVE);
-    const bool head = valid && (lane == 0 || prev != c);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long valids = __ballot(valid);
-    const unsigned long long upTo = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1);
-    Run r;
-    r.head = heads & upTo ? 63 - __clzll(heads & upTo) : lane;
-    const unsigned long long above = heads & ~upTo;
-    const int end = above ? __builtin_ctzll(above) : __popcll(valids);
-    r.len = end - lane;
-    return r;
-}
-
 __global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g, int32_t* cellOf, int32_t* counts)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
