@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--adress-cpu-steps", type=int, default=20, help="CPU sample of the AdResS slab")
     ap.add_argument("--adress-only", action="store_true",
                     help="only run the AdResS slab (no CPU sample) and print its JSON line")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the ensemble comparison and print its JSON line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -271,6 +273,12 @@ def main():
         import mrmd_amd as M
         res, _ = adress_slab(M.Context(0), args.adress_steps)
         print(json.dumps({"adress_slab": res}))
+        return
+    if args.ensemble_only:
+        import mrmd_amd as M
+        res = ensemble_bench(M.Context(0), args.ensemble_replicas, args.ensemble_steps, args.serial_steps,
+                             tune=not args.no_tune)
+        print(json.dumps({"ensemble": res}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",

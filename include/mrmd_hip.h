@@ -432,6 +432,11 @@ int mrmd_batch_set_lj(mrmd_batch* b, double cap, double sigma, double epsilon, i
  * height), workgroup size (64, 128, 192 or 256); forces a list rebuild at the next step */
 int mrmd_batch_set_tiling(mrmd_batch* b, int target_staged, int block);
 int mrmd_batch_get_tiling(const mrmd_batch* b, int* target_staged, int* block, int32_t* tile_cells);
+/* upper bound on a workgroup's LDS stage area in records (0 = sized from the expected
+ * neighbourhood, the default); tiles whose neighbourhood is larger read their records from
+ * global memory with identical arithmetic, so results do not depend on it (a tuning and
+ * testing knob, no reference counterpart); forces a list rebuild at the next step */
+int mrmd_batch_set_stage_cap(mrmd_batch* b, int limit);
 /* all replicas' atoms concatenated in replica order, n_total x 3 row-major (POS, VEL; FORCE
  * for download) */
 int mrmd_batch_upload(mrmd_batch* b, int field, const double* host);

@@ -346,7 +346,6 @@ __global__ void k_bTiles(BS b)
         }
     ti.nr = nr;
     ti.staged = staged;
-    if (staged > b.stageCap) atomicOr(b.err, 2);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -394,6 +393,32 @@ __device__ inline void stageShifted(const TileB& ti, const double4* __restrict__
     }
 }
 
+// A tile's records by staged slot: from LDS when its neighbourhood fits the stage area, else
+// straight from the slot-ordered records with the same periodic shift (the same arithmetic,
+// so the same bits) -- a slower path that keeps tiles of crowded cells (lattice starts,
+// density fluctuations) exact instead of dropping them.
+struct LdsFetch
+{
+    const double4* st;
+    __device__ double4 operator()(int q) const { return st[q]; }
+};
+struct GlobalFetch
+{
+    const TileB* ti;
+    const double4* sp;
+    double L0, L1, L2;
+    __device__ double4 operator()(int q) const
+    {
+        int r = 0;
+        while (r + 1 < ti->nr && ti->base[r + 1] <= q) ++r;
+        double4 v = sp[ti->begin[r] + (q - ti->base[r])];
+        v.x += double(ti->sx[r]) * L0;
+        v.y += double(ti->sy[r]) * L1;
+        v.z += double(ti->sz[r]) * L2;
+        return v;
+    }
+};
+
 // squared distance from p to the (unwrapped) cell (ix, iy, iz) of the replica grid
 __device__ inline double cellDistSqr(const RepD& R, double px, double py, double pz, int ix, int iy, int iz)
 {
@@ -410,44 +435,17 @@ __device__ inline double cellDistSqr(const RepD& R, double px, double py, double
     return r2;
 }
 
-// full-row list of every tile of a flagged replica (entries: LDS slots, rows padded to
-// whole chunks with the row's own slot)
-__global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
+// rows of one tile of a flagged replica (entries: staged slots, rows padded to whole chunks
+// with the row's own slot)
+template <class Fetch>
+__device__ void bBuildRows(const BS& b, const TileB& ti, const TileG& G, const RepD& R, const int* cstart,
+                           const Fetch& fetch)
 {
-    extern __shared__ double4 st[];
-    __shared__ TileB ti;
-    __shared__ int cstart[9 * (B_SPAN + 1)];
-    const unsigned t = xcdBlock();
-    const TileG G = b.tg[t];
-    if (!b.flag[G.rep]) return;
-    if (threadIdx.x == 0) ti = b.tiles[t];
-    __syncthreads();
-    if (ti.staged > b.stageCap || ti.rowEnd <= ti.rowBegin) return;
-    const RepD R = b.reps[G.rep];
-    stageShifted(ti, b.sortedPos, st, R);
-    const int nzr = G.k1 - G.k0 + 2;
-    if (threadIdx.x < 9)
-    {
-        const int c = threadIdx.x, di = c / 3 - 1, dj = c % 3 - 1;
-        int ii = G.i + di, jj = G.j + dj;
-        ii = ii < 0 ? ii + R.nc[0] : (ii >= R.nc[0] ? ii - R.nc[0] : ii);
-        jj = jj < 0 ? jj + R.nc[1] : (jj >= R.nc[1] ? jj - R.nc[1] : jj);
-        int run = ti.colBase[c];
-        for (int z = 0; z < nzr; ++z)
-        {
-            int kz = G.k0 - 1 + z;
-            kz = kz < 0 ? kz + R.nc[2] : (kz >= R.nc[2] ? kz - R.nc[2] : kz);
-            cstart[c * (B_SPAN + 1) + z] = run;
-            run += b.counts[cellIndex(R, ii, jj, kz)];
-        }
-        cstart[c * (B_SPAN + 1) + nzr] = run;
-    }
-    __syncthreads();
     int localMax = 0;
     for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
     {
         const int qSelf = ti.base[ti.selfPiece] + (s - ti.begin[ti.selfPiece]);
-        const double4 xp = st[qSelf];
+        const double4 xp = fetch(qSelf);
         const int kz = (b.cellOf[b.sorted[s]] - R.c0) % R.nc[2];
         int c = 0;
         for (int di = -1; di <= 1; ++di)
@@ -464,7 +462,7 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
                         for (int u = 0; u < 4; ++u)
                         {
                             const bool valid = q + u < q1 && q + u != qSelf;
-                            const double4 xn = st[valid ? q + u : qSelf];
+                            const double4 xn = fetch(valid ? q + u : qSelf);
                             const double dx = xp.x - xn.x, dy = xp.y - xn.y, dz = xp.z - xn.z;
                             hit[u] = valid && dx * dx + dy * dy + dz * dz <= b.rnSqr;
                         }
@@ -485,12 +483,51 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
     if (localMax > b.capList) atomicOr(b.err, 1);
 }
 
+// full-row list of every tile of a flagged replica
+__global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileB ti;
+    __shared__ int cstart[9 * (B_SPAN + 1)];
+    const unsigned t = xcdBlock();
+    const TileG G = b.tg[t];
+    if (!b.flag[G.rep]) return;
+    if (threadIdx.x == 0) ti = b.tiles[t];
+    __syncthreads();
+    if (ti.rowEnd <= ti.rowBegin) return;
+    const RepD R = b.reps[G.rep];
+    const bool fits = ti.staged <= b.stageCap;
+    if (fits) stageShifted(ti, b.sortedPos, st, R);
+    const int nzr = G.k1 - G.k0 + 2;
+    if (threadIdx.x < 9)
+    {
+        const int c = threadIdx.x, di = c / 3 - 1, dj = c % 3 - 1;
+        int ii = G.i + di, jj = G.j + dj;
+        ii = ii < 0 ? ii + R.nc[0] : (ii >= R.nc[0] ? ii - R.nc[0] : ii);
+        jj = jj < 0 ? jj + R.nc[1] : (jj >= R.nc[1] ? jj - R.nc[1] : jj);
+        int run = ti.colBase[c];
+        for (int z = 0; z < nzr; ++z)
+        {
+            int kz = G.k0 - 1 + z;
+            kz = kz < 0 ? kz + R.nc[2] : (kz >= R.nc[2] ? kz - R.nc[2] : kz);
+            cstart[c * (B_SPAN + 1) + z] = run;
+            run += b.counts[cellIndex(R, ii, jj, kz)];
+        }
+        cstart[c * (B_SPAN + 1) + nzr] = run;
+    }
+    __syncthreads();
+    if (fits)
+        bBuildRows(b, ti, G, R, cstart, LdsFetch{st});
+    else
+        bBuildRows(b, ti, G, R, cstart, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
+}
+
 // ---------------------------------------------------------------------------------------
 // forces: weighted LJ (LJ_IdealGas.cpp:80-200 with one atom per molecule; weight 1 for
 // plain replicas, which reduces to LennardJones.hpp:168-197), drift and compensation terms
 
-template <bool CAP>
-__device__ inline void chunkW(const BS& b, const double4& pi, const double4* st, const uint4 cur, double& fx,
+template <bool CAP, class Fetch>
+__device__ inline void chunkW(const BS& b, const double4& pi, const Fetch& fetch, const uint4 cur, double& fx,
                               double& fy, double& fz, double& ew, double& e0)
 {
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -498,7 +535,7 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
 #pragma unroll
     for (int u = 0; u < CHUNK; ++u)
     {
-        const double4 pj = st[(wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu];
+        const double4 pj = fetch(int((wd[u >> 1] >> ((u & 1) * 16)) & 0x7fffu));
         dx[u] = pi.x - pj.x;
         dy[u] = pi.y - pj.y;
         dz[u] = pi.z - pj.z;
@@ -559,24 +596,15 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const double4* st,
     for (int u = 0; u < CHUNK; ++u) accumulate(u, ffv[u], env[u]);
 }
 
-template <bool CAP>
-__global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
+// the rows of one tile: weighted pair forces, drift and compensation terms; returns the
+// lane's energy sum
+template <bool CAP, class Fetch>
+__device__ double bForceRows(const BS& b, const TileB& ti, const RepD& R, const Fetch& fetch)
 {
-    extern __shared__ double4 st[];
-    __shared__ TileB ti;
-    __shared__ double red[BLOCK / WAVE];
-    const unsigned t = xcdBlock();
-    const TileG G = b.tg[t];
-    if (threadIdx.x == 0) ti = b.tiles[t];
-    __syncthreads();
-    const bool ok = ti.staged <= b.stageCap;
-    const RepD R = b.reps[G.rep];
-    if (ok) stageShifted(ti, b.sortedPos, st, R);
-    __syncthreads();
     const int64_t step = *b.step;
     const bool sampling = R.adress && step % SAMPLING_INTERVAL == 0;
     double eTile = 0.0;
-    for (int s = ti.rowBegin + threadIdx.x; ok && s < ti.rowEnd; s += blockDim.x)
+    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
     {
         const int c = b.cnt[s];
         const int32_t i = b.sorted[s];
@@ -589,7 +617,7 @@ __global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
         {
             uint4 nxt = cur;
             if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * b.strideS];
-            chunkW<CAP>(b, pi, st, cur, fx, fy, fz, ew, e0);
+            chunkW<CAP>(b, pi, fetch, cur, fx, fy, fz, ew, e0);
             cur = nxt;
         }
         if (R.adress && pi.w > 0.0 && pi.w < 1.0)
@@ -614,8 +642,27 @@ __global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
         b.force[2 * b.cap + i] = fz;
         eTile += ew;
     }
-    eTile = blockSum(0.5 * eTile, red);
-    if (threadIdx.x == 0) b.ePart[t] = eTile;
+    return eTile;
+}
+
+template <bool CAP>
+__global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
+{
+    extern __shared__ double4 st[];
+    __shared__ TileB ti;
+    __shared__ double red[BLOCK / WAVE];
+    const unsigned t = xcdBlock();
+    const TileG G = b.tg[t];
+    if (threadIdx.x == 0) ti = b.tiles[t];
+    __syncthreads();
+    const bool fits = ti.staged <= b.stageCap;
+    const RepD R = b.reps[G.rep];
+    if (fits) stageShifted(ti, b.sortedPos, st, R);
+    __syncthreads();
+    const double eTile = fits ? bForceRows<CAP>(b, ti, R, LdsFetch{st})
+                              : bForceRows<CAP>(b, ti, R, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
+    const double e = blockSum(0.5 * eTile, red);
+    if (threadIdx.x == 0) b.ePart[t] = e;
 }
 
 // LJ_IdealGas::updateMeanCompensationEnergy (LJ_IdealGas.cpp:21-50) at the reference's
@@ -761,6 +808,7 @@ struct mrmd_batch
     Pre lj{};
     bool cap_ = false;
     int targetStaged = 1000, block = 256, stageCap = 1600;
+    int stageCapLimit = 0;  ///< 0: the stage area follows the expected neighbourhood size
     int32_t capList = 0;
     int64_t steps = 0;
 
@@ -866,6 +914,7 @@ void mrmd_batch::layoutTiles()
     ntiles = t0;
     // LDS capacity: the expected largest neighbourhood with room for density fluctuations
     stageCap = int(std::min(double(STAGE_MAX), std::ceil(1.6 * maxStage + 64.0)));
+    if (stageCapLimit > 0) stageCap = std::min(stageCap, stageCapLimit);
     mrmd_ctx* c = ctx;
     dreps.reserve(c, int64_t(reps.size()));
     HIPCHECK(hipMemcpyAsync(dreps.p, reps.data(), reps.size() * sizeof(RepD), hipMemcpyHostToDevice, c->stream));
@@ -1096,6 +1145,15 @@ int mrmd_batch_set_tiling(mrmd_batch* b, int target, int block)
     MRMD_API_END
 }
 
+int mrmd_batch_set_stage_cap(mrmd_batch* b, int limit)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && limit >= 0 && limit <= STAGE_MAX, "the stage limit must lie in [0, %d]", STAGE_MAX);
+    b->stageCapLimit = limit;
+    b->layoutTiles();
+    MRMD_API_END
+}
+
 int mrmd_batch_get_tiling(const mrmd_batch* b, int* target, int* block, int32_t* tileCells)
 {
     if (!b) return MRMD_ERR_ARG;
@@ -1192,9 +1250,6 @@ int mrmd_batch_run(mrmd_batch* b, int64_t nsteps, int graphSteps)
     sync(c);
     b->steps += nsteps;
     if (e & 1) fail(MRMD_ERR_STATE, "a neighbour row exceeded the list width %d (density too high)", b->capList);
-    if (e & 2)
-        fail(MRMD_ERR_STATE, "a tile neighbourhood exceeded %d staged records: lower the target with "
-             "mrmd_batch_set_tiling", b->stageCap);
     MRMD_API_END
 }
 

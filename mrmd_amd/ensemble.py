@@ -48,6 +48,10 @@ class Ensemble:
     def set_tiling(self, target_staged, block):
         self.ctx.check(_L.mrmd_batch_set_tiling(self.h, int(target_staged), int(block)))
 
+    def set_stage_cap(self, limit):
+        """cap on the LDS stage area in records (0 = automatic); larger tiles read global memory"""
+        self.ctx.check(_L.mrmd_batch_set_stage_cap(self.h, int(limit)))
+
     def tiling(self):
         t, b = C.c_int(), C.c_int()
         k = np.zeros(self.n_rep, np.int32)
