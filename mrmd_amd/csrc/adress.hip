@@ -287,7 +287,8 @@ __device__ inline uint32_t igChunk(const IGArgs& g, const Pre& p, const double4&
     auto accumulate = [&](int u, double ff, double en) {
         const bool nonRow = signbit(lb[u]);
         const double lam = fabs(lb[u]);
-        // beyond rc, padding (d^2 = 0), both CG (LJ_IdealGas.cpp:95), unflagged non-row
+        // beyond rc (padding entries point at the sentinel record), both CG
+        // (LJ_IdealGas.cpp:95), unflagged non-row
         const bool in = d2[u] <= g.rcSqr && d2[u] > 0.0 && !(isCG(pi.w) && isCG(lam)) &&
                         (!nonRow || ((flag >> u) & 1u));
         const double w = 0.5 * (pi.w + lam);
