@@ -358,6 +358,10 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* mols, mrmd_verlet_list* v
 /* which: 0 compensation energy, 1 counter, 2 mean (getMeanCompensationEnergy), 200 x numTypes */
 int mrmd_ljig_get_histogram(const mrmd_lj_idealgas* s, int which, double* host);
 int mrmd_ljig_set_histogram(mrmd_lj_idealgas* s, int which, const double* host);
+/* number of run() calls so far, which decides the sampling and update steps of the
+ * compensation energy (runCounter_, LJ_IdealGas.cpp:241-257): get (if not NULL) reads it
+ * before set (if not NULL) replaces it; for checkpoints */
+int mrmd_ljig_run_counter(mrmd_lj_idealgas* s, int64_t* get, const int64_t* set);
 
 /* ThermodynamicForce(targetDensity[], subdomain, binWidth, modulation[], enforceSymmetry,
  * usePeriodicity) (ThermodynamicForce.cpp:25-57) */
@@ -381,6 +385,9 @@ int mrmd_tf_get(const mrmd_thermo_force* tf, int which, double* host);
 int mrmd_tf_set_force(mrmd_thermo_force* tf, const double* host);
 /* getNumberOfDensityProfileSamples (ThermodynamicForce.hpp:63) */
 int mrmd_tf_samples(const mrmd_thermo_force* tf, int64_t* samples);
+/* restore the accumulated density profile (nbins x numTypes) and its sample count, the
+ * state sample() builds between updates (ThermodynamicForce.cpp:74-86); for checkpoints */
+int mrmd_tf_set_density(mrmd_thermo_force* tf, const double* host, int64_t samples);
 /* getMuLeft / getMuRight (ThermodynamicForce.cpp:98-130) */
 int mrmd_tf_mu(const mrmd_thermo_force* tf, double* left, double* right);
 
@@ -450,6 +457,17 @@ int mrmd_batch_energies(mrmd_batch* b, double* potential, double* kinetic);
 int mrmd_batch_stats(mrmd_batch* b, int64_t* rebuilds, int64_t* steps);
 /* thermodynamic force (which 0) or density profile (which 1) of one replica's x-bins */
 int mrmd_batch_tf_get(mrmd_batch* b, int64_t replica, int which, double* host, int64_t n);
+/* Checkpoint of the ensemble's dynamic state (no counterpart in the reference, whose
+ * checkpoints are H5MD files of one system, io/DumpH5MD.hpp:30-75, io/RestoreH5MD.hpp:36-39):
+ * positions, velocities, forces, the step counter that keys the Langevin noise, rebuild
+ * and density-sample counts, thermodynamic-force and density bins, compensation-energy
+ * sums, counts and means.  state_bytes gives the blob's size; save writes it, load reads a
+ * blob of the same replicas into an ensemble created with the same replica descriptions.
+ * Both end the current list epoch (every replica rebuilds at its next step), so the run
+ * that continues after a save and the run restored from it stay bit-identical. */
+int mrmd_batch_state_bytes(const mrmd_batch* b, int64_t* bytes);
+int mrmd_batch_state_save(mrmd_batch* b, void* host, int64_t bytes);
+int mrmd_batch_state_load(mrmd_batch* b, const void* host, int64_t bytes);
 
 #ifdef __cplusplus
 }

@@ -592,6 +592,18 @@ class LJ_IdealGas:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self.ctx.check(_L.mrmd_ljig_set_histogram(self.h, which, _p(a)))
 
+    @property
+    def run_counter(self):
+        """run() calls so far (they decide the compensation sampling and update steps)"""
+        n = C.c_int64()
+        self.ctx.check(_L.mrmd_ljig_run_counter(self.h, C.byref(n), None))
+        return n.value
+
+    @run_counter.setter
+    def run_counter(self, value):
+        n = C.c_int64(int(value))
+        self.ctx.check(_L.mrmd_ljig_run_counter(self.h, None, C.byref(n)))
+
 
 class ThermodynamicForce:
     """action::ThermodynamicForce (ThermodynamicForce.hpp:32-96)"""
@@ -651,6 +663,11 @@ class ThermodynamicForce:
     def set_force(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64)
         self.ctx.check(_L.mrmd_tf_set_force(self.h, _p(a)))
+
+    def set_density_profile(self, density, samples):
+        """restore the accumulated density profile and its sample count (checkpoints)"""
+        a = np.ascontiguousarray(density, dtype=np.float64).reshape(self.num_bins, self.num_types)
+        self.ctx.check(_L.mrmd_tf_set_density(self.h, _p(a), int(samples)))
 
     def grid(self):
         """bin centres, MultiHistogram::getBinPosition (MultiHistogram.hpp:68-74)"""

@@ -919,6 +919,14 @@ int mrmd_ljig_get_histogram(const mrmd_lj_idealgas* s, int which, double* host)
     MRMD_API_END
 }
 
+int mrmd_ljig_run_counter(mrmd_lj_idealgas* s, int64_t* get, const int64_t* set)
+{
+    if (!s || (!get && !set) || (set && *set < 0)) return MRMD_ERR_ARG;
+    if (get) *get = s->runCounter;
+    if (set) s->runCounter = *set;
+    return MRMD_OK;
+}
+
 int mrmd_ljig_set_histogram(mrmd_lj_idealgas* s, int which, const double* host)
 {
     MRMD_API_BEGIN(s ? s->ctx : nullptr)
@@ -989,6 +997,17 @@ int mrmd_tf_samples(const mrmd_thermo_force* t, int64_t* s)
     if (!t || !s) return MRMD_ERR_ARG;
     *s = t->samples;
     return MRMD_OK;
+}
+
+int mrmd_tf_set_density(mrmd_thermo_force* t, const double* host, int64_t samples)
+{
+    MRMD_API_BEGIN(t ? t->ctx : nullptr)
+    REQUIRE(t && host && samples >= 0, "invalid arguments");
+    const int64_t n = t->hd.numBins * t->hd.numHist;
+    HIPCHECK(hipMemcpyAsync(t->density.p, host, size_t(n) * 8, hipMemcpyHostToDevice, t->ctx->stream));
+    sync(t->ctx);
+    t->samples = samples;
+    MRMD_API_END
 }
 
 int mrmd_tf_sample(mrmd_thermo_force* t, const mrmd_atoms* a)

@@ -959,6 +959,56 @@ void mrmd_batch::enqueueStep()
     hipLaunchKernelGGL(k_bStepEnd, dim3(1), dim3(64), 0, c->stream, b);
 }
 
+// Checkpoint blob of the ensemble's dynamic state: a header of eight int64 (magic, version,
+// replicas, atoms, thermodynamic-force bins, compensation bins per replica, 0, 0), then the
+// parts of stateParts in their order.  The Verlet lists are not part of it: a save and a
+// load both end the current list epoch (every replica rebuilds at the next step), so a run
+// that goes on after a save and a run restored from it take bit-identical steps.
+namespace
+{
+constexpr int64_t STATE_MAGIC = 0x3154415453444d4dLL;  // bytes "MMDSTAT1"
+constexpr int64_t STATE_VERSION = 1;
+constexpr int STATE_HEADER = 8;
+
+struct StatePart
+{
+    void* dev;
+    size_t bytes;
+};
+
+std::vector<StatePart> stateParts(mrmd_batch* b)
+{
+    const size_t na = size_t(b->natoms), nr = b->reps.size(), cb = nr * COMP_BINS;
+    std::vector<StatePart> p;
+    p.push_back({b->pos.p, na * sizeof(double4)});
+    for (int d = 0; d < 3; ++d) p.push_back({b->vel.p + d * b->cap, na * 8});
+    for (int d = 0; d < 3; ++d) p.push_back({b->force.p + d * b->cap, na * 8});
+    p.push_back({b->step.p, 8});
+    p.push_back({b->rebuilds.p, nr * 8});
+    p.push_back({b->tfSamples.p, nr * 8});
+    p.push_back({b->tfDensity.p, size_t(b->nbins) * 8});
+    p.push_back({b->tfForce.p, size_t(b->nbins) * 8});
+    p.push_back({b->comp.p, cb * 8});
+    p.push_back({b->compCount.p, cb * 8});
+    p.push_back({b->meanComp.p, cb * 8});
+    return p;
+}
+
+int64_t stateBytes(mrmd_batch* b)
+{
+    int64_t n = STATE_HEADER * 8;
+    for (const StatePart& p : stateParts(b)) n += int64_t(p.bytes);
+    return n;
+}
+
+void endListEpoch(mrmd_batch* b)
+{
+    hipLaunchKernelGGL(k_bFillD, dim3(gridFor(int64_t(b->reps.size()))), dim3(BLOCK), 0, b->ctx->stream,
+                       b->dispAcc.p, int64_t(b->reps.size()), DBL_MAX);
+    checkLaunch();
+}
+}  // namespace
+
 extern "C" {
 
 int mrmd_batch_create(mrmd_ctx* c, const mrmd_replica* reps, int64_t nrep, double rc, double skin, mrmd_batch** out)
@@ -1280,6 +1330,64 @@ int mrmd_batch_stats(mrmd_batch* b, int64_t* rebuilds, int64_t* steps)
     if (rebuilds)
         HIPCHECK(hipMemcpyAsync(rebuilds, b->rebuilds.p, b->reps.size() * 8, hipMemcpyDeviceToHost, c->stream));
     if (steps) HIPCHECK(hipMemcpyAsync(steps, b->step.p, 8, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    MRMD_API_END
+}
+
+int mrmd_batch_state_bytes(const mrmd_batch* b, int64_t* bytes)
+{
+    if (!b || !bytes) return MRMD_ERR_ARG;
+    *bytes = stateBytes(const_cast<mrmd_batch*>(b));
+    return MRMD_OK;
+}
+
+int mrmd_batch_state_save(mrmd_batch* b, void* host, int64_t bytes)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && host, "null argument");
+    const int64_t need = stateBytes(b);
+    REQUIRE(bytes >= need, "the buffer holds %lld of %lld bytes", (long long)bytes, (long long)need);
+    mrmd_ctx* c = b->ctx;
+    const int64_t hdr[STATE_HEADER] = {STATE_MAGIC, STATE_VERSION, int64_t(b->reps.size()), b->natoms, b->nbins,
+                                       COMP_BINS, 0, 0};
+    char* out = static_cast<char*>(host);
+    std::memcpy(out, hdr, sizeof hdr);
+    size_t off = sizeof hdr;
+    for (const StatePart& p : stateParts(b))
+    {
+        HIPCHECK(hipMemcpyAsync(out + off, p.dev, p.bytes, hipMemcpyDeviceToHost, c->stream));
+        off += p.bytes;
+    }
+    sync(c);
+    endListEpoch(b);
+    sync(c);
+    MRMD_API_END
+}
+
+int mrmd_batch_state_load(mrmd_batch* b, const void* host, int64_t bytes)
+{
+    MRMD_API_BEGIN(b ? b->ctx : nullptr)
+    REQUIRE(b && host, "null argument");
+    const int64_t need = stateBytes(b);
+    REQUIRE(bytes == need, "the state holds %lld bytes, this ensemble's %lld", (long long)bytes, (long long)need);
+    int64_t hdr[STATE_HEADER];
+    std::memcpy(hdr, host, sizeof hdr);
+    REQUIRE(hdr[0] == STATE_MAGIC && hdr[1] == STATE_VERSION, "not an ensemble state (version %d)",
+            int(STATE_VERSION));
+    REQUIRE(hdr[2] == int64_t(b->reps.size()) && hdr[3] == b->natoms && hdr[4] == b->nbins && hdr[5] == COMP_BINS,
+            "the state is of %lld replicas / %lld atoms / %lld bins, this ensemble %lld / %lld / %lld",
+            (long long)hdr[2], (long long)hdr[3], (long long)hdr[4], (long long)b->reps.size(),
+            (long long)b->natoms, (long long)b->nbins);
+    mrmd_ctx* c = b->ctx;
+    const char* in = static_cast<const char*>(host);
+    size_t off = sizeof hdr;
+    for (const StatePart& p : stateParts(b))
+    {
+        HIPCHECK(hipMemcpyAsync(p.dev, in + off, p.bytes, hipMemcpyHostToDevice, c->stream));
+        off += p.bytes;
+    }
+    sync(c);
+    endListEpoch(b);
     sync(c);
     MRMD_API_END
 }

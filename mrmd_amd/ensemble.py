@@ -14,6 +14,20 @@ from .capi import Replica
 from .lattice import maxwell_boltzmann, rng
 
 
+_INT_FIELDS = (C.c_int32, C.c_int64, C.c_uint64)
+
+
+def _replica_column(name, ctype, replicas):
+    """one mrmd_replica field over all replicas, in a dtype that holds it exactly"""
+    if name == "box":
+        return np.array([[float(x) for x in r.get("box", (0.0, 0.0, 0.0))] for r in replicas])
+    if ctype is C.c_uint64:
+        return np.array([int(r.get(name, 0)) for r in replicas], np.uint64)
+    if ctype in _INT_FIELDS:
+        return np.array([int(r.get(name, 0)) for r in replicas], np.int64)
+    return np.array([float(r.get(name, 0.0)) for r in replicas])
+
+
 class Ensemble:
     def __init__(self, ctx, replicas, rc=2.5, skin=0.3):
         """replicas: sequence of dicts with the fields of mrmd_replica"""
@@ -30,6 +44,10 @@ class Ensemble:
                     else:
                         setattr(arr[k], name, v)
         self.replicas = [dict(r) for r in replicas]
+        self.rc, self.skin = float(rc), float(skin)
+        self.lj = dict(cap=0.0, sigma=1.0, epsilon=1.0, shifted=True)
+        self.tiling_set = None
+        self.stage_cap = 0
         self.counts = np.array([int(r["num_atoms"]) for r in replicas], dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)])
         self.n_total = int(self.offsets[-1])
@@ -44,13 +62,68 @@ class Ensemble:
 
     def set_lj(self, cap=0.0, sigma=1.0, epsilon=1.0, shifted=True):
         self.ctx.check(_L.mrmd_batch_set_lj(self.h, float(cap), float(sigma), float(epsilon), 1 if shifted else 0))
+        self.lj = dict(cap=float(cap), sigma=float(sigma), epsilon=float(epsilon), shifted=bool(shifted))
 
     def set_tiling(self, target_staged, block):
         self.ctx.check(_L.mrmd_batch_set_tiling(self.h, int(target_staged), int(block)))
+        self.tiling_set = (int(target_staged), int(block))
 
     def set_stage_cap(self, limit):
         """cap on the LDS stage area in records (0 = automatic); larger tiles read global memory"""
         self.ctx.check(_L.mrmd_batch_set_stage_cap(self.h, int(limit)))
+        self.stage_cap = int(limit)
+
+    # ---- checkpoints ------------------------------------------------------------------
+    def state(self):
+        """the dynamic state as bytes (mrmd_batch_state_save); ends the current list epoch"""
+        n = C.c_int64()
+        self.ctx.check(_L.mrmd_batch_state_bytes(self.h, C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        self.ctx.check(_L.mrmd_batch_state_save(self.h, _p(buf), n.value))
+        return buf
+
+    def load_state(self, buf):
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        self.ctx.check(_L.mrmd_batch_state_load(self.h, _p(b), b.size))
+
+    def save(self, path):
+        """checkpoint file (.npz): replica descriptions, cut-offs, LJ table and launch shape,
+        and the dynamic state; Ensemble.restore(ctx, path) resumes from it bit-identically
+        to this ensemble going on from here"""
+        cols = {f"replica/{name}": _replica_column(name, t, self.replicas) for name, t in Replica._fields_}
+        tiling = self.tiling_set if self.tiling_set is not None else (-1, -1)
+        np.savez(path, state=self.state(), cutoffs=np.array([self.rc, self.skin]),
+                 lj=np.array([self.lj["cap"], self.lj["sigma"], self.lj["epsilon"], float(self.lj["shifted"])]),
+                 launch=np.array([tiling[0], tiling[1], self.stage_cap], np.int64), **cols)
+
+    @classmethod
+    def restore(cls, ctx, path):
+        with np.load(path, allow_pickle=False) as f:
+            cols = {name: f[f"replica/{name}"] for name, _ in Replica._fields_}
+            reps = []
+            for i in range(cols["num_atoms"].shape[0]):
+                r = {}
+                for name, t in Replica._fields_:
+                    v = cols[name][i]
+                    if name == "box":
+                        r[name] = [float(x) for x in v]
+                    elif t in _INT_FIELDS:
+                        r[name] = int(v)
+                    else:
+                        r[name] = float(v)
+                reps.append(r)
+            rc, skin = (float(x) for x in f["cutoffs"])
+            lj = f["lj"]
+            launch = f["launch"]
+            state = f["state"]
+            e = cls(ctx, reps, rc=rc, skin=skin)
+            e.set_lj(cap=float(lj[0]), sigma=float(lj[1]), epsilon=float(lj[2]), shifted=bool(lj[3]))
+            if launch[0] >= 0:
+                e.set_tiling(int(launch[0]), int(launch[1]))
+            if launch[2]:
+                e.set_stage_cap(int(launch[2]))
+            e.load_state(state)
+        return e
 
     def tiling(self):
         t, b = C.c_int(), C.c_int()

@@ -222,3 +222,32 @@ def test_lattice_start_runs(ctx):
     F = e2.forces()[0]
     assert np.abs(F - f2).max() <= 1e-12 * np.abs(f2).max()
     assert np.isfinite(f).all()
+
+
+def test_checkpoint_resume_bit_identical(ctx, relaxed, tmp_path):
+    """Ensemble.save / Ensemble.restore: the restored ensemble and the one that goes on
+    after the save take bit-identical steps (graph replay, list rebuilds, the
+    thermodynamic-force update of step 49 after the resume point)"""
+    from mrmd_amd.ensemble import Ensemble
+    reps, pos, vel = relaxed
+    a = _ensemble(ctx, reps, pos, vel)
+    a.run(45, 10)
+    path = str(tmp_path / "ensemble.npz")
+    a.save(path)
+    a.run(35, 10)
+    b = Ensemble.restore(ctx, path)
+    assert b.stats()[1] == 45
+    b.run(35, 10)
+    ra, sa = a.stats()
+    rb, sb = b.stats()
+    assert sa == sb == 80 and np.array_equal(ra, rb)
+    for x, y in zip(a.positions(), b.positions()):
+        assert np.array_equal(x, y)
+    for x, y in zip(a.velocities(), b.velocities()):
+        assert np.array_equal(x, y)
+    for k in (1, 3, 4):
+        assert np.array_equal(a.tf(k, 0), b.tf(k, 0)) and np.array_equal(a.tf(k, 1), b.tf(k, 1)), k
+    assert np.abs(a.tf(1, 0)).max() > 0.0
+    pa, ka = a.energies()
+    pb, kb = b.energies()
+    assert np.array_equal(pa, pb) and np.array_equal(ka, kb)
