@@ -47,6 +47,7 @@ struct RepD
     int32_t t0, ntiles, K, tilesPerCol;
     double L[3], dxc[3], rdxc[3];
     double dt, zeta, temperature, mass, halfSkin;
+    double damping, sigma;  ///< O-step factors of the replica's mass, computed on the host
     uint64_t seed;
     int32_t adress, tfEvery;
     mrmd_slab slab;
@@ -126,30 +127,40 @@ __device__ inline int locate(double x, double rdx, int n)
 // VelocityVerletLangevinThermostat::preForceIntegrate (VelocityVerletLangevinThermostat.hpp:63-136)
 __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
 {
+    __shared__ WaveDraws draws[BLOCK / WAVE];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = i < b.natoms;
     const int r = valid ? b.atomRep[i] : -1;
     double d2 = 0.0;
+    const RepD* R = valid ? &b.reps[r] : nullptr;
+    const int64_t cap = b.cap;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    double vx = 0.0, vy = 0.0, vz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0, dtHalf = 0.0;
     if (valid)
     {
-        const RepD& R = b.reps[r];
-        const int64_t cap = b.cap;
-        const double dt = R.dt, m = R.mass, dtHalf = 0.5 * dt;
-        double4 p = b.pos[i];
-        double vx = b.vel[i], vy = b.vel[cap + i], vz = b.vel[2 * cap + i];
+        const double dt = R->dt, m = R->mass;
+        dtHalf = 0.5 * dt;
+        p = b.pos[i];
+        vx = b.vel[i];
+        vy = b.vel[cap + i];
+        vz = b.vel[2 * cap + i];
         const double dtfm = dtHalf / m;
         vx += dtfm * b.force[i];
         vy += dtfm * b.force[cap + i];
         vz += dtfm * b.force[2 * cap + i];
-        const double ox = p.x, oy = p.y, oz = p.z;
+        ox = p.x;
+        oy = p.y;
+        oz = p.z;
         p.x += dtHalf * vx;
         p.y += dtHalf * vy;
         p.z += dtHalf * vz;
-        double xi[3];
-        normals3(R.seed, uint64_t(i - R.a0), uint64_t(*b.step), xi);
-        const double dtm = dt / m;
-        const double damping = detExp(-R.zeta * dtm);
-        const double sigma = sqrt(R.temperature / m * (1.0 - detExp(-2.0 * R.zeta * dtm)));
+    }
+    double xi[3];
+    waveNormals3(valid ? R->seed : 0, valid ? uint64_t(i - R->a0) : 0, uint64_t(*b.step), valid,
+                 draws[threadIdx.x / WAVE], xi);
+    if (valid)
+    {
+        const double damping = R->damping, sigma = R->sigma;
         vx *= damping;
         vy *= damping;
         vz *= damping;
@@ -1048,6 +1059,12 @@ int mrmd_batch_create(mrmd_ctx* c, const mrmd_replica* reps, int64_t nrep, doubl
             R.zeta = H.gamma;
             R.temperature = H.temperature;
             R.mass = H.mass;
+            {
+                // the expressions of the per-atom form; the host evaluates them to the same bits
+                const double dtm = R.dt / R.mass;
+                R.damping = detExp(-R.zeta * dtm);
+                R.sigma = sqrt(R.temperature / R.mass * (1.0 - detExp(-2.0 * R.zeta * dtm)));
+            }
             R.halfSkin = 0.5 * skin;
             R.seed = H.seed;
             R.adress = H.adress ? 1 : 0;

@@ -126,4 +126,109 @@ __host__ __device__ inline void normals3(uint64_t seed, uint64_t id, uint64_t ca
         if (have < 3) out[have++] = v * f;
     }
 }
+
+// The normals of normals3 for every lane of a 64-lane wave, with the same Philox draws, the
+// same polar acceptances in the same round order and the same arithmetic, hence the same
+// bits, but the work regrouped for the wave:
+//  1. every lane draws rounds 0-2 (88 % of lanes hold two accepted pairs after them);
+//  2. lanes still short of two pairs are served by the whole wave: each lane draws one
+//     (short lane, round) item of the following rounds, the draws pass through LDS, and
+//     each short lane takes its accepted pairs in round order, until none is short;
+//  3. the logarithm, division and square root of the transform run once per pair with the
+//     wave converged, not in every pass of a divergent rejection loop.
+// Every lane of the wave calls it with its own (seed, id) (need = false where no normals
+// are wanted); `call` is the same for the whole wave.
+struct WaveDraws
+{
+    uint64_t seed[64], id[64];
+    double u[64], v[64];
+};
+
+// beyond this round a lane keeps zero normals; it is not reached (a lane is short after
+// r rounds with probability below 0.22^(r-1))
+constexpr uint32_t MAX_NORMAL_ROUNDS = 4096;
+
+__device__ inline void waveNormals3(uint64_t seed, uint64_t id, uint64_t call, bool need, WaveDraws& w,
+                                    double out[3])
+{
+    const uint32_t c1 = uint32_t(call), c2 = uint32_t(call >> 32);
+    const int lane = int(__lane_id());
+    double u0 = 0.0, v0 = 0.0, s0 = 1.0, u1 = 0.0, v1 = 0.0, s1 = 1.0;
+    int have = need ? 0 : 2;
+    auto take = [&](double u, double v, double s) {
+        if (have == 0)
+        {
+            u0 = u;
+            v0 = v;
+            s0 = s;
+        }
+        else
+        {
+            u1 = u;
+            v1 = v;
+            s1 = s;
+        }
+        ++have;
+    };
+#pragma unroll
+    for (uint32_t r = 0; r < 3; ++r)
+    {
+        uint32_t o[4];
+        philox4x32_10(uint32_t(id), c1, c2, r, uint32_t(seed), uint32_t(seed >> 32), o);
+        const double u = symUniform(o[0], o[1]);
+        const double v = symUniform(o[2], o[3]);
+        const double s = u * u + v * v;
+        if (s < 1.0 && s > 0.0 && have < 2) take(u, v, s);
+    }
+    // every short lane has been served the same rounds so far: `next` is wave-uniform
+    for (uint32_t next = 3; next < MAX_NORMAL_ROUNDS;)
+    {
+        const bool shortOf = have < 2;
+        const uint64_t m = __ballot(shortOf);
+        if (m == 0) break;
+        const int ns = __popcll(m);
+        const int rank = __popcll(m & ((uint64_t(1) << lane) - 1));
+        if (shortOf)
+        {
+            w.seed[rank] = seed;
+            w.id[rank] = id;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int per = 64 / ns;  // rounds drawn for each short lane in this pass
+        const int k = lane / per, e = lane % per;
+        bool ok = false;
+        if (k < ns)
+        {
+            const uint64_t sk = w.seed[k];
+            uint32_t o[4];
+            philox4x32_10(uint32_t(w.id[k]), c1, c2, next + uint32_t(e), uint32_t(sk), uint32_t(sk >> 32), o);
+            const double u = symUniform(o[0], o[1]);
+            const double v = symUniform(o[2], o[3]);
+            const double s = u * u + v * v;
+            ok = s < 1.0 && s > 0.0;
+            w.u[lane] = u;
+            w.v[lane] = v;
+        }
+        const uint64_t okm = __ballot(ok);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (shortOf)
+            for (int j = 0; j < per && have < 2; ++j)
+            {
+                const int src = rank * per + j;
+                if ((okm >> src) & 1)
+                {
+                    const double u = w.u[src], v = w.v[src];
+                    take(u, v, u * u + v * v);
+                }
+            }
+        next += uint32_t(per);
+        // the next pass rewrites w
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    const double f0 = sqrt(-2.0 * detLog(s0) / s0);
+    const double f1 = sqrt(-2.0 * detLog(s1) / s1);
+    out[0] = u0 * f0;
+    out[1] = v0 * f0;
+    out[2] = u1 * f1;
+}
 }  // namespace mrmd_hip
