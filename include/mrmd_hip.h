@@ -148,6 +148,8 @@ int mrmd_ctx_set_stream(mrmd_ctx* ctx, void* hip_stream);
 int mrmd_ctx_synchronize(mrmd_ctx* ctx);
 /* per-kernel HIP-event timing on the context stream (off by default) */
 int mrmd_ctx_set_profiling(mrmd_ctx* ctx, int enable);
+/* time only the entry points of this timer name (e.g. "lj_force"); NULL or "" = all */
+int mrmd_ctx_set_profile_filter(mrmd_ctx* ctx, const char* kernel);
 int mrmd_ctx_kernel_stats(mrmd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
 int mrmd_ctx_reset_stats(mrmd_ctx* ctx);
 /* bytes of device memory currently held by this context's objects */
@@ -322,6 +324,32 @@ int mrmd_langevin_pre(mrmd_atoms* atoms, double dt, double zeta, double temperat
  * scaled by sqrt(1 + gamma (target / current - 1)); nothing happens for current <= 0, and a
  * target <= 0 is an error */
 int mrmd_berendsen_apply(mrmd_atoms* atoms, double current_temperature, double target_temperature, double gamma);
+
+/* The time loop of the reference's LJ examples over the calls above, run natively:
+ * examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:135-186 (velocity Verlet; exchange,
+ * ghosts and list on a rebuild, ghost update otherwise) with the Langevin pre-force
+ * integrator of examples/01_LennardJones_NVT.cpp:139-214 (langevin = 1) and, when `cells` is
+ * not NULL, LinkedCellList + Atoms::permute on every rebuild (tests/NVT/NVT.cpp:138-144).
+ * The box's corners give the list grid (min - r_n, max + r_n) and the cell list (delta r_n).
+ * The state fields are read and updated, so consecutive calls continue one run; the
+ * results equal those of the same calls made one by one. */
+typedef struct mrmd_md_loop {
+    double dt;
+    double neighbor_radius; /* r_n = rc + skin of the list builds */
+    double skin;            /* rebuild when the accumulated displacement reaches skin / 2 */
+    double cell_ratio;
+    int64_t max_neigh;
+    int32_t langevin;       /* 0: VelocityVerlet, 1: VelocityVerletLangevinThermostat */
+    int32_t reserved;
+    double zeta, temperature;
+    uint64_t seed;
+    uint64_t call;          /* state: Langevin noise counter, one per step */
+    double max_disp;        /* state: displacement accumulated since the last rebuild */
+    int64_t rebuilds;       /* state */
+    int64_t steps;          /* state */
+} mrmd_md_loop;
+int mrmd_md_run(mrmd_atoms* atoms, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_lj* lj,
+                mrmd_cell_list* cells, const mrmd_box* box, mrmd_md_loop* loop, int64_t nsteps);
 /* analysis::getKineticEnergy (mrmd/analysis/KineticEnergy.hpp:26-39); synchronises */
 int mrmd_kinetic_energy(const mrmd_atoms* atoms, double* energy);
 

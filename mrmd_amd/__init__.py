@@ -64,7 +64,9 @@ class Context:
 
     profiling = False
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, only=None):
+        """per-kernel HIP-event timing; only = one timer name (the others record nothing)"""
+        self.check(_L.mrmd_ctx_set_profile_filter(self.h, only.encode() if only else None))
         self.check(_L.mrmd_ctx_set_profiling(self.h, 1 if on else 0))
         self.profiling = bool(on)
 

@@ -47,8 +47,15 @@ class Spherical(C.Structure):
                 ("exponent", C.c_int64)]
 
 
+class MdLoop(C.Structure):
+    _fields_ = [("dt", C.c_double), ("neighbor_radius", C.c_double), ("skin", C.c_double), ("cell_ratio", C.c_double),
+                ("max_neigh", C.c_int64), ("langevin", C.c_int32), ("reserved", C.c_int32), ("zeta", C.c_double),
+                ("temperature", C.c_double), ("seed", C.c_uint64), ("call", C.c_uint64), ("max_disp", C.c_double),
+                ("rebuilds", C.c_int64), ("steps", C.c_int64)]
+
+
 STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab, "mrmd_replica": Replica,
-           "mrmd_spherical": Spherical}
+           "mrmd_spherical": Spherical, "mrmd_md_loop": MdLoop}
 
 _SCALARS = {
     "int": C.c_int, "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64,

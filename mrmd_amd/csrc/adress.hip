@@ -555,13 +555,22 @@ __global__ void k_updateMeanComp(double* comp, double* cnt, double* mean, int64_
     cnt[k] = 0.0;
 }
 
-__global__ void k_sumOrdered(const double* partials, int64_t n, double* out)
+// host (if not null): pinned host memory that receives the sum too, written by the kernel
+__global__ void k_sumOrdered(const double* partials, int64_t n, double* out, double* host)
 {
     __shared__ double red[BLOCK / WAVE];
     double s = 0.0;
     for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k];
     s = blockSum(s, red);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0)
+    {
+        *out = s;
+        if (host)
+        {
+            *host = s;
+            __threadfence_system();
+        }
+    }
 }
 
 __global__ void k_zeroD(double* p, int64_t n)
@@ -865,6 +874,13 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     if (!tiled) vl->ensureHalf();
     const unsigned grid = tiled ? unsigned(vl->ntiles) : gridFor(rows);
     double* part = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid + 1) * sizeof(double)));
+    // the energy's host copy: written by the summing kernel, 0.0 when nothing runs
+    double* host = nullptr;
+    if (energy)
+    {
+        ctx->pinned[0] = 0.0;
+        host = ctx->pinned;
+    }
     const int64_t nh = COMP_BINS * s->numTypes;
     const double compBin = (1.0 - 0.0) / double(COMP_BINS);
     IGArgs g{m->view(), a->view(), vl->hnbr.p, vl->hcnt.p, vl->stride, rows, s->dtable.p, int(s->numTypes), s->rcSqr,
@@ -884,13 +900,15 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
             hipLaunchKernelGGL(k_igTiled<true>, dim3(grid), dim3(vl->block), lds, ctx->stream, g, tl, p, part);
         else
             hipLaunchKernelGGL(k_igTiled<false>, dim3(grid), dim3(vl->block), lds, ctx->stream, g, tl, p, part);
-        hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), part + grid);
+        hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), part + grid,
+                           host);
     }
     else if (rows > 0)
     {
         KTimer t(ctx, "lj_idealgas");
         hipLaunchKernelGGL(k_ljIdealGas, dim3(grid), dim3(BLOCK), 0, ctx->stream, g, part);
-        hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), part + grid);
+        hipLaunchKernelGGL(k_sumOrdered, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), part + grid,
+                           host);
     }
     else
         hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, ctx->stream, part + grid, int64_t(1));
@@ -901,7 +919,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     ++s->runCounter;
     if (energy)
     {
-        HIPCHECK(hipMemcpyAsync(ctx->pinned, part + grid, 8, hipMemcpyDeviceToHost, ctx->stream));
+        // written by k_sumOrdered (0.0 from the host when there are no rows)
         sync(ctx);
         *energy = ctx->pinned[0];
     }

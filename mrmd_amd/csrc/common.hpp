@@ -68,6 +68,7 @@ struct mrmd_ctx
     bool ownStream = false;
     std::string err;
     bool profiling = false;
+    std::string profileOnly;  ///< non-empty: only timers of this name record
     std::map<std::string, mrmd_hip::KStat> stats;
     std::vector<hipEvent_t> eventPool;
     double* pinned = nullptr;  // small pinned host staging area (64 doubles)
@@ -207,7 +208,7 @@ struct KTimer
     hipEvent_t a = nullptr, b = nullptr;
     KTimer(mrmd_ctx* c, const char* n) : ctx(c), name(n)
     {
-        if (!ctx->profiling) return;
+        if (!ctx->profiling || (!ctx->profileOnly.empty() && ctx->profileOnly != name)) return;
         a = take();
         b = take();
         HIPCHECK(hipEventRecord(a, ctx->stream));

@@ -174,6 +174,12 @@ __global__ void k_fill(double* p, int64_t n, double v)
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
 }
+// the three components of a vector field in one launch: component blockIdx.y at p + y * stride
+__global__ void k_fill3(double* p, int64_t stride, int64_t n, double v)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[blockIdx.y * stride + k] = v;
+}
 __global__ void k_fillPos(double4* p, int64_t n, double v)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -370,6 +376,13 @@ int mrmd_ctx_set_profiling(mrmd_ctx* c, int enable)
 {
     MRMD_API_BEGIN(c)
     c->profiling = enable != 0;
+    MRMD_API_END
+}
+
+int mrmd_ctx_set_profile_filter(mrmd_ctx* c, const char* kernel)
+{
+    MRMD_API_BEGIN(c)
+    c->profileOnly = kernel ? kernel : "";
     MRMD_API_END
 }
 
@@ -583,8 +596,7 @@ int mrmd_atoms_fill(mrmd_atoms* a, int field, double value)
         case MRMD_FORCE:
         {
             double* p = field == MRMD_VEL ? a->vel.p : a->force.p;
-            for (int d = 0; d < 3; ++d)
-                hipLaunchKernelGGL(k_fill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, p + d * a->cap, n, value);
+            hipLaunchKernelGGL(k_fill3, dim3(gridFor(n), 3), dim3(BLOCK), 0, ctx->stream, p, a->cap, n, value);
             break;
         }
         case MRMD_TYPE:
@@ -805,7 +817,7 @@ int mrmd_molecules_fill(mrmd_molecules* m, int field, double value)
         case MRMD_MOL_GRADLAMBDA:
         {
             double* p = field == MRMD_MOL_FORCE ? m->force.p : m->grad.p;
-            for (int d = 0; d < 3; ++d) hipLaunchKernelGGL(k_fill, g, b, 0, ctx->stream, p + d * m->cap, n, value);
+            hipLaunchKernelGGL(k_fill3, dim3(gridFor(n), 3), b, 0, ctx->stream, p, m->cap, n, value);
             break;
         }
         case MRMD_MOL_LAMBDA:

@@ -183,7 +183,9 @@ __global__ void k_scaleVel(AtomsView a, int64_t n, double beta)
 
 // per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
 // eight independent loads per lane in flight)
-__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out)
+// host (if not null) is pinned host memory that receives the result too, written by the
+// kernel itself (no copy launch after it)
+__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, double* host)
 {
     __shared__ double red[BLOCK / WAVE];
     double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -196,7 +198,15 @@ __global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, i
     for (; k < n; k += BLOCK) m[0] = fmax(m[0], partials[k]);
     double r = fmax(fmax(fmax(m[0], m[1]), fmax(m[2], m[3])), fmax(fmax(m[4], m[5]), fmax(m[6], m[7])));
     r = blockMax(r, red);
-    if (threadIdx.x == 0) *out = r;
+    if (threadIdx.x == 0)
+    {
+        *out = r;
+        if (host)
+        {
+            *host = r;
+            __threadfence_system();
+        }
+    }
 }
 
 // scratch slot 3: [0] maximum squared displacement of the last pre-step, [1 ..] partials
@@ -206,12 +216,21 @@ double* dispPartials(mrmd_atoms* a, unsigned grid)
     return static_cast<double*>(ctxScratch(a->ctx, 3, (size_t(grid) + 1) * sizeof(double))) + 1;
 }
 
+// the reduction kernel has written the maximum to ctx->pinned[0] (or the caller stored 0.0
+// there when no atom moved); the synchronisation makes it visible here
 void finishDisp(mrmd_atoms* a, double* maxDisp)
 {
     if (!maxDisp) return;
-    HIPCHECK(hipMemcpyAsync(a->ctx->pinned, dispScalar(a), 8, hipMemcpyDeviceToHost, a->ctx->stream));
     sync(a->ctx);
     *maxDisp = sqrt(a->ctx->pinned[0]);
+}
+
+// target of the reduction's host copy: the context's pinned slot when the caller wants the value
+double* dispHost(mrmd_atoms* a, const double* maxDisp)
+{
+    if (!maxDisp) return nullptr;
+    a->ctx->pinned[0] = 0.0;
+    return a->ctx->pinned;
 }
 }  // namespace
 
@@ -225,11 +244,12 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     const unsigned grid = gridFor(a->nlocal);
     double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
+    double* host = dispHost(a, maxDisp);
     if (a->nlocal > 0)
     {
         KTimer t(ctx, "vv_pre");
         hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
     }
     else
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
@@ -259,6 +279,7 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
     const unsigned grid = gridFor(a->nlocal);
     double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
+    double* host = dispHost(a, maxDisp);
     if (a->nlocal == 0) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
     {
@@ -275,7 +296,7 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
             uni ? launch(k_langevinPre<true, true>) : launch(k_langevinPre<true, false>);
         else
             uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
     }
     checkLaunch();
     finishDisp(a, maxDisp);

@@ -1,0 +1,75 @@
+// The time loop of the reference's LJ examples, run natively over this library's entry
+// points: examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:135-186 (pre-force
+// integration, rebuild check, exchange + ghosts + list on a rebuild, ghost update
+// otherwise, zero forces, LennardJones::apply, ghost force fold, post-force integration),
+// with the Langevin pre-force integrator of examples/01_LennardJones_NVT.cpp:139-214 and the
+// LinkedCellList + Atoms::permute of tests/NVT/NVT.cpp:138-144 on every rebuild when a cell
+// list is given.  Each step makes the calls a caller of the actions makes, in the same
+// order, so its results are those of the same calls made one by one.
+#include <cfloat>
+
+#include "containers.hpp"
+
+using namespace mrmd_hip;
+
+extern "C" {
+
+int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_lj* lj, mrmd_cell_list* cells,
+                const mrmd_box* box, mrmd_md_loop* s, int64_t nsteps)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && gl && vl && lj && box && s && nsteps >= 0, "invalid arguments");
+    REQUIRE(s->dt > 0.0 && s->neighbor_radius > 0.0 && s->skin >= 0.0 && s->cell_ratio > 0.0,
+            "dt, neighbor_radius and cell_ratio must be positive, skin non-negative");
+    const double rn = s->neighbor_radius;
+    double gridMin[3], gridMax[3], delta[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        gridMin[d] = box->min[d] - rn;
+        gridMax[d] = box->max[d] + rn;
+        delta[d] = rn;
+    }
+#define MD_CALL(expr)                         \
+    do                                        \
+    {                                         \
+        const int rc_ = (expr);               \
+        if (rc_ != MRMD_OK) return rc_;       \
+    } while (0)
+    for (int64_t k = 0; k < nsteps; ++k)
+    {
+        double d = 0.0;
+        if (s->langevin)
+        {
+            MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
+            ++s->call;
+        }
+        else
+            MD_CALL(mrmd_vv_pre(a, s->dt, &d));
+        s->max_disp += d;
+        if (s->max_disp >= s->skin * 0.5)
+        {
+            s->max_disp = 0.0;
+            MD_CALL(mrmd_halo_wrap(a, box));
+            if (cells)
+            {
+                MD_CALL(mrmd_cell_list_build(cells, a, 0, a->nlocal, delta, box->min, box->max));
+                MD_CALL(mrmd_atoms_permute(a, cells));
+            }
+            int64_t ng = 0;
+            MD_CALL(mrmd_halo_create(gl, a, box, &ng));
+            MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, gridMin, gridMax, s->max_neigh));
+            ++s->rebuilds;
+        }
+        else
+            MD_CALL(mrmd_halo_update(gl, a, box));
+        MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
+        MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
+        MD_CALL(mrmd_halo_fold_forces(gl, a));
+        MD_CALL(mrmd_vv_post(a, s->dt));
+        ++s->steps;
+    }
+#undef MD_CALL
+    MRMD_API_END
+}
+
+}  // extern "C"

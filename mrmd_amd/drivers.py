@@ -14,8 +14,12 @@ ContributeMoleculeForceToAtoms, ghost fold, ThermodynamicForce sample / periodic
 apply, Langevin post-force integration (action/LJ_IdealGas.cpp:227-260,
 action/ContributeMoleculeForceToAtoms.cpp:23-48, action/ThermodynamicForce.hpp:98-152).
 """
+import ctypes as C
+
 import numpy as np
 
+from . import _L
+from .capi import MdLoop
 from . import (FORCE, POS, PRED_SLAB_HY, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList,
                LJ_IdealGas, Molecules, Slab, ThermodynamicForce, VelocityVerlet, VelocityVerletLangevinThermostat,
                make_box, pred1)
@@ -78,14 +82,34 @@ class LJSystem:
         VelocityVerlet.post_force_integrate(a, dt)
         self.step_count += 1
 
-    def run(self, nsteps, dt, energy_every=0):
-        rec = []
-        for s in range(nsteps):
-            self.step(dt)
-            if energy_every and s % energy_every == 0:
-                e, w = self.lj.energy_virial()
-                rec.append((e, self.atoms.kinetic_energy(), w))
-        return np.array(rec)
+    def run(self, nsteps, dt, energy_every=0, native=True):
+        """nsteps steps.  Without energy records they run in the library's native loop
+        (mrmd_md_run: the calls of step(), in its order, made from C++), otherwise step by
+        step; both give the same results."""
+        if energy_every or not native:
+            rec = []
+            for s in range(nsteps):
+                self.step(dt)
+                if energy_every and s % energy_every == 0:
+                    e, w = self.lj.energy_virial()
+                    rec.append((e, self.atoms.kinetic_energy(), w))
+            return np.array(rec)
+        a = self.atoms
+        if self.permute and self.cells is None:
+            self.cells = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
+        lang = self.langevin
+        loop = MdLoop(dt=dt, neighbor_radius=self.rn, skin=self.skin, cell_ratio=self.cell_ratio,
+                      max_neigh=int(self.max_neigh), langevin=1 if lang is not None else 0,
+                      zeta=lang.zeta if lang else 0.0, temperature=lang.temperature if lang else 0.0,
+                      seed=lang.seed if lang else 0, call=lang.call if lang else 0, max_disp=self.max_disp,
+                      rebuilds=self.rebuilds, steps=self.step_count)
+        self.ctx.check(_L.mrmd_md_run(a.h, self.ghosts.h, self.vl.h, self.lj.h,
+                                      self.cells.h if self.permute else None, C.byref(self.box), C.byref(loop),
+                                      int(nsteps)))
+        self.max_disp, self.rebuilds, self.step_count = loop.max_disp, loop.rebuilds, loop.steps
+        if lang is not None:
+            lang.call = loop.call
+        return np.zeros((0, 3))
 
     def local(self, field):
         return self.atoms.download(field, self.atoms.nlocal)

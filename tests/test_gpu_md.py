@@ -79,3 +79,27 @@ def test_langevin_permute_loop_matches_early_steps(M, ctx):
     np.testing.assert_allclose(eg[:, 1], ec[:, 1], rtol=1e-10)
     # same permutation and state on both sides
     np.testing.assert_allclose(sysg.local(M.POS), a.get(O.POS, a.nlocal), rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("langevin,permute", [(False, False), (True, True)], ids=["nve", "nvt-permute"])
+def test_native_loop_equals_step_calls(M, ctx, langevin, permute):
+    """mrmd_md_run (the example loop in C++) against the same calls made step by step from
+    Python: bit-identical positions, velocities and forces, same rebuilds and noise counter,
+    across two native calls that continue one run"""
+    from mrmd_amd.drivers import LJSystem
+    pos, L = fcc(8, 0.8442)
+    vel = maxwell_boltzmann(pos.shape[0], 1.2, 17)
+    kw = dict(langevin=langevin, gamma=1.0, temperature=1.2, seed=5, permute=permute)
+    a = LJSystem(ctx, pos, vel, L, **kw)
+    b = LJSystem(ctx, pos, vel, L, **kw)
+    for _ in range(60):
+        a.step(0.005)
+    b.run(25, 0.005)
+    b.run(35, 0.005)
+    assert a.rebuilds == b.rebuilds and a.rebuilds >= 3 and a.step_count == b.step_count == 60
+    assert a.max_disp == b.max_disp
+    if langevin:
+        assert a.langevin.call == b.langevin.call == 60
+    assert a.atoms.nghost == b.atoms.nghost
+    for f in (M.POS, M.VEL, M.FORCE):
+        assert np.array_equal(a.local(f), b.local(f)), f
