@@ -419,6 +419,34 @@ int mrmd_tf_set_density(mrmd_thermo_force* tf, const double* host, int64_t sampl
 /* getMuLeft / getMuRight (ThermodynamicForce.cpp:98-130) */
 int mrmd_tf_mu(const mrmd_thermo_force* tf, double* left, double* right);
 
+/* The LJ / ideal-gas AdResS slab step over the calls above, run natively: Langevin
+ * pre-force integration; on a rebuild molecule centres, MultiResPeriodicGhostExchange
+ * (exchange + ghosts) and the molecule Verlet list (grid box - r_n .. box + r_n, cell ratio
+ * 1), otherwise a ghost refresh; UpdateMolecules with the slab weighting; zero atom and
+ * molecule forces; LJ_IdealGas::run (LJ_IdealGas.cpp:227-260);
+ * ContributeMoleculeForceToAtoms; ghost fold; ThermodynamicForce::sample, update_if every
+ * tf_every steps (ThermodynamicForce.hpp:124-152), apply; post-force integration.  State
+ * fields are read and updated, so consecutive calls continue one run; results equal those
+ * of the same calls made one by one. */
+typedef struct mrmd_adress_loop {
+    double dt;
+    double neighbor_radius;
+    double skin;
+    double zeta, temperature;
+    uint64_t seed;
+    int64_t tf_every;      /* thermodynamic-force update interval in steps, 0 = never */
+    double tf_sigma, tf_range;
+    mrmd_pred1 tf_pred;    /* update_if predicate over the bin centres */
+    uint64_t call;         /* state: Langevin noise counter */
+    double max_disp;       /* state */
+    int64_t rebuilds;      /* state */
+    int64_t steps;         /* state: steps taken (the update interval counts them) */
+    double energy;         /* out: LJ_IdealGas energy of the last step */
+} mrmd_adress_loop;
+int mrmd_adress_md_run(mrmd_atoms* atoms, mrmd_molecules* mols, mrmd_ghost_layer* gl, mrmd_verlet_list* vl,
+                       mrmd_lj_idealgas* ig, mrmd_thermo_force* tf, const mrmd_slab* slab, const mrmd_box* box,
+                       mrmd_adress_loop* loop, int64_t nsteps);
+
 /* MultiHistogram operations on host arrays [nbins x nhist], executed on the device
  * (MultiHistogram.cpp:113-212, MultiHistogram.cpp:76-90) */
 int mrmd_hist_gradient(mrmd_ctx* ctx, double min, double max, int64_t nbins, int64_t nhist,

@@ -54,8 +54,15 @@ class MdLoop(C.Structure):
                 ("rebuilds", C.c_int64), ("steps", C.c_int64)]
 
 
+class AdressLoop(C.Structure):
+    _fields_ = [("dt", C.c_double), ("neighbor_radius", C.c_double), ("skin", C.c_double), ("zeta", C.c_double),
+                ("temperature", C.c_double), ("seed", C.c_uint64), ("tf_every", C.c_int64), ("tf_sigma", C.c_double),
+                ("tf_range", C.c_double), ("tf_pred", Pred1), ("call", C.c_uint64), ("max_disp", C.c_double),
+                ("rebuilds", C.c_int64), ("steps", C.c_int64), ("energy", C.c_double)]
+
+
 STRUCTS = {"mrmd_box": Box, "mrmd_pred1": Pred1, "mrmd_pred2": Pred2, "mrmd_slab": Slab, "mrmd_replica": Replica,
-           "mrmd_spherical": Spherical, "mrmd_md_loop": MdLoop}
+           "mrmd_spherical": Spherical, "mrmd_md_loop": MdLoop, "mrmd_adress_loop": AdressLoop}
 
 _SCALARS = {
     "int": C.c_int, "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64,

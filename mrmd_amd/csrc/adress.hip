@@ -589,12 +589,23 @@ __global__ __launch_bounds__(BLOCK) void k_tfSample(const double4* pos, int64_t 
     const int64_t cells = h.numBins * h.numHist;
     for (int64_t c = threadIdx.x; c < cells; c += blockDim.x) loc[c] = 0.0;
     __syncthreads();
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    // four records in flight per lane, then their LDS increments
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride)
     {
-        const double4 p = pos[i];
-        const int64_t bin = getBin(p.x, h.min, h.invBinSize, h.numBins);
-        if (bin == -1) continue;
-        atomicAdd(&loc[bin * h.numHist + int64_t(p.w)], 1.0);
+        double4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int64_t i = i0 + u * stride;
+            p[u] = i < n ? pos[i] : make_double4(h.min - 1.0, 0.0, 0.0, 0.0);  // below min: no bin
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int64_t bin = getBin(p[u].x, h.min, h.invBinSize, h.numBins);
+            if (bin != -1) atomicAdd(&loc[bin * h.numHist + int64_t(p[u].w)], 1.0);
+        }
     }
     __syncthreads();
     for (int64_t c = threadIdx.x; c < cells; c += blockDim.x)
@@ -1036,7 +1047,9 @@ int mrmd_tf_sample(mrmd_thermo_force* t, const mrmd_atoms* a)
     if (a->nlocal > 0)
     {
         KTimer k(t->ctx, "tf_sample");
-        const unsigned grid = std::min<unsigned>(gridFor(a->nlocal), 1024u);
+        // about 4096 records per workgroup: every workgroup ends with one global atomic per
+        // non-empty bin, and those contend on the few cache lines of the profile
+        const unsigned grid = unsigned(std::min<int64_t>(std::max<int64_t>((a->nlocal + 4095) / 4096, 1), 256));
         hipLaunchKernelGGL(k_tfSample, dim3(grid), dim3(BLOCK), 0, t->ctx->stream, a->pos.p, a->nlocal, t->hd,
                            t->density.p);
         checkLaunch();

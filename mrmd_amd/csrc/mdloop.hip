@@ -72,4 +72,69 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     MRMD_API_END
 }
 
+// One step of the LJ / ideal-gas AdResS slab system in the order the actions' contracts
+// force (molecule forces reach ghost atoms before the ghost fold; lambda is current before
+// LJ_IdealGas::run): Langevin pre-force integration; on a rebuild UpdateMolecules (centres),
+// MultiResPeriodicGhostExchange (exchange + ghosts), UpdateMolecules, the molecule Verlet
+// list, otherwise a ghost refresh; UpdateMolecules with the slab weighting; zero atom and
+// molecule forces; LJ_IdealGas::run; ContributeMoleculeForceToAtoms; ghost fold;
+// ThermodynamicForce sample, update_if every tf_every steps, apply; post-force integration
+// (action/LJ_IdealGas.cpp:227-260, action/ContributeMoleculeForceToAtoms.cpp:23-48,
+// action/ThermodynamicForce.hpp:98-152).  The energy is read back on the last step only.
+int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, mrmd_verlet_list* vl,
+                       mrmd_lj_idealgas* ig, mrmd_thermo_force* tf, const mrmd_slab* slab, const mrmd_box* box,
+                       mrmd_adress_loop* s, int64_t nsteps)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && m && gl && vl && ig && tf && slab && box && s && nsteps >= 0, "invalid arguments");
+    REQUIRE(s->dt > 0.0 && s->neighbor_radius > 0.0 && s->skin >= 0.0 && s->tf_every >= 0,
+            "dt and neighbor_radius must be positive, skin and tf_every non-negative");
+    const double rn = s->neighbor_radius;
+    double gridMin[3], gridMax[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        gridMin[d] = box->min[d] - rn;
+        gridMax[d] = box->max[d] + rn;
+    }
+#define MD_CALL(expr)                         \
+    do                                        \
+    {                                         \
+        const int rc_ = (expr);               \
+        if (rc_ != MRMD_OK) return rc_;       \
+    } while (0)
+    for (int64_t k = 0; k < nsteps; ++k)
+    {
+        double d = 0.0;
+        MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
+        ++s->call;
+        s->max_disp += d;
+        if (s->max_disp >= s->skin * 0.5)
+        {
+            s->max_disp = 0.0;
+            MD_CALL(mrmd_molecules_update(m, a, nullptr));
+            MD_CALL(mrmd_multires_wrap(m, a, box));
+            MD_CALL(mrmd_multires_halo_create(gl, m, a, box));
+            MD_CALL(mrmd_molecules_update(m, a, nullptr));
+            MD_CALL(mrmd_verlet_build_molecules(vl, m, 0, m->nlocal, rn, 1.0, gridMin, gridMax, 0));
+            ++s->rebuilds;
+        }
+        else
+            MD_CALL(mrmd_halo_update(gl, a, box));
+        MD_CALL(mrmd_molecules_update(m, a, slab));
+        MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
+        MD_CALL(mrmd_molecules_fill(m, MRMD_MOL_FORCE, 0.0));
+        MD_CALL(mrmd_ljig_run(ig, m, vl, a, k + 1 == nsteps ? &s->energy : nullptr));
+        MD_CALL(mrmd_molecules_contribute_force(m, a));
+        MD_CALL(mrmd_halo_fold_forces(gl, a));
+        MD_CALL(mrmd_tf_sample(tf, a));
+        ++s->steps;
+        if (s->tf_every && s->steps % s->tf_every == 0)
+            MD_CALL(mrmd_tf_update(tf, s->tf_sigma, s->tf_range, &s->tf_pred));
+        MD_CALL(mrmd_tf_apply(tf, a, nullptr));
+        MD_CALL(mrmd_vv_post(a, s->dt));
+    }
+#undef MD_CALL
+    MRMD_API_END
+}
+
 }  // extern "C"

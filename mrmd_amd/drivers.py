@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 
 from . import _L
-from .capi import MdLoop
+from .capi import AdressLoop, MdLoop
 from . import (FORCE, POS, PRED_SLAB_HY, VEL, Atoms, GhostLayer, HalfVerletList, LennardJones, LinkedCellList,
                LJ_IdealGas, Molecules, Slab, ThermodynamicForce, VelocityVerlet, VelocityVerletLangevinThermostat,
                make_box, pred1)
@@ -176,9 +176,27 @@ class AdResSSystem:
         self.tf.apply(a)
         self.langevin.post_force_integrate(a, dt)
 
-    def run(self, nsteps, dt):
-        for _ in range(nsteps):
-            self.step(dt)
+    def run(self, nsteps, dt, native=True):
+        """nsteps steps: in the library's native loop (mrmd_adress_md_run: the calls of
+        step(), in its order, made from C++; the energy is read back on the last step only)
+        or step by step; both give the same results"""
+        if not native:
+            for _ in range(nsteps):
+                self.step(dt)
+            return
+        if nsteps <= 0:
+            return
+        lang = self.langevin
+        loop = AdressLoop(dt=dt, neighbor_radius=self.rn, skin=self.skin, zeta=lang.zeta,
+                          temperature=lang.temperature, seed=lang.seed, tf_every=int(self.tf_every or 0),
+                          tf_sigma=self.tf_sigma, tf_range=self.tf_range, tf_pred=self.tf_pred, call=lang.call,
+                          max_disp=self.max_disp, rebuilds=self.rebuilds, steps=self.step_count)
+        self.ctx.check(_L.mrmd_adress_md_run(self.atoms.h, self.mols.h, self.ghosts.h, self.vl.h, self.ljig.h,
+                                             self.tf.h, C.byref(self.slab.params), C.byref(self.box), C.byref(loop),
+                                             int(nsteps)))
+        self.max_disp, self.rebuilds, self.step_count = loop.max_disp, loop.rebuilds, loop.steps
+        lang.call = loop.call
+        self.energy = loop.energy
 
     def local(self, field):
         return self.atoms.download(field, self.atoms.nlocal)

@@ -313,3 +313,32 @@ def test_adress_trajectory_matches_oracle(M, ctx):
     assert np.abs(dv).max() < 1e-7, np.abs(dv).max()
     np.testing.assert_allclose(gsys.tf.get_force(), osys.tf.force(), rtol=1e-9, atol=1e-10)
     assert np.abs(osys.tf.force()).max() > 0.0
+
+
+def test_adress_native_loop_equals_step_calls(M, ctx):
+    """mrmd_adress_md_run (the AdResS step in C++) against the same calls made step by step
+    from Python: same rebuilds, noise counter, run counter and thermodynamic-force updates;
+    positions agree to 1e-9 (ghost-partner forces are added by atomics, so the order of
+    additions, and hence the last bits, can differ between any two runs)"""
+    from mrmd_amd.drivers import AdResSSystem
+    from mrmd_amd.lattice import maxwell_boltzmann
+    L = [40.0, 12.0, 12.0]
+    g = np.random.default_rng(21)
+    grid = np.stack(np.meshgrid(np.arange(32) * 1.25, np.arange(10) * 1.2, np.arange(10) * 1.2, indexing="ij"), -1)
+    pos = np.mod(grid.reshape(-1, 3) + 0.3 + g.uniform(-0.05, 0.05, (3200, 3)), L)
+    vel = maxwell_boltzmann(pos.shape[0], 1.0, 5)
+    kw = dict(cap=0.7, seed=11, tf_every=10)
+    a = AdResSSystem(ctx, pos, vel, L, 8.0, 5.0, 1, **kw)
+    b = AdResSSystem(ctx, pos, vel, L, 8.0, 5.0, 1, **kw)
+    for _ in range(40):
+        a.step(0.005)
+    b.run(15, 0.005)
+    b.run(25, 0.005)
+    assert a.rebuilds == b.rebuilds and a.rebuilds >= 2
+    assert a.step_count == b.step_count == 40 and a.langevin.call == b.langevin.call == 40
+    assert a.ljig.run_counter == b.ljig.run_counter == 40 and a.tf.samples == b.tf.samples
+    assert a.energy == pytest.approx(b.energy, rel=1e-10)
+    d = a.local(M.POS) - b.local(M.POS)
+    d -= np.asarray(L) * np.round(d / np.asarray(L))
+    assert np.abs(d).max() < 1e-9, np.abs(d).max()
+    assert np.abs(a.tf.get_force() - b.tf.get_force()).max() < 1e-9
