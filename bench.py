@@ -206,7 +206,7 @@ def adress_slab(ctx, steps, warm=50, profile_steps=100):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     rebuilds = s.rebuilds - r0
-    names = ("lj_idealgas_tiled", "lj_idealgas", "verlet_build", "langevin_pre", "vv_post", "halo_update",
+    names = ("lj_idealgas_tiled", "lj_idealgas", "verlet_build", "post_pre", "langevin_pre", "vv_post", "halo_update",
              "halo_fold", "update_molecules", "contribute_molecule_force", "tf_sample", "tf_apply")
     ctx.set_profiling(True)
     ctx.reset_stats()
@@ -319,7 +319,7 @@ def main():
     ctx.set_profiling(True)
     ctx.reset_stats()
     s.run(args.profile_steps, dt)
-    stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "langevin_pre", "vv_post", "halo_create",
+    stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "post_pre", "langevin_pre", "vv_post", "halo_create",
                                               "halo_update", "halo_fold", "atoms_permute", "cell_list_build")}
     nrec = s.atoms.nlocal + s.atoms.nghost
     full_entries = s.vl.info()["total_full"]
@@ -337,10 +337,13 @@ def main():
                 "avg_ms": t / k if k else None, "frac_of_hbm_peak": nbytes / (t / k / 1e3) / 1e9 / HBM_PEAK_GBS
                 if k else None}
 
-    # algorithmic bytes: integrators 128 B/atom (pre) and 80 B/atom (post); list build:
-    # read 32 B + write 4 B cell index + 4 B slot + 32 B staged copy per record, 2 B per
-    # list entry and 4 B per row count
-    bandwidth = {"langevin_pre": gbs("langevin_pre", 128.0 * n), "vv_post": gbs("vv_post", 80.0 * n),
+    # algorithmic bytes: Langevin pre-force step 136 B/atom (read position 32, velocity 24,
+    # force 24; write position 32, velocity 24), post-force kick 80 B/atom (read velocity,
+    # force, mass; write velocity), and the fused post + next pre pass of the time loop the
+    # same 136 B/atom as the pre-force step alone; list build: read 32 B + write 4 B cell
+    # index + 4 B slot + 32 B staged copy per record, 2 B per list entry and 4 B per row count
+    bandwidth = {"post_pre": gbs("post_pre", 136.0 * n), "langevin_pre": gbs("langevin_pre", 136.0 * n),
+                 "vv_post": gbs("vv_post", 80.0 * n),
                  "verlet_build": gbs("verlet_build", 72.0 * nrec + 2.0 * full_entries + 4.0 * nrec)}
 
     cpu = None

@@ -332,7 +332,8 @@ int mrmd_berendsen_apply(mrmd_atoms* atoms, double current_temperature, double t
  * not NULL, LinkedCellList + Atoms::permute on every rebuild (tests/NVT/NVT.cpp:138-144).
  * The box's corners give the list grid (min - r_n, max + r_n) and the cell list (delta r_n).
  * The state fields are read and updated, so consecutive calls continue one run; the
- * results equal those of the same calls made one by one. */
+ * results equal those of the same calls made one by one, bit for bit (a step's post-force
+ * and the next step's pre-force integration run as one pass with the same arithmetic). */
 typedef struct mrmd_md_loop {
     double dt;
     double neighbor_radius; /* r_n = rc + skin of the list builds */

@@ -112,4 +112,10 @@ __device__ inline double coord(const double4& p, int axis)
 {
     return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
 }
+
+// a time step's post-force integration fused with the next step's pre-force integration
+// (velocity Verlet, or Langevin without a predicate), bit-identical to the two calls;
+// *maxDisp = sqrt(max |dx|^2) of the pre-force part (synchronises)
+int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
+                 uint64_t call, double* maxDisp);
 }  // namespace mrmd_hip

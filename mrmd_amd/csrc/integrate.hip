@@ -147,6 +147,87 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
     if (threadIdx.x == 0) partials[blockIdx.x] = mx;
 }
 
+// A step's post-force integration (k_vvPost) fused with the next step's pre-force
+// integration, for time loops that run both back to back: per atom the two kernels'
+// arithmetic in the same order (the force is the same between them), so the results are
+// bit-identical, with one pass over the arrays instead of two.  LANGEVIN: the pre-force
+// part is k_langevinPre<false, UNI>, otherwise k_vvPre.
+template <bool LANGEVIN, bool UNI>
+__global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
+                                                   uint64_t seed, uint64_t call, LangevinFactors uf, double* partials)
+{
+    __shared__ double red[BLOCK / WAVE];
+    __shared__ WaveDraws draws[LANGEVIN ? BLOCK / WAVE : 1];
+    double d2 = 0.0;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const double dtHalf = 0.5 * dt;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    double vx = 0.0, vy = 0.0, vz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+    LangevinFactors f = uf;
+    if (live)
+    {
+        const double fx = a.force[i], fy = a.force[a.cap + i], fz = a.force[2 * a.cap + i];
+        vx = a.vel[i];
+        vy = a.vel[a.cap + i];
+        vz = a.vel[2 * a.cap + i];
+        // post-force kick: (0.5 dt) / m, the expression of k_vvPost and of langevinFactors
+        if (LANGEVIN && !UNI) f = langevinFactors(a.mass[i], dt, zeta, temperature);
+        const double dtfm = UNI ? uf.dtfm : (LANGEVIN ? f.dtfm : dtHalf / a.mass[i]);
+        vx += dtfm * fx;
+        vy += dtfm * fy;
+        vz += dtfm * fz;
+        // pre-force kick and drift of the next step
+        p = a.pos[i];
+        vx += dtfm * fx;
+        vy += dtfm * fy;
+        vz += dtfm * fz;
+        ox = p.x;
+        oy = p.y;
+        oz = p.z;
+        if (LANGEVIN)
+        {
+            p.x += dtHalf * vx;
+            p.y += dtHalf * vy;
+            p.z += dtHalf * vz;
+        }
+        else
+        {
+            p.x += dt * vx;
+            p.y += dt * vy;
+            p.z += dt * vz;
+        }
+    }
+    if (LANGEVIN && __ballot(live) != 0)
+    {
+        double xi[3];
+        waveNormals3(seed, uint64_t(i), call, live, draws[LANGEVIN ? threadIdx.x / WAVE : 0], xi);
+        if (live)
+        {
+            vx *= f.damping;
+            vy *= f.damping;
+            vz *= f.damping;
+            vx += f.sigma * xi[0];
+            vy += f.sigma * xi[1];
+            vz += f.sigma * xi[2];
+            p.x += dtHalf * vx;
+            p.y += dtHalf * vy;
+            p.z += dtHalf * vz;
+        }
+    }
+    if (live)
+    {
+        a.pos[i] = p;
+        a.vel[i] = vx;
+        a.vel[a.cap + i] = vy;
+        a.vel[2 * a.cap + i] = vz;
+        const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
+        d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
+    }
+    const double mx = blockMax(d2, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = mx;
+}
+
 // analysis::getKineticEnergy (KineticEnergy.hpp:26-39): per-workgroup partials
 __global__ __launch_bounds__(BLOCK) void k_kinetic(AtomsView a, int64_t n, double* partials)
 {
@@ -233,6 +314,38 @@ double* dispHost(mrmd_atoms* a, const double* maxDisp)
     return a->ctx->pinned;
 }
 }  // namespace
+
+// the fused post/pre step of time loops (k_postPre); synchronises for the displacement
+int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
+                           uint64_t call, double* maxDisp)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && maxDisp, "invalid arguments");
+    mrmd_ctx* ctx = a->ctx;
+    const unsigned grid = gridFor(a->nlocal);
+    double* part = dispPartials(a, grid);
+    double* ms = dispScalar(a);
+    double* host = dispHost(a, maxDisp);
+    if (a->nlocal == 0) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
+    if (a->nlocal > 0)
+    {
+        KTimer t(ctx, "post_pre");
+        const bool uni = a->massUniform && a->massValue > 0.0;
+        const LangevinFactors uf = uni ? langevinFactors(a->massValue, dt, zeta, temperature) : LangevinFactors{};
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, zeta,
+                               temperature, seed, call, uf, part);
+        };
+        if (langevin)
+            uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);
+        else
+            uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
+    }
+    checkLaunch();
+    finishDisp(a, maxDisp);
+    MRMD_API_END
+}
 
 extern "C" {
 

@@ -5,7 +5,9 @@
 // with the Langevin pre-force integrator of examples/01_LennardJones_NVT.cpp:139-214 and the
 // LinkedCellList + Atoms::permute of tests/NVT/NVT.cpp:138-144 on every rebuild when a cell
 // list is given.  Each step makes the calls a caller of the actions makes, in the same
-// order, so its results are those of the same calls made one by one.
+// order, except that a step's post-force integration and the next step's pre-force
+// integration, adjacent in the loop, run as one pass over the atoms with the same
+// arithmetic; the results are those of the same calls made one by one, bit for bit.
 #include <cfloat>
 
 #include "containers.hpp"
@@ -35,16 +37,20 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         const int rc_ = (expr);               \
         if (rc_ != MRMD_OK) return rc_;       \
     } while (0)
-    for (int64_t k = 0; k < nsteps; ++k)
+    // a step's post-force integration and the next step's pre-force integration run as one
+    // fused pass (fusedPostPre, bit-identical to the two calls); the first pre-force and the
+    // last post-force integration of the call run alone
+    double d = 0.0;
+    if (nsteps > 0)
     {
-        double d = 0.0;
         if (s->langevin)
-        {
             MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
-            ++s->call;
-        }
         else
             MD_CALL(mrmd_vv_pre(a, s->dt, &d));
+    }
+    for (int64_t k = 0; k < nsteps; ++k)
+    {
+        if (s->langevin) ++s->call;
         s->max_disp += d;
         if (s->max_disp >= s->skin * 0.5)
         {
@@ -65,7 +71,10 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
         MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
         MD_CALL(mrmd_halo_fold_forces(gl, a));
-        MD_CALL(mrmd_vv_post(a, s->dt));
+        if (k + 1 < nsteps)
+            MD_CALL(fusedPostPre(a, s->dt, s->langevin != 0, s->zeta, s->temperature, s->seed, s->call, &d));
+        else
+            MD_CALL(mrmd_vv_post(a, s->dt));
         ++s->steps;
     }
 #undef MD_CALL
@@ -80,7 +89,8 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
 // molecule forces; LJ_IdealGas::run; ContributeMoleculeForceToAtoms; ghost fold;
 // ThermodynamicForce sample, update_if every tf_every steps, apply; post-force integration
 // (action/LJ_IdealGas.cpp:227-260, action/ContributeMoleculeForceToAtoms.cpp:23-48,
-// action/ThermodynamicForce.hpp:98-152).  The energy is read back on the last step only.
+// action/ThermodynamicForce.hpp:98-152).  The energy is read back on the last step only;
+// post- and next pre-force integration run fused as in mrmd_md_run.
 int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, mrmd_verlet_list* vl,
                        mrmd_lj_idealgas* ig, mrmd_thermo_force* tf, const mrmd_slab* slab, const mrmd_box* box,
                        mrmd_adress_loop* s, int64_t nsteps)
@@ -102,10 +112,11 @@ int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, m
         const int rc_ = (expr);               \
         if (rc_ != MRMD_OK) return rc_;       \
     } while (0)
+    // post-force integration fused with the next step's pre-force integration, as above
+    double d = 0.0;
+    if (nsteps > 0) MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
     for (int64_t k = 0; k < nsteps; ++k)
     {
-        double d = 0.0;
-        MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
         ++s->call;
         s->max_disp += d;
         if (s->max_disp >= s->skin * 0.5)
@@ -131,7 +142,10 @@ int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, m
         if (s->tf_every && s->steps % s->tf_every == 0)
             MD_CALL(mrmd_tf_update(tf, s->tf_sigma, s->tf_range, &s->tf_pred));
         MD_CALL(mrmd_tf_apply(tf, a, nullptr));
-        MD_CALL(mrmd_vv_post(a, s->dt));
+        if (k + 1 < nsteps)
+            MD_CALL(fusedPostPre(a, s->dt, true, s->zeta, s->temperature, s->seed, s->call, &d));
+        else
+            MD_CALL(mrmd_vv_post(a, s->dt));
     }
 #undef MD_CALL
     MRMD_API_END
