@@ -115,7 +115,12 @@ __device__ inline double coord(const double4& p, int axis)
 
 // a time step's post-force integration fused with the next step's pre-force integration
 // (velocity Verlet, or Langevin without a predicate), bit-identical to the two calls;
-// *maxDisp = sqrt(max |dx|^2) of the pre-force part (synchronises)
+// *maxDisp = sqrt(max |dx|^2) of the pre-force part (synchronises); sortedPos (if not
+// null) receives each atom's new position at its sorted slot slotOf[i]
 int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
-                 uint64_t call, double* maxDisp);
+                 uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr);
+// UpdateGhostAtoms::updateOnlyPos that also writes each ghost's new position to
+// sortedPos[slotOf[ghost]] (the slot-order copy of a Verlet list)
+int ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,
+                      const int32_t* slotOf);
 }  // namespace mrmd_hip

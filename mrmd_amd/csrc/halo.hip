@@ -179,8 +179,10 @@ __global__ void k_ghostScatter(AtomsView a, int32_t* corr, int64_t total, int ax
     }
 }
 
-// UpdateGhostAtoms.cpp:31-68
-__global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, int64_t end, SubD s)
+// UpdateGhostAtoms.cpp:31-68; sortedPos (if not null): the new position is also written to
+// the record's sorted slot slotOf[i] of a Verlet list (the time loop's gather-free path)
+__global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, int64_t end, SubD s,
+                              double4* sortedPos, const int32_t* slotOf)
 {
     const int64_t i = begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= end) return;
@@ -197,6 +199,7 @@ __global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, 
     p.y = x[1];
     p.z = x[2];
     pos[i] = p;
+    if (sortedPos) sortedPos[slotOf[i]] = p;
 }
 
 // AccumulateForce.cpp:25-47
@@ -484,6 +487,23 @@ void checkBox(const mrmd_box* box)
 }
 }  // namespace
 
+// UpdateGhostAtoms::updateOnlyPos with each new ghost position also written to its sorted
+// slot of a Verlet list (time loops that keep the list's slot-order copy current)
+int mrmd_hip::ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,
+                                const int32_t* slotOf)
+{
+    MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
+    REQUIRE(gl && a && sortedPos && slotOf, "invalid arguments");
+    checkBox(box);
+    REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "correspondingRealAtom shorter than the atoms");
+    if (a->nghost == 0) return MRMD_OK;
+    KTimer t(gl->ctx, "halo_update");
+    hipLaunchKernelGGL(k_ghostUpdate, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
+                       gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box), sortedPos, slotOf);
+    checkLaunch();
+    MRMD_API_END
+}
+
 extern "C" {
 
 int mrmd_ghost_layer_create(mrmd_ctx* c, mrmd_ghost_layer** out)
@@ -550,7 +570,7 @@ int mrmd_halo_update(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
     if (a->nghost == 0) return MRMD_OK;
     KTimer t(gl->ctx, "halo_update");
     hipLaunchKernelGGL(k_ghostUpdate, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
-                       gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box));
+                       gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box), nullptr, nullptr);
     checkLaunch();
     MRMD_API_END
 }

@@ -151,10 +151,12 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
 // integration, for time loops that run both back to back: per atom the two kernels'
 // arithmetic in the same order (the force is the same between them), so the results are
 // bit-identical, with one pass over the arrays instead of two.  LANGEVIN: the pre-force
-// part is k_langevinPre<false, UNI>, otherwise k_vvPre.
+// part is k_langevinPre<false, UNI>, otherwise k_vvPre.  sortedPos (if not null): each new
+// position also goes to the atom's sorted slot slotOf[i] of the Verlet list.
 template <bool LANGEVIN, bool UNI>
 __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
-                                                   uint64_t seed, uint64_t call, LangevinFactors uf, double* partials)
+                                                   uint64_t seed, uint64_t call, LangevinFactors uf, double* partials,
+                                                   double4* sortedPos, const int32_t* slotOf)
 {
     __shared__ double red[BLOCK / WAVE];
     __shared__ WaveDraws draws[LANGEVIN ? BLOCK / WAVE : 1];
@@ -218,6 +220,7 @@ __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, doubl
     if (live)
     {
         a.pos[i] = p;
+        if (sortedPos) sortedPos[slotOf[i]] = p;
         a.vel[i] = vx;
         a.vel[a.cap + i] = vy;
         a.vel[2 * a.cap + i] = vz;
@@ -317,7 +320,7 @@ double* dispHost(mrmd_atoms* a, const double* maxDisp)
 
 // the fused post/pre step of time loops (k_postPre); synchronises for the displacement
 int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
-                           uint64_t call, double* maxDisp)
+                           uint64_t call, double* maxDisp, double4* sortedPos, const int32_t* slotOf)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && maxDisp, "invalid arguments");
@@ -334,7 +337,7 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
         const LangevinFactors uf = uni ? langevinFactors(a->massValue, dt, zeta, temperature) : LangevinFactors{};
         auto launch = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, zeta,
-                               temperature, seed, call, uf, part);
+                               temperature, seed, call, uf, part, sortedPos, slotOf);
         };
         if (langevin)
             uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);

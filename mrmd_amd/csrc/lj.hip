@@ -613,6 +613,8 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
             "atom type %lld has no entry in the %lld-type table", (long long)at->maxType, (long long)lj->numTypes);
     // the reference loops over local atoms with the list rows (LennardJones.hpp:146)
     const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
+    const bool fresh = vl->sortedFresh;  // consumed by this call whatever path it takes
+    vl->sortedFresh = false;
     mrmd_ctx* ctx = lj->ctx;
     const bool full = lj->symmetric && predIsSymmetric(pred);
     const unsigned grid = full ? unsigned(std::max<int64_t>(vl->ntiles, 1)) : gridFor(rows);
@@ -625,7 +627,7 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
         if (full)
         {
             REQUIRE(vl->ntiles > 0, "the Verlet list has not been built");
-            vl->refreshSorted(at->pos.p);
+            if (!fresh) vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
                                vl->bin.sortedPos.p, vl->maxStaged};

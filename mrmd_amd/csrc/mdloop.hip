@@ -11,6 +11,7 @@
 #include <cfloat>
 
 #include "containers.hpp"
+#include "nlist.hpp"
 
 using namespace mrmd_hip;
 
@@ -48,10 +49,17 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         else
             MD_CALL(mrmd_vv_pre(a, s->dt, &d));
     }
+    // The force kernel reads positions in the list's sorted-slot order, normally gathered at
+    // the start of every force call.  Here the passes that move the atoms write that copy
+    // themselves: the fused post/pre pass stores each local atom's new position at its slot,
+    // the ghost update each ghost's, and a rebuild's binning gathers it; the force call then
+    // skips its gather (same values, so the same bits).
+    bool localsScattered = false;  // the last pre-force pass wrote every local atom's slot
     for (int64_t k = 0; k < nsteps; ++k)
     {
         if (s->langevin) ++s->call;
         s->max_disp += d;
+        bool fresh = false;
         if (s->max_disp >= s->skin * 0.5)
         {
             s->max_disp = 0.0;
@@ -65,14 +73,27 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
             MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, gridMin, gridMax, s->max_neigh));
             ++s->rebuilds;
+            fresh = vl->nrows == a->nlocal + a->nghost;  // the build binned (and gathered) every record
+        }
+        else if (localsScattered && vl->nrows == a->nlocal + a->nghost)
+        {
+            MD_CALL(ghostUpdateSorted(gl, a, box, vl->bin.sortedPos.p, vl->ensureSlotOf()));
+            fresh = true;
         }
         else
             MD_CALL(mrmd_halo_update(gl, a, box));
         MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
+        vl->sortedFresh = fresh;
         MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
         MD_CALL(mrmd_halo_fold_forces(gl, a));
+        localsScattered = false;
         if (k + 1 < nsteps)
-            MD_CALL(fusedPostPre(a, s->dt, s->langevin != 0, s->zeta, s->temperature, s->seed, s->call, &d));
+        {
+            const bool scatter = vl->ntiles > 0 && vl->nrows == a->nlocal + a->nghost;
+            MD_CALL(fusedPostPre(a, s->dt, s->langevin != 0, s->zeta, s->temperature, s->seed, s->call, &d,
+                                 scatter ? vl->bin.sortedPos.p : nullptr, scatter ? vl->ensureSlotOf() : nullptr));
+            localsScattered = scatter;
+        }
         else
             MD_CALL(mrmd_vv_post(a, s->dt));
         ++s->steps;

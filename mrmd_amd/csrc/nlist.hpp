@@ -224,8 +224,17 @@ struct mrmd_verlet_list
     const mrmd_molecules* monoOf = nullptr;  ///< the molecules, when all records were monatomic
     uint64_t monoVersion = 0;                ///< their layout version at that build
 
+    /// sorted slot of each binned record (inverse of bin.sorted), made on demand per build
+    mrmd_hip::DBuf<int32_t> slotOf;
+    bool slotOfValid = false;
+    /// set by a caller that has itself written the current positions of every binned record
+    /// into bin.sortedPos; the next force call then skips its gather and clears the flag
+    bool sortedFresh = false;
+
     /// copy the current positions into sorted-slot order (bin.sortedPos) for the tiled kernels
     void refreshSorted(const double4* pos);
+    /// slotOf, inverted from bin.sorted when the list was rebuilt since the last call
+    const int32_t* ensureSlotOf();
     /// materialise the global-index full rows from the tiled list (no-op when current)
     void ensureGlobal();
     /// materialise the half list from the flagged full rows (no-op when current)

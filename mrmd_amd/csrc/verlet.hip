@@ -267,6 +267,12 @@ __global__ void k_gatherSorted(const double4* pos, const int32_t* sorted, int64_
     if (s < n) out[s] = pos[sorted[s]];
 }
 
+__global__ void k_invertSorted(const int32_t* sorted, int64_t n, int32_t* slotOf)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < n) slotOf[sorted[s]] = int32_t(s);
+}
+
 __global__ void k_zeroI32v(int32_t* p, int64_t n)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -392,6 +398,8 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     vl->bin.build(ctx, pos, 0, nall, g, true, true);
     vl->globalValid = false;
     vl->halfValid = false;
+    vl->slotOfValid = false;
+    vl->sortedFresh = false;
     // tile shape: a x a columns (a <= 3; 1 when the stencil reaches two cells), height K
     // chosen to stage ~1650 records; shrink until every tile fits in LDS
     const int R = vl->range;
@@ -472,6 +480,20 @@ void mrmd_verlet_list::refreshSorted(const double4* pos)
         hipLaunchKernelGGL(k_gatherSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, pos, bin.sorted.p, nrows,
                            bin.sortedPos.p);
     checkLaunch();
+}
+
+const int32_t* mrmd_verlet_list::ensureSlotOf()
+{
+    if (!slotOfValid)
+    {
+        slotOf.ensure(ctx, std::max<int64_t>(nrows, 1));
+        if (nrows > 0)
+            hipLaunchKernelGGL(k_invertSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, bin.sorted.p, nrows,
+                               slotOf.p);
+        checkLaunch();
+        slotOfValid = true;
+    }
+    return slotOf.p;
 }
 
 void mrmd_verlet_list::ensureGlobal()

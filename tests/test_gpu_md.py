@@ -81,17 +81,24 @@ def test_langevin_permute_loop_matches_early_steps(M, ctx):
     np.testing.assert_allclose(sysg.local(M.POS), a.get(O.POS, a.nlocal), rtol=0, atol=1e-10)
 
 
-@pytest.mark.parametrize("langevin,permute", [(False, False), (True, True)], ids=["nve", "nvt-permute"])
-def test_native_loop_equals_step_calls(M, ctx, langevin, permute):
-    """mrmd_md_run (the example loop in C++) against the same calls made step by step from
-    Python: bit-identical positions, velocities and forces, same rebuilds and noise counter,
-    across two native calls that continue one run"""
+@pytest.mark.parametrize("langevin,permute,masses", [(False, False, False), (True, True, False), (True, True, True),
+                                                    (False, True, True)],
+                         ids=["nve", "nvt-permute", "nvt-permute-masses", "nve-permute-masses"])
+def test_native_loop_equals_step_calls(M, ctx, langevin, permute, masses):
+    """mrmd_md_run (the example loop in C++: fused post/pre passes, slot-order positions
+    written by the integrator and the ghost update instead of a gather) against the same
+    calls made step by step from Python: bit-identical positions, velocities and forces,
+    same rebuilds and noise counter, across two native calls that continue one run"""
     from mrmd_amd.drivers import LJSystem
     pos, L = fcc(8, 0.8442)
     vel = maxwell_boltzmann(pos.shape[0], 1.2, 17)
     kw = dict(langevin=langevin, gamma=1.0, temperature=1.2, seed=5, permute=permute)
     a = LJSystem(ctx, pos, vel, L, **kw)
     b = LJSystem(ctx, pos, vel, L, **kw)
+    if masses:
+        m = 0.8 + 0.4 * np.random.default_rng(3).random(pos.shape[0])
+        a.atoms.upload(M.MASS, m)
+        b.atoms.upload(M.MASS, m)
     for _ in range(60):
         a.step(0.005)
     b.run(25, 0.005)
