@@ -214,6 +214,22 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
 // =========================================================================================
 namespace
 {
+// every field of records [lo, hi) copied unchanged (the records a permutation leaves alone)
+__global__ void k_copyRecords(AtomsView src, AtomsView dst, int64_t lo, int64_t hi)
+{
+    const int64_t i = lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    dst.pos[i] = src.pos[i];
+    for (int c = 0; c < 3; ++c)
+    {
+        dst.vel[c * dst.cap + i] = src.vel[c * src.cap + i];
+        dst.force[c * dst.cap + i] = src.force[c * src.cap + i];
+    }
+    dst.mass[i] = src.mass[i];
+    dst.charge[i] = src.charge[i];
+    dst.relmass[i] = src.relmass[i];
+}
+
 // Cabana::permute: new[begin + k] = old[perm[k]] for every field
 __global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* perm, int64_t n, int64_t begin)
 {
@@ -323,21 +339,8 @@ int mrmd_atoms_permute(mrmd_atoms* a, const mrmd_cell_list* cl)
     auto copyRange = [&](int64_t lo, int64_t hi)
     {
         if (hi <= lo) return;
-        HIPCHECK(hipMemcpyAsync(a->sparePos.p + lo, a->pos.p + lo, size_t(hi - lo) * 32, hipMemcpyDeviceToDevice,
-                                ctx->stream));
-        for (int c = 0; c < 3; ++c)
-        {
-            HIPCHECK(hipMemcpyAsync(a->spareVel.p + c * a->cap + lo, a->vel.p + c * a->cap + lo, size_t(hi - lo) * 8,
-                                    hipMemcpyDeviceToDevice, ctx->stream));
-            HIPCHECK(hipMemcpyAsync(a->spareForce.p + c * a->cap + lo, a->force.p + c * a->cap + lo,
-                                    size_t(hi - lo) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        }
-        HIPCHECK(hipMemcpyAsync(a->spareMass.p + lo, a->mass.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice,
-                                ctx->stream));
-        HIPCHECK(hipMemcpyAsync(a->spareCharge.p + lo, a->charge.p + lo, size_t(hi - lo) * 8, hipMemcpyDeviceToDevice,
-                                ctx->stream));
-        HIPCHECK(hipMemcpyAsync(a->spareRelmass.p + lo, a->relmass.p + lo, size_t(hi - lo) * 8,
-                                hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_copyRecords, dim3(gridFor(hi - lo)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                           a->spareView(), lo, hi);
     };
     copyRange(0, b);
     copyRange(e, s);
