@@ -500,7 +500,10 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
     extern __shared__ double4 st[];
     __shared__ TileB ti;
     __shared__ int cstart[9 * (B_SPAN + 1)];
-    const unsigned t = xcdBlock();
+    // plain block order: when only some replicas rebuild, their tiles are contiguous, and the
+    // hardware's round-robin of consecutive workgroups over the XCDs spreads them over all
+    // eight (grouping consecutive tiles on one XCD would put a whole replica on one)
+    const unsigned t = blockIdx.x;
     const TileG G = b.tg[t];
     if (!b.flag[G.rep]) return;
     if (threadIdx.x == 0) ti = b.tiles[t];
@@ -623,7 +626,7 @@ __device__ double bForceRows(const BS& b, const TileB& ti, const RepD& R, const 
         double fx = 0.0, fy = 0.0, fz = 0.0, ew = 0.0, e0 = 0.0;
         const uint4* chunk = reinterpret_cast<const uint4*>(b.nbr) + s;
         const int nch = (c + CHUNK - 1) / CHUNK;
-        uint4 cur = nch > 0 ? chunk[0] : uint4{0, 0, 0, 0};
+        uint4 cur = chunk[0];  // the first chunk column exists for every slot (read even when c = 0)
         for (int ch = 0; ch < nch; ++ch)
         {
             uint4 nxt = cur;
