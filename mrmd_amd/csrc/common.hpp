@@ -71,7 +71,8 @@ struct mrmd_ctx
     std::string profileOnly;  ///< non-empty: only timers of this name record
     std::map<std::string, mrmd_hip::KStat> stats;
     std::vector<hipEvent_t> eventPool;
-    double* pinned = nullptr;  // small pinned host staging area (64 doubles)
+    double* pinned = nullptr;  // small pinned host staging area (64 doubles, coherent)
+    double dispSeq = 0.0;      // sequence number of the last displacement written to pinned[1]
     int64_t bytes = 0;         // device bytes held by objects of this context
     void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};  // per-purpose device scratch
     size_t scratchBytes[4] = {0, 0, 0, 0};

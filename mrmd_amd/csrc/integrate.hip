@@ -3,6 +3,8 @@
 // displacement is reduced with wave shuffles to one value per workgroup and then by one
 // small ordered pass (no atomics on a single address).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 
 #include "containers.hpp"
 #include "detmath.hpp"
@@ -267,9 +269,11 @@ __global__ void k_scaleVel(AtomsView a, int64_t n, double beta)
 
 // per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
 // eight independent loads per lane in flight)
-// host (if not null) is pinned host memory that receives the result too, written by the
-// kernel itself (no copy launch after it)
-__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, double* host)
+// host (if not null) is coherent pinned host memory that receives the result in host[0]
+// and then, after a system-scope fence, the sequence number seq in host[1]: the host can
+// take the value as soon as the number appears, without waiting for the stream
+__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, double* host,
+                                                       double seq)
 {
     __shared__ double red[BLOCK / WAVE];
     double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -287,7 +291,9 @@ __global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, i
         *out = r;
         if (host)
         {
-            *host = r;
+            host[0] = r;
+            __threadfence_system();
+            host[1] = seq;
             __threadfence_system();
         }
     }
@@ -300,21 +306,39 @@ double* dispPartials(mrmd_atoms* a, unsigned grid)
     return static_cast<double*>(ctxScratch(a->ctx, 3, (size_t(grid) + 1) * sizeof(double))) + 1;
 }
 
-// the reduction kernel has written the maximum to ctx->pinned[0] (or the caller stored 0.0
-// there when no atom moved); the synchronisation makes it visible here
+// The reduction kernel writes the maximum to ctx->pinned[0] and then the call's sequence
+// number to pinned[1] (dispHost stores both itself when no kernel runs).  The host waits
+// for the number by polling, which returns as soon as it lands, and falls back to a stream
+// synchronisation after 20 ms (which also reports any launch error).
 void finishDisp(mrmd_atoms* a, double* maxDisp)
 {
     if (!maxDisp) return;
-    sync(a->ctx);
-    *maxDisp = sqrt(a->ctx->pinned[0]);
+    volatile const double* h = a->ctx->pinned;
+    const double want = a->ctx->dispSeq;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (h[1] != want)
+    {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+        {
+            sync(a->ctx);
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    if (h[1] != want) fail(MRMD_ERR_STATE, "the displacement reduction did not report");
+    *maxDisp = sqrt(h[0]);
 }
 
-// target of the reduction's host copy: the context's pinned slot when the caller wants the value
+// target of the reduction's host copy (the context's pinned slots) when the caller wants
+// the value; nlocal = 0: no kernel writes them, so they are set here
 double* dispHost(mrmd_atoms* a, const double* maxDisp)
 {
     if (!maxDisp) return nullptr;
-    a->ctx->pinned[0] = 0.0;
-    return a->ctx->pinned;
+    mrmd_ctx* c = a->ctx;
+    c->dispSeq += 1.0;
+    c->pinned[0] = 0.0;
+    c->pinned[1] = a->nlocal > 0 ? -1.0 : c->dispSeq;
+    return c->pinned;
 }
 }  // namespace
 
@@ -343,7 +367,8 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
             uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);
         else
             uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+                           ctx->dispSeq);
     }
     checkLaunch();
     finishDisp(a, maxDisp);
@@ -365,7 +390,8 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     {
         KTimer t(ctx, "vv_pre");
         hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+                           ctx->dispSeq);
     }
     else
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
@@ -412,7 +438,8 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
             uni ? launch(k_langevinPre<true, true>) : launch(k_langevinPre<true, false>);
         else
             uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host);
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+                           ctx->dispSeq);
     }
     checkLaunch();
     finishDisp(a, maxDisp);
