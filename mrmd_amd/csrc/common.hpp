@@ -72,7 +72,7 @@ struct mrmd_ctx
     std::map<std::string, mrmd_hip::KStat> stats;
     std::vector<hipEvent_t> eventPool;
     double* pinned = nullptr;  // small pinned host staging area (64 doubles, coherent)
-    double dispSeq = 0.0;      // sequence number of the last displacement written to pinned[1]
+    double hostSeq = 0.0;      // last sequence number a kernel was asked to post to pinned
     int64_t bytes = 0;         // device bytes held by objects of this context
     void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};  // per-purpose device scratch
     size_t scratchBytes[4] = {0, 0, 0, 0};
@@ -253,6 +253,19 @@ inline void flushStats(mrmd_ctx* ctx)
 }
 
 inline void sync(mrmd_ctx* ctx) { HIPCHECK(hipStreamSynchronize(ctx->stream)); }
+
+// Kernels post small results to the context's coherent pinned buffer followed by a sequence
+// number in a slot of their own; the host polls that slot (waitHostSeq), which returns as
+// soon as the number lands, and synchronises the stream only as a fallback after 20 ms.
+// Slots (doubles): 0 the displacement maximum, 1 its sequence number, 2 the sequence number
+// of readbackI32, 8.. its integers.
+constexpr int DISP_SEQ_SLOT = 1;
+constexpr int READBACK_SEQ_SLOT = 2;
+constexpr int READBACK_INT_SLOT = 8;
+constexpr int READBACK_INTS = 16;
+void waitHostSeq(mrmd_ctx* ctx, int slot, double want);
+// n (<= READBACK_INTS) int32 device values to the host through the pinned buffer
+void readbackI32(mrmd_ctx* ctx, const int32_t* dev, int n, int32_t* out);
 
 inline void checkLaunch() { HIPCHECK(hipGetLastError()); }
 

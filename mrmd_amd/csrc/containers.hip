@@ -2,6 +2,9 @@
 #include <algorithm>
 #include <cstring>
 
+#include <atomic>
+#include <chrono>
+
 #include "containers.hpp"
 
 using namespace mrmd_hip;
@@ -87,6 +90,52 @@ void scanExclusiveI64(mrmd_ctx* ctx, const int32_t* in, int64_t* out, int64_t n,
 void scanExclusiveI32(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int32_t* totalDev)
 {
     scanImpl<int32_t>(ctx, in, out, n, totalDev);
+}
+
+void waitHostSeq(mrmd_ctx* ctx, int slot, double want)
+{
+    volatile const double* h = ctx->pinned;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (h[slot] != want)
+    {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+        {
+            sync(ctx);
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    if (h[slot] != want) fail(MRMD_ERR_STATE, "a device result did not reach the host");
+}
+}  // namespace mrmd_hip
+
+namespace
+{
+// copies n int32 values into the coherent pinned buffer, then (after a system-scope fence)
+// the sequence number
+__global__ void k_postI32(const int32_t* src, int n, int32_t* hostInts, double* hostSeq, double seq)
+{
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < n; ++k) hostInts[k] = src[k];
+    __threadfence_system();
+    *hostSeq = seq;
+    __threadfence_system();
+}
+}  // namespace
+
+namespace mrmd_hip
+{
+void readbackI32(mrmd_ctx* ctx, const int32_t* dev, int n, int32_t* out)
+{
+    if (n < 0 || n > READBACK_INTS) fail(MRMD_ERR_ARG, "readback of %d values (at most %d)", n, READBACK_INTS);
+    ctx->hostSeq += 1.0;
+    ctx->pinned[READBACK_SEQ_SLOT] = -1.0;
+    int32_t* ints = reinterpret_cast<int32_t*>(ctx->pinned + READBACK_INT_SLOT);
+    hipLaunchKernelGGL(k_postI32, dim3(1), dim3(64), 0, ctx->stream, dev, n, ints, ctx->pinned + READBACK_SEQ_SLOT,
+                       ctx->hostSeq);
+    checkLaunch();
+    waitHostSeq(ctx, READBACK_SEQ_SLOT, ctx->hostSeq);
+    for (int k = 0; k < n; ++k) out[k] = ints[k];
 }
 }  // namespace mrmd_hip
 

@@ -418,10 +418,8 @@ void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int ax
     checkLaunch();
     scanExclusiveI32(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, tot);
     scanExclusiveI32(ctx, gl->tileCnt.p + tiles, gl->tileOff.p + tiles, tiles, tot + 1);
-    // pinned staging: a pageable 8-byte readback costs far more than the copy itself
-    int32_t* h = reinterpret_cast<int32_t*>(ctx->pinned);
-    HIPCHECK(hipMemcpyAsync(h, tot, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
+    int32_t h[2];
+    readbackI32(ctx, tot, 2, h);
     const int64_t nP = h[0], nN = h[1];
     const int64_t newTotal = total + nP + nN;
     // atoms.resize(...) keeps existing records (GhostExchange.cpp:111-112)
@@ -453,9 +451,8 @@ void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const
                        s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, tiles);
     checkLaunch();
     for (int k = 0; k < 4; ++k) scanExclusiveI32(ctx, gl->tileCnt.p + k * tiles, gl->tileOff.p + k * tiles, tiles, tot + k);
-    int32_t* h = reinterpret_cast<int32_t*>(ctx->pinned);
-    HIPCHECK(hipMemcpyAsync(h, tot, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
+    int32_t h[4];
+    readbackI32(ctx, tot, 4, h);
     const int64_t newM = totM + h[0] + h[2], newA = totA + h[1] + h[3];
     m->ensureCap(newM);
     m->size = newM;

@@ -307,26 +307,13 @@ double* dispPartials(mrmd_atoms* a, unsigned grid)
 }
 
 // The reduction kernel writes the maximum to ctx->pinned[0] and then the call's sequence
-// number to pinned[1] (dispHost stores both itself when no kernel runs).  The host waits
-// for the number by polling, which returns as soon as it lands, and falls back to a stream
-// synchronisation after 20 ms (which also reports any launch error).
+// number to pinned[DISP_SEQ_SLOT] (dispHost stores both itself when no kernel runs); the
+// host polls for the number (waitHostSeq).
 void finishDisp(mrmd_atoms* a, double* maxDisp)
 {
     if (!maxDisp) return;
-    volatile const double* h = a->ctx->pinned;
-    const double want = a->ctx->dispSeq;
-    const auto t0 = std::chrono::steady_clock::now();
-    while (h[1] != want)
-    {
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
-        {
-            sync(a->ctx);
-            break;
-        }
-    }
-    std::atomic_thread_fence(std::memory_order_acquire);
-    if (h[1] != want) fail(MRMD_ERR_STATE, "the displacement reduction did not report");
-    *maxDisp = sqrt(h[0]);
+    waitHostSeq(a->ctx, DISP_SEQ_SLOT, a->ctx->hostSeq);
+    *maxDisp = sqrt(static_cast<volatile const double*>(a->ctx->pinned)[0]);
 }
 
 // target of the reduction's host copy (the context's pinned slots) when the caller wants
@@ -335,9 +322,9 @@ double* dispHost(mrmd_atoms* a, const double* maxDisp)
 {
     if (!maxDisp) return nullptr;
     mrmd_ctx* c = a->ctx;
-    c->dispSeq += 1.0;
+    c->hostSeq += 1.0;
     c->pinned[0] = 0.0;
-    c->pinned[1] = a->nlocal > 0 ? -1.0 : c->dispSeq;
+    c->pinned[DISP_SEQ_SLOT] = a->nlocal > 0 ? -1.0 : c->hostSeq;
     return c->pinned;
 }
 }  // namespace
@@ -368,7 +355,7 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
         else
             uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->dispSeq);
+                           ctx->hostSeq);
     }
     checkLaunch();
     finishDisp(a, maxDisp);
@@ -391,7 +378,7 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
         KTimer t(ctx, "vv_pre");
         hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->dispSeq);
+                           ctx->hostSeq);
     }
     else
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
@@ -439,7 +426,7 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
         else
             uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->dispSeq);
+                           ctx->hostSeq);
     }
     checkLaunch();
     finishDisp(a, maxDisp);

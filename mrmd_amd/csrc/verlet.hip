@@ -361,9 +361,8 @@ void setupTiles(mrmd_verlet_list* vl, int K)
     hipLaunchKernelGGL(k_tileSetup, dim3(gridFor(vl->ntiles)), dim3(BLOCK), 0, ctx->stream, geom(vl), vl->ntiles,
                        vl->bin.offsets.p, vl->bin.counts.p, vl->tiles.p, dmax);
     checkLaunch();
-    int32_t* h = reinterpret_cast<int32_t*>(ctx->pinned);
-    HIPCHECK(hipMemcpyAsync(h, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
+    int32_t h[1];
+    readbackI32(ctx, dmax, 1, h);
     vl->maxStaged = h[0];
 }
 
@@ -459,9 +458,8 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                                    vl->capFull, vl->cnt16.p, dmax, stagedCap);
         }
         checkLaunch();
-        int32_t* hp = reinterpret_cast<int32_t*>(ctx->pinned);
-        HIPCHECK(hipMemcpyAsync(hp, dmax, 8, hipMemcpyDeviceToHost, ctx->stream));
-        sync(ctx);
+        int32_t hp[2];
+        readbackI32(ctx, dmax, 2, hp);
         const int32_t hmax = hp[0];
         mono = hp[1] == 0;
         vl->maxFull = hmax;
