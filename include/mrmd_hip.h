@@ -330,7 +330,8 @@ int mrmd_berendsen_apply(mrmd_atoms* atoms, double current_temperature, double t
  * ghosts and list on a rebuild, ghost update otherwise) with the Langevin pre-force
  * integrator of examples/01_LennardJones_NVT.cpp:139-214 (langevin = 1) and, when `cells` is
  * not NULL, LinkedCellList + Atoms::permute on every rebuild (tests/NVT/NVT.cpp:138-144).
- * The box's corners give the list grid (min - r_n, max + r_n) and the cell list (delta r_n).
+ * grid_min / grid_max (with cell_ratio) give the Verlet list's binning grid, the box's
+ * corners the cell list (delta r_n).
  * The state fields are read and updated, so consecutive calls continue one run; the
  * results equal those of the same calls made one by one, bit for bit (a step's post-force
  * and the next step's pre-force integration run as one pass with the same arithmetic). */
@@ -339,6 +340,8 @@ typedef struct mrmd_md_loop {
     double neighbor_radius; /* r_n = rc + skin of the list builds */
     double skin;            /* rebuild when the accumulated displacement reaches skin / 2 */
     double cell_ratio;
+    double grid_min[3];     /* the list's binning grid (HalfVerletList::build gridMin/gridMax) */
+    double grid_max[3];
     int64_t max_neigh;
     int32_t langevin;       /* 0: VelocityVerlet, 1: VelocityVerletLangevinThermostat */
     int32_t reserved;

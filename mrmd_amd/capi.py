@@ -49,7 +49,7 @@ class Spherical(C.Structure):
 
 class MdLoop(C.Structure):
     _fields_ = [("dt", C.c_double), ("neighbor_radius", C.c_double), ("skin", C.c_double), ("cell_ratio", C.c_double),
-                ("max_neigh", C.c_int64), ("langevin", C.c_int32), ("reserved", C.c_int32), ("zeta", C.c_double),
+                ("grid_min", C.c_double * 3), ("grid_max", C.c_double * 3), ("max_neigh", C.c_int64), ("langevin", C.c_int32), ("reserved", C.c_int32), ("zeta", C.c_double),
                 ("temperature", C.c_double), ("seed", C.c_uint64), ("call", C.c_uint64), ("max_disp", C.c_double),
                 ("rebuilds", C.c_int64), ("steps", C.c_int64)]
 

@@ -25,13 +25,8 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     REQUIRE(s->dt > 0.0 && s->neighbor_radius > 0.0 && s->skin >= 0.0 && s->cell_ratio > 0.0,
             "dt, neighbor_radius and cell_ratio must be positive, skin non-negative");
     const double rn = s->neighbor_radius;
-    double gridMin[3], gridMax[3], delta[3];
-    for (int d = 0; d < 3; ++d)
-    {
-        gridMin[d] = box->min[d] - rn;
-        gridMax[d] = box->max[d] + rn;
-        delta[d] = rn;
-    }
+    double delta[3];
+    for (int d = 0; d < 3; ++d) delta[d] = rn;
 #define MD_CALL(expr)                         \
     do                                        \
     {                                         \
@@ -71,7 +66,8 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
             }
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
-            MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, gridMin, gridMax, s->max_neigh));
+            MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, s->grid_min, s->grid_max,
+                                      s->max_neigh));
             ++s->rebuilds;
             fresh = vl->nrows == a->nlocal + a->nghost;  // the build binned (and gathered) every record
         }

@@ -30,6 +30,26 @@ def _edges(L):
     return e
 
 
+def list_grid(L, rn, cell_ratio=1.0):
+    """Binning grid of the Verlet list: the reference's ghost corners [-r_n, L + r_n] with
+    cells of cell_ratio r_n, except that for cell_ratio 1 and a box whose three edges give
+    the cell list the same cell edge c = L / floor(L / r_n), the grid is [-c, L + c] in
+    cells of c: the cell list's cells plus one layer for the ghosts, so that the list's
+    slot order follows the order in which Atoms::permute leaves the atoms.  Any grid of
+    cells no smaller than r_n that covers the ghosts gives the same neighbour sets.
+    Returns (grid_min, grid_max, cell_ratio)."""
+    L = np.asarray(L, dtype=np.float64)
+    if cell_ratio == 1.0:
+        nc = np.floor(L / rn)
+        c = L / nc
+        if np.all(nc >= 1) and np.all(c == c[0]):
+            delta = c[0] * (1.0 - 1e-9)  # floor((L + 2c) / delta) is then nc + 2 beyond doubt
+            lo, hi = -c, L + c
+            if delta >= rn and np.all(np.floor((hi - lo) * (1.0 / delta)) == nc + 2):
+                return lo, hi, delta / rn
+    return np.full(3, -rn), L + rn, cell_ratio
+
+
 class LJSystem:
     def __init__(self, ctx, pos, vel, L, rc=2.5, skin=0.3, sigma=1.0, epsilon=1.0, cap=0.0, shifted=True,
                  cell_ratio=1.0, max_neigh=0, langevin=False, gamma=1.0, temperature=1.0, seed=1234,
@@ -44,6 +64,7 @@ class LJSystem:
         self.vl = HalfVerletList(ctx)
         self.lj = LennardJones(ctx, rc, sigma, epsilon, cap, num_types, shifted, type_stride)
         self.cell_ratio, self.max_neigh = cell_ratio, max_neigh
+        self.grid_min, self.grid_max, self.list_ratio = list_grid(self.L, self.rn, cell_ratio)
         self.langevin = VelocityVerletLangevinThermostat(gamma, temperature, seed) if langevin else None
         self.permute = permute
         self.cells = None
@@ -61,8 +82,7 @@ class LJSystem:
                 self.cells.build(a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
             a.permute(self.cells)
         self.ghosts.create_ghost_atoms(a, self.box)
-        self.vl.build(a, 0, a.nlocal, self.rn, self.cell_ratio, np.full(3, -self.rn), self.L + self.rn,
-                      self.max_neigh)
+        self.vl.build(a, 0, a.nlocal, self.rn, self.list_ratio, self.grid_min, self.grid_max, self.max_neigh)
         self.rebuilds += 1
 
     def step(self, dt):
@@ -98,7 +118,8 @@ class LJSystem:
         if self.permute and self.cells is None:
             self.cells = LinkedCellList(self.ctx, a, 0, a.nlocal, [self.rn] * 3, [0, 0, 0], self.L)
         lang = self.langevin
-        loop = MdLoop(dt=dt, neighbor_radius=self.rn, skin=self.skin, cell_ratio=self.cell_ratio,
+        loop = MdLoop(dt=dt, neighbor_radius=self.rn, skin=self.skin, cell_ratio=self.list_ratio,
+                      grid_min=(C.c_double * 3)(*self.grid_min), grid_max=(C.c_double * 3)(*self.grid_max),
                       max_neigh=int(self.max_neigh), langevin=1 if lang is not None else 0,
                       zeta=lang.zeta if lang else 0.0, temperature=lang.temperature if lang else 0.0,
                       seed=lang.seed if lang else 0, call=lang.call if lang else 0, max_disp=self.max_disp,
