@@ -100,7 +100,7 @@ struct TiledList
 template <bool PRED, bool ONE, bool CAP>
 __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, int ti0, const double4& pi,
                                const void* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
-                               double& w)
+                               double& w, double& n)
 {
 #pragma clang fp contract(fast)
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -135,6 +135,30 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
         e += en;
         w += ff * d2[u];
     };
+    if constexpr (ONE && !CAP)
+    {
+        // single-type table without capping: the reciprocal is masked (0 outside rc), and
+        // instead of each pair's energy and virial the lane keeps e = sum f6, w = sum g with
+        // g = f6 (ff1 f6 - ff2) = ff d^2, and n = pairs inside rc; the kernel turns them into
+        // energy ef1 sum f6^2 - ef2 sum f6 - shift n, sum f6^2 = (sum g + ff2 sum f6) / ff1
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u)
+        {
+            bool in = d2[u] <= a.rcSqr;
+            if (PRED) in = in && ty[u] >= 0.0;
+            const double f2 = in ? frac2[u] : 0.0;
+            const double f6 = f2 * f2 * f2;
+            const double g = f6 * (p1.ff1 * f6 - p1.ff2);
+            const double ff = g * f2;
+            fx += dx[u] * ff;
+            fy += dy[u] * ff;
+            fz += dz[u] * ff;
+            e += f6;
+            w += g;
+            n += in ? 1.0 : 0.0;
+        }
+        return;
+    }
     if constexpr (!CAP || !ONE)
     {
         // multi-type tables keep the capped branch per pair (the split form below would hold
@@ -219,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
     stageTile<ONE>(ti, tl.sortedPos, st);
     putSentinel<ONE>(st, tl.maxStaged, 0.0);
     __syncthreads();
-    double e = 0.0, w = 0.0;
+    double e = 0.0, w = 0.0, n = 0.0;
     for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
         if (t != int(threadIdx.x))
@@ -240,12 +264,18 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
         {
             uint4 nxt = cur;
             if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * tl.strideS];
-            ljChunk<PRED, ONE, CAP>(a, tab, p1, ti0, pi, st, cur, fx, fy, fz, e, w);
+            ljChunk<PRED, ONE, CAP>(a, tab, p1, ti0, pi, st, cur, fx, fy, fz, e, w, n);
             cur = nxt;
         }
         a.force[i] += fx;
         a.force[a.cap + i] += fy;
         a.force[2 * a.cap + i] += fz;
+    }
+    if constexpr (ONE && !CAP)
+    {
+        // e = sum f6, w = sum g over this lane's pairs (see ljChunk)
+        const double s12 = p1.ff1 != 0.0 ? (w + p1.ff2 * e) / p1.ff1 : 0.0;
+        e = p1.ef1 * s12 - p1.ef2 * e - p1.shift * n;
     }
     // every pair is visited from both ends: E = 1/2 sum, W = -1/2 sum(ff d2) / 2
     e = blockSum(0.5 * e, red);
