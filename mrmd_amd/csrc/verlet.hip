@@ -157,6 +157,10 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
             continue;
         }
         const int qSelf = ldsSlot(ti, ti.selfRange[self], s);
+        // entry k of this row: rowBase[(k / 8) * chunkStride + k % 8] (entryIndex in 32-bit
+        // arithmetic; the host keeps cap * strideS below 2^31)
+        uint16_t* const rowBase = nbr + int64_t(s) * CHUNK;
+        const int chunkStride = int(strideS) * CHUNK;
         const double4 xp = Staged<true>::get(st, qSelf);
         const int ck = cell % tg.n2, i = (cell / tg.n2) / tg.n1, j = (cell / tg.n2) % tg.n1;
         int c = 0;
@@ -214,15 +218,15 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                             hits |= uint32_t(valid && dx * dx + dy * dy + dz * dz <= rsq) << u;
                         }
                     }
-                    // append the hits in slot order with their half-list bit
+                    // append the hits in slot order with their half-list bit (Cabana's
+                    // lexicographic order, evaluated without branches on all three coordinates)
                     while (hits)
                     {
                         const int u = __builtin_ctz(hits);
                         hits &= hits - 1;
                         const double4 n = Staged<true>::get(st, qb + u);
-                        const bool half =
-                            (n.x > xp.x) || ((n.x == xp.x) && ((n.y > xp.y) || ((n.y == xp.y) && (n.z > xp.z))));
-                        if (c < cap) nbr[entryIndex(c, s, strideS)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
+                        const bool half = (n.x > xp.x) | ((n.x == xp.x) & ((n.y > xp.y) | ((n.y == xp.y) & (n.z > xp.z))));
+                        if (c < cap) rowBase[(c >> 3) * chunkStride + (c & 7)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
                         ++c;
                     }
                 }
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
         // pad the last chunk with the sentinel slot (a record beyond every cutoff), so the
         // force kernels' chunk loop needs no bounds test
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
-            nbr[entryIndex(k, s, strideS)] = uint16_t(sentinelSlot(stagedCap));
+            rowBase[(k >> 3) * chunkStride + (k & 7)] = uint16_t(sentinelSlot(stagedCap));
         cnt[s] = min(c, cap);  // the true width goes to maxCnt; reads stay inside the list
         localMax = max(localMax, c);
     }
@@ -442,6 +446,8 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                        size_t((vl->tileCols + 2 * R) * (vl->tileCols + 2 * R) * (vl->tileK + 2 * R + 1)) * sizeof(int);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
+        REQUIRE(int64_t(vl->capFull) * vl->strideS < (int64_t(1) << 31), "list of %d entries x %lld rows too large",
+                vl->capFull, (long long)vl->strideS);
         vl->nbr16.ensure(ctx, int64_t(vl->capFull) * vl->strideS);
         hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
         if (vl->ntiles > 0 && end > begin)
