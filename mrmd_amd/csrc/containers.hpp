@@ -119,6 +119,8 @@ __device__ inline double coord(const double4& p, int axis)
 // null) receives each atom's new position at its sorted slot slotOf[i]
 int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
                  uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr);
+// whether mrmd_lj_apply without a predicate uses the full-row kernel (no ghost forces)
+bool ljFullRows(const mrmd_lj* lj);
 // UpdateGhostAtoms::updateOnlyPos that also writes each ghost's new position to
 // sortedPos[slotOf[ghost]] (the slot-order copy of a Verlet list)
 int ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,

@@ -41,6 +41,10 @@ struct mrmd_lj
     void upload();
 };
 
+// true when mrmd_lj_apply without a predicate runs the full-row tiled kernel, which writes
+// the forces of list rows only (ghost records receive none)
+bool mrmd_hip::ljFullRows(const mrmd_lj* lj) { return lj && lj->symmetric; }
+
 void mrmd_lj::upload()
 {
     dtable.reserve(ctx, int64_t(table.size()));
@@ -100,7 +104,7 @@ struct TiledList
 template <bool PRED, bool ONE, bool CAP>
 __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, int ti0, const double4& pi,
                                const void* st, const uint4 cur, double& fx, double& fy, double& fz, double& e,
-                               double& w, double& n)
+                               double& w, int& n)
 {
 #pragma clang fp contract(fast)
     const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -155,7 +159,7 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
             fz += dz[u] * ff;
             e += f6;
             w += g;
-            n += in ? 1.0 : 0.0;
+            n += int(in);
         }
         return;
     }
@@ -243,7 +247,8 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
     stageTile<ONE>(ti, tl.sortedPos, st);
     putSentinel<ONE>(st, tl.maxStaged, 0.0);
     __syncthreads();
-    double e = 0.0, w = 0.0, n = 0.0;
+    double e = 0.0, w = 0.0;
+    int n = 0;
     for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
         if (t != int(threadIdx.x))
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
     {
         // e = sum f6, w = sum g over this lane's pairs (see ljChunk)
         const double s12 = p1.ff1 != 0.0 ? (w + p1.ff2 * e) / p1.ff1 : 0.0;
-        e = p1.ef1 * s12 - p1.ef2 * e - p1.shift * n;
+        e = p1.ef1 * s12 - p1.ef2 * e - p1.shift * double(n);
     }
     // every pair is visited from both ends: E = 1/2 sum, W = -1/2 sum(ff d2) / 2
     e = blockSum(0.5 * e, red);

@@ -81,7 +81,9 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
         vl->sortedFresh = fresh;
         MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
-        MD_CALL(mrmd_halo_fold_forces(gl, a));
+        // the full-row kernel leaves every ghost force at the zero just filled in, so the
+        // fold (which adds only non-zero ghost forces) would change nothing
+        if (!ljFullRows(lj)) MD_CALL(mrmd_halo_fold_forces(gl, a));
         localsScattered = false;
         if (k + 1 < nsteps)
         {
