@@ -210,14 +210,17 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
 #pragma unroll
                         for (int u = u0; u < u0 + 8; ++u)
                         {
-                            const bool valid = u < nq && qb + u != qSelf;
-                            const double4 n = Staged<true>::get(st, valid ? qb + u : qSelf);
+                            // reads stay inside the run; the bits past its end and the row's
+                            // own slot are cleared below
+                            const double4 n = Staged<true>::get(st, min(qb + u, q1 - 1));
                             const double dx = xp.x - n.x;
                             const double dy = xp.y - n.y;
                             const double dz = xp.z - n.z;
-                            hits |= uint32_t(valid && dx * dx + dy * dy + dz * dz <= rsq) << u;
+                            hits |= uint32_t(dx * dx + dy * dy + dz * dz <= rsq) << u;
                         }
                     }
+                    hits &= nq == 32 ? 0xffffffffu : (1u << nq) - 1u;
+                    if (qSelf >= qb && qSelf < qb + nq) hits &= ~(1u << (qSelf - qb));
                     // append the hits in slot order with their half-list bit (Cabana's
                     // lexicographic order, evaluated without branches on all three coordinates)
                     while (hits)
