@@ -205,10 +205,10 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     // branch-free passes over up to 32 candidates: hit bits
                     uint32_t hits = 0;
                     const int nq = min(32, q1 - qb);
-                    for (int u0 = 0; u0 < nq; u0 += 8)
+                    for (int u0 = 0; u0 < nq; u0 += 4)
                     {
 #pragma unroll
-                        for (int u = u0; u < u0 + 8; ++u)
+                        for (int u = u0; u < u0 + 4; ++u)
                         {
                             // reads stay inside the run; the bits past its end and the row's
                             // own slot are cleared below
