@@ -458,6 +458,9 @@ __device__ void bBuildRows(const BS& b, const TileB& ti, const TileG& G, const R
         const int qSelf = ti.base[ti.selfPiece] + (s - ti.begin[ti.selfPiece]);
         const double4 xp = fetch(qSelf);
         const int kz = (b.cellOf[b.sorted[s]] - R.c0) % R.nc[2];
+        // entry k of this row: rowBase[(k / 8) * chunkStride + k % 8] (entryIndex)
+        uint16_t* const rowBase = b.nbr + int64_t(s) * CHUNK;
+        const int64_t chunkStride = b.strideS * CHUNK;
         int c = 0;
         for (int di = -1; di <= 1; ++di)
             for (int dj = -1; dj <= 1; ++dj)
@@ -468,26 +471,30 @@ __device__ void bBuildRows(const BS& b, const TileB& ti, const TileG& G, const R
                     const int q1 = cs[1];
                     for (int q = cs[0]; q < q1; q += 4)
                     {
-                        bool hit[4];
+                        // reads stay inside the cell's run; the bits past its end and the
+                        // row's own slot are cleared after the distances
+                        uint32_t hits = 0;
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
                         {
-                            const bool valid = q + u < q1 && q + u != qSelf;
-                            const double4 xn = fetch(valid ? q + u : qSelf);
+                            const double4 xn = fetch(min(q + u, q1 - 1));
                             const double dx = xp.x - xn.x, dy = xp.y - xn.y, dz = xp.z - xn.z;
-                            hit[u] = valid && dx * dx + dy * dy + dz * dz <= b.rnSqr;
+                            hits |= uint32_t(dx * dx + dy * dy + dz * dz <= b.rnSqr) << u;
                         }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (hit[u])
-                            {
-                                if (c < b.capList) b.nbr[entryIndex(c, s, b.strideS)] = uint16_t(q + u);
-                                ++c;
-                            }
+                        const int nq = min(4, q1 - q);
+                        hits &= (1u << nq) - 1u;
+                        if (qSelf >= q && qSelf < q + nq) hits &= ~(1u << (qSelf - q));
+                        while (hits)
+                        {
+                            const int u = __builtin_ctz(hits);
+                            hits &= hits - 1;
+                            if (c < b.capList) rowBase[int64_t(c >> 3) * chunkStride + (c & 7)] = uint16_t(q + u);
+                            ++c;
+                        }
                     }
                 }
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < b.capList; ++k)
-            b.nbr[entryIndex(k, s, b.strideS)] = uint16_t(qSelf);
+            rowBase[int64_t(k >> 3) * chunkStride + (k & 7)] = uint16_t(qSelf);
         b.cnt[s] = min(c, b.capList);  // an overflow is reported through err; reads stay in the list
         localMax = max(localMax, c);
     }

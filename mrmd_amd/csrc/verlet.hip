@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <type_traits>
+
 #include "nlist.hpp"
 
 using namespace mrmd_hip;
@@ -90,7 +92,8 @@ constexpr int MAX_SPAN = 64;  // cells of a tile's column range along z (K + 2R)
 // min-distance prune (the surviving cells of a column as one slot range, cut by binary
 // search to the z window of the row, the records being z-sorted), eight candidates per pass
 // (independent LDS reads and distances, then the appends in slot order).
-template <int R>
+// WIDE: list offsets in 64-bit arithmetic (lists of 2^31 entries or more); 32-bit otherwise
+template <int R, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict__ sortedPos,
                                                        const int32_t* __restrict__ sorted,
                                                        const int32_t* __restrict__ cellOf,
@@ -157,10 +160,10 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
             continue;
         }
         const int qSelf = ldsSlot(ti, ti.selfRange[self], s);
-        // entry k of this row: rowBase[(k / 8) * chunkStride + k % 8] (entryIndex in 32-bit
-        // arithmetic; the host keeps cap * strideS below 2^31)
+        // entry k of this row: rowBase[(k / 8) * chunkStride + k % 8] (entryIndex)
+        using Off = std::conditional_t<WIDE, int64_t, int32_t>;
         uint16_t* const rowBase = nbr + int64_t(s) * CHUNK;
-        const int chunkStride = int(strideS) * CHUNK;
+        const Off chunkStride = Off(strideS) * CHUNK;
         const double4 xp = Staged<true>::get(st, qSelf);
         const int ck = cell % tg.n2, i = (cell / tg.n2) / tg.n1, j = (cell / tg.n2) % tg.n1;
         int c = 0;
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                         hits &= hits - 1;
                         const double4 n = Staged<true>::get(st, qb + u);
                         const bool half = (n.x > xp.x) | ((n.x == xp.x) & ((n.y > xp.y) | ((n.y == xp.y) & (n.z > xp.z))));
-                        if (c < cap) rowBase[(c >> 3) * chunkStride + (c & 7)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
+                        if (c < cap) rowBase[Off(c >> 3) * chunkStride + (c & 7)] = uint16_t((qb + u) | (half ? 0x8000 : 0));
                         ++c;
                     }
                 }
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
         // pad the last chunk with the sentinel slot (a record beyond every cutoff), so the
         // force kernels' chunk loop needs no bounds test
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < cap; ++k)
-            rowBase[(k >> 3) * chunkStride + (k & 7)] = uint16_t(sentinelSlot(stagedCap));
+            rowBase[Off(k >> 3) * chunkStride + (k & 7)] = uint16_t(sentinelSlot(stagedCap));
         cnt[s] = min(c, cap);  // the true width goes to maxCnt; reads stay inside the list
         localMax = max(localMax, c);
     }
@@ -449,22 +452,21 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                        size_t((vl->tileCols + 2 * R) * (vl->tileCols + 2 * R) * (vl->tileK + 2 * R + 1)) * sizeof(int);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
-        REQUIRE(int64_t(vl->capFull) * vl->strideS < (int64_t(1) << 31), "list of %d entries x %lld rows too large",
-                vl->capFull, (long long)vl->strideS);
         vl->nbr16.ensure(ctx, int64_t(vl->capFull) * vl->strideS);
         hipLaunchKernelGGL(k_zeroI32v, dim3(1), dim3(64), 0, ctx->stream, dmax, int64_t(1));
         if (vl->ntiles > 0 && end > begin)
         {
+            auto launch = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
+                                   vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
+                                   vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
+                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
+            };
+            const bool wide = int64_t(vl->capFull) * vl->strideS >= (int64_t(1) << 31);
             if (R == 1)
-                hipLaunchKernelGGL(k_verletTiled<1>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
-                                   vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
-                                   vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
+                wide ? launch(k_verletTiled<1, true>) : launch(k_verletTiled<1, false>);
             else
-                hipLaunchKernelGGL(k_verletTiled<2>, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
-                                   vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
-                                   vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
+                wide ? launch(k_verletTiled<2, true>) : launch(k_verletTiled<2, false>);
         }
         checkLaunch();
         int32_t hp[2];
