@@ -181,9 +181,12 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                 // records inside a column run are z-sorted: the candidates that can pass are
                 // those whose (dz)^2, as the test computes it, stays within rsq; d^2 >= (dz)^2
                 // for the rounded sums, so the cut changes no decision
-                int lo = cs0[kLo - kl], hi = cs0[kHi + 1 - kl];
+                // the window's lower edge lies in cell kLo and its upper edge in cell kHi (a
+                // cell wholly outside the window in z is out of reach and was pruned), so each
+                // binary search runs over one cell's records
+                int lo, hi;
                 {
-                    int a = lo, b = hi;  // first slot not below the window
+                    int a = cs0[kLo - kl], b = cs0[kLo + 1 - kl];  // first slot not below the window
                     while (a < b)
                     {
                         const int m = (a + b) >> 1;
@@ -192,7 +195,8 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                         else b = m;
                     }
                     lo = a;
-                    b = hi;  // first slot above the window
+                    a = max(lo, cs0[kHi - kl]);
+                    b = cs0[kHi + 1 - kl];  // first slot above the window
                     while (a < b)
                     {
                         const int m = (a + b) >> 1;
