@@ -918,3 +918,14 @@ extern "C" int mrmd_molecules_for_each_atom(mrmd_molecules* m, const mrmd_atoms*
     ++m->layoutVersion;
     MRMD_API_END
 }
+
+int mrmd_hip::fillForceRange(mrmd_atoms* a, int64_t lo, int64_t hi)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && lo >= 0 && lo <= hi && hi <= a->size, "invalid record range");
+    if (hi > lo)
+        hipLaunchKernelGGL(k_fill3, dim3(gridFor(hi - lo), 3), dim3(BLOCK), 0, a->ctx->stream, a->force.p + lo, a->cap,
+                           hi - lo, 0.0);
+    checkLaunch();
+    MRMD_API_END
+}

@@ -121,6 +121,12 @@ int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double te
                  uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr);
 // whether mrmd_lj_apply without a predicate uses the full-row kernel (no ghost forces)
 bool ljFullRows(const mrmd_lj* lj);
+// whether the table has the store form (single type, no capping): ljApplyAssign then
+// computes what zeroing the forces of the list rows and mrmd_lj_apply(pred = NULL) leave
+bool ljAssignable(const mrmd_lj* lj);
+int ljApplyAssign(mrmd_lj* lj, mrmd_atoms* atoms, mrmd_verlet_list* vl);
+// zero the forces of records [lo, hi)
+int fillForceRange(mrmd_atoms* atoms, int64_t lo, int64_t hi);
 // UpdateGhostAtoms::updateOnlyPos that also writes each ghost's new position to
 // sortedPos[slotOf[ghost]] (the slot-order copy of a Verlet list)
 int ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,

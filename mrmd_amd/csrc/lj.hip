@@ -217,7 +217,9 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
     }
 }
 
-template <bool PRED, bool ONE, bool CAP>
+// ASSIGN: the forces of the rows are stored as 0.0 + F instead of added (the caller zeroed
+// nothing there; 0.0 + F is what zeroing and adding leave, -0.0 included)
+template <bool PRED, bool ONE, bool CAP, bool ASSIGN = false>
 __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p1, double* partials)
 {
     extern __shared__ double4 st[];
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
             pi = tl.sortedPos[s];
             cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
         }
-        if (c == 0 || i >= a.rows) continue;
+        if ((!ASSIGN && c == 0) || i >= a.rows) continue;
         const int ti0 = ONE ? 0 : int(pi.w) * a.typeStride;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const uint4* chunk = reinterpret_cast<const uint4*>(tl.nbr) + s;
@@ -272,9 +274,18 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
             ljChunk<PRED, ONE, CAP>(a, tab, p1, ti0, pi, st, cur, fx, fy, fz, e, w, n);
             cur = nxt;
         }
-        a.force[i] += fx;
-        a.force[a.cap + i] += fy;
-        a.force[2 * a.cap + i] += fz;
+        if (ASSIGN)
+        {
+            a.force[i] = 0.0 + fx;
+            a.force[a.cap + i] = 0.0 + fy;
+            a.force[2 * a.cap + i] = 0.0 + fz;
+        }
+        else
+        {
+            a.force[i] += fx;
+            a.force[a.cap + i] += fy;
+            a.force[2 * a.cap + i] += fz;
+        }
     }
     if constexpr (ONE && !CAP)
     {
@@ -637,9 +648,44 @@ int mrmd_lj_eval(mrmd_lj* lj, const double* d2, const int64_t* idx, int64_t n, d
     MRMD_API_END
 }
 
-int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_pred2* pred)
+}  // extern "C"
+
+namespace
+{
+// the body of mrmd_lj_apply; assign: store the rows' forces instead of adding them, honoured
+// only by the full-row kernel of a single-type table without capping and predicate
+// (returns whether it was)
+bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_pred2* pred, bool assign);
+}  // namespace
+
+bool mrmd_hip::ljAssignable(const mrmd_lj* lj)
+{
+    if (!lj || !lj->symmetric || lj->table.size() != 1) return false;
+    return !(lj->table[0].capDistSqr > 0.0);
+}
+
+int mrmd_hip::ljApplyAssign(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl)
 {
     MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(ljAssignable(lj), "the table has no store form");
+    // without rows nothing is stored (and nothing needs to be)
+    if (!ljApplyImpl(lj, at, vl, nullptr, true) && at && vl && std::min<int64_t>(at->nlocal, vl->end) > 0)
+        fail(MRMD_ERR_STATE, "the store form did not run");
+    MRMD_API_END
+}
+
+extern "C" int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_pred2* pred)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    ljApplyImpl(lj, at, vl, pred, false);
+    MRMD_API_END
+}
+
+namespace
+{
+bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_pred2* pred, bool assign)
+{
+    bool assigned = false;
     REQUIRE(lj && at && vl && at->ctx == lj->ctx && vl->ctx == lj->ctx, "invalid arguments");
     validatePred2(pred);
     REQUIRE(vl->begin == 0 && vl->end <= at->nlocal + at->nghost, "the list rows must start at record 0");
@@ -675,7 +721,12 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
             for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
             auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, partials); };
             const int variant = (a.usePred ? 4 : 0) | (one ? 2 : 0) | (cap ? 1 : 0);
-            switch (variant)
+            if (assign && variant == 2)
+            {
+                launch(k_ljTiled<false, true, false, true>);
+                assigned = true;
+            }
+            else switch (variant)
             {
                 case 0: launch(k_ljTiled<false, false, false>); break;
                 case 1: launch(k_ljTiled<false, false, true>); break;
@@ -709,8 +760,11 @@ int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_
     }
     checkLaunch();
     lj->evValid = false;
-    MRMD_API_END
+    return assigned;
 }
+}  // namespace
+
+extern "C" {
 
 int mrmd_lj_energy_virial(mrmd_lj* lj, double* e, double* w)
 {

@@ -78,9 +78,19 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         }
         else
             MD_CALL(mrmd_halo_update(gl, a, box));
-        MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
         vl->sortedFresh = fresh;
-        MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
+        if (ljAssignable(lj))
+        {
+            // the force kernel stores the rows' forces (0.0 + F, what zeroing and adding
+            // leave); only the ghost records need their zero
+            MD_CALL(fillForceRange(a, a->nlocal, a->nlocal + a->nghost));
+            MD_CALL(ljApplyAssign(lj, a, vl));
+        }
+        else
+        {
+            MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
+            MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
+        }
         // the full-row kernel leaves every ghost force at the zero just filled in, so the
         // fold (which adds only non-zero ghost forces) would change nothing
         if (!ljFullRows(lj)) MD_CALL(mrmd_halo_fold_forces(gl, a));
