@@ -286,7 +286,7 @@ def main():
         print(json.dumps({"ensemble": res}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU (the kernels on one GPU's share); runs on several GPUs are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -179,11 +179,20 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                 while (kHi >= kLo && g.minDistSqr(xp.x, xp.y, xp.z, ii, jj, kHi) > rsq) --kHi;
                 if (kLo > kHi) continue;
                 // records inside a column run are z-sorted: the candidates that can pass are
-                // those whose (dz)^2, as the test computes it, stays within rsq; d^2 >= (dz)^2
-                // for the rounded sums, so the cut changes no decision
+                // those whose (dz)^2, as the test computes it, stays within rsq less the
+                // squared distance from the row to the column's cross-section (every record of
+                // the column lies at least that far away laterally); the lateral distance is
+                // shrunk by 1e-9 of a cell, far more than a record binned across a face by
+                // rounding can stray, and the bound widened by 1e-12 rsq, more than the
+                // rounding of the test's sums, so the cut changes no decision
                 // the window's lower edge lies in cell kLo and its upper edge in cell kHi (a
                 // cell wholly outside the window in z is out of reach and was pruned), so each
                 // binary search runs over one cell's records
+                double ex = fabs(xp.x - (g.min[0] + (double(ii) + 0.5) * g.dx[0])) - 0.5 * g.dx[0] - 1e-9 * g.dx[0];
+                double ey = fabs(xp.y - (g.min[1] + (double(jj) + 0.5) * g.dx[1])) - 0.5 * g.dx[1] - 1e-9 * g.dx[1];
+                ex = ex > 0.0 ? ex : 0.0;
+                ey = ey > 0.0 ? ey : 0.0;
+                const double zsq = rsq - (ex * ex + ey * ey) + 1e-12 * rsq;
                 int lo, hi;
                 {
                     int a = cs0[kLo - kl], b = cs0[kLo + 1 - kl];  // first slot not below the window
@@ -191,7 +200,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     {
                         const int m = (a + b) >> 1;
                         const double dz = xp.z - Staged<true>::get(st, m).z;
-                        if (dz > 0.0 && dz * dz > rsq) a = m + 1;
+                        if (dz > 0.0 && dz * dz > zsq) a = m + 1;
                         else b = m;
                     }
                     lo = a;
@@ -201,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     {
                         const int m = (a + b) >> 1;
                         const double dz = xp.z - Staged<true>::get(st, m).z;
-                        if (dz < 0.0 && dz * dz > rsq) b = m;
+                        if (dz < 0.0 && dz * dz > zsq) b = m;
                         else a = m + 1;
                     }
                     hi = a;
