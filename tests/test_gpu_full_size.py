@@ -1,0 +1,76 @@
+"""Parity at the full 4M-atom size of the LJ benchmark (FCC 100^3 x 4 at rho = 0.8442, each
+position displaced by a seeded uniform offset of up to 0.05 sigma so that the forces are not
+the perfect lattice's zeros): ghost count, every row's half-list count and the total pair
+count bit-exact against the oracle's Cabana restatement, LJ energy and virial within 1e-12
+relative, forces within 1e-12 of the largest force, and Newton's third law over the whole
+system (the full-row kernel computes F_i and F_j independently, so their sum vanishes only if
+both ends evaluate the same pair terms).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from mrmd_amd.lattice import fcc
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mrmd_amd
+    return mrmd_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(M):
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def test_lj_4m_lists_energy_forces_match_oracle(M, ctx):
+    pos, L = fcc(100, 0.8442)
+    n = pos.shape[0]
+    assert n == 4_000_000
+    rng = np.random.default_rng(20240)
+    pos = np.mod(pos + rng.uniform(-0.05, 0.05, pos.shape), L)
+    rn, rc = 2.8, 2.5
+    lo, hi = [-rn] * 3, [L + rn] * 3
+
+    g = M.Atoms.from_arrays(ctx, pos)
+    gl = M.GhostLayer(ctx)
+    ng = gl.create_ghost_atoms(g, M.make_box([0, 0, 0], [L] * 3, rn))
+    vl = M.HalfVerletList(ctx)
+    vl.build(g, 0, n, rn, 1.0, lo, hi)
+    lj = M.LennardJones(ctx, rc, 1.0, 1.0)
+    g.set_force(0.0)
+    lj.apply(g, vl)
+    gl.contribute_back_ghost_to_real(g)
+    E, W = lj.energy_virial()
+    Fg = g.download(M.FORCE, n)
+    gcnt, _ = vl.half_counts(n)
+    gpairs = vl.num_pairs()
+
+    O.lib().or_set_threads(16)
+    c = O.Atoms.from_arrays(pos)
+    og = O.Ghost()
+    og.create_xyz(c, O.make_box([0, 0, 0], [L] * 3, rn))
+    assert c.nghost == ng
+    ov = O.Verlet()
+    ov.build(c, c.nlocal + c.nghost, 0, n, rn, 1.0, lo, hi)
+    assert gpairs == ov.total()
+    np.testing.assert_array_equal(gcnt, ov.counts()[:n])
+    olj = O.LJ(0.0, rc, 1.0, 1.0)
+    cE, cW = olj.apply(c, ov)
+    og.fold(c)
+    Fc = c.get(O.FORCE, n)
+
+    assert E == pytest.approx(cE, rel=REL)
+    assert W == pytest.approx(cW, rel=REL)
+    fmax = np.abs(Fc).max()
+    assert np.abs(Fg - Fc).max() <= REL * fmax
+    # sum of all forces: zero up to the rounding of 4M per-atom sums (a pair missing at one
+    # end would leave a force of order 1 against a total magnitude of order 1e7)
+    assert np.abs(Fg.sum(axis=0)).max() <= 1e-12 * np.abs(Fg).sum(axis=0).max()
