@@ -14,9 +14,15 @@
 //    VelocityVerletLangevinThermostat.test.cpp; Langevin noise: Philox4x32-10 known-answer
 //    vectors (Random123) plus moment tests - the reference's Kokkos XorShift1024 stream is
 //    parity unpinned (statistical agreement only).
-//  * AdResS LJ/ideal gas: LJ_IdealGas.test.cpp 4-atom forces; exact gradient identity by
-//    finite differences; the compensation-energy counter of the reference is a data race
-//    (LJ_IdealGas.cpp:204-208) and is parity unpinned (this restatement counts atomically).
+//  * AdResS LJ/ideal gas: LJ_IdealGas.test.cpp 4-atom forces (grad lambda = 0 there); the
+//    drift term by the gradient identity (atom forces after ContributeMoleculeForceToAtoms
+//    and the ghost fold = -grad of the returned weighted energy, central differences, nu = 2
+//    so lambda_mod != lambda, diatomic molecules and a periodic slab across the boundary);
+//    compensation samples, counts, the running mean and the mean compensation force by a
+//    literal fixture (equilateral triangle at the LJ minimum: every V_ij = -epsilon/2) and a
+//    closed-form five-molecule case over all three regions and two types
+//    (tests/test_oracle_adress_pins.py).  The reference's compensation counter is a data
+//    race (LJ_IdealGas.cpp:204-208); this restatement counts atomically.
 //  * thermodynamic force / histograms: ThermodynamicForce.test.cpp, MultiHistogram.test.cpp.
 //  * slab weighting / regions / molecule update / force distribution: Slab.test.cpp,
 //    CheckRegion.test.cpp, UpdateMolecules.test.cpp, ContributeMoleculeForceToAtoms.test.cpp,
