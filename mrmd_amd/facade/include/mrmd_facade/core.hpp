@@ -10,6 +10,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <iostream>
 #include <limits>
@@ -1193,6 +1194,88 @@ inline data::Atoms restoreAtoms(const std::string& filename)
     data::Atoms a(n);
     data::deep_copy(a, h);
     return a;
+}
+
+/// io::restoreGRO (mrmd/io/RestoreGRO.cpp:24-147): a comment line, the atom count, one
+/// fixed-width line per atom (residue number, residue name, atom name, atom number,
+/// position and optionally velocity, scanned as "%5d%5s%5s%5d%8lf%8lf%8lf%8lf%8lf%8lf"),
+/// then the box edges, which set the subdomain's diameter from its minimum corner (ghost
+/// layer thickness kept).  Mass and relative mass 1, type 0, force 0; a line without
+/// velocities gives velocity 0 (with one warning).  Files with ghost atoms are refused.
+inline void restoreGRO(const std::string& filename,
+                       data::Subdomain& subdomain,
+                       data::Atoms& atoms,
+                       bool containsGhostAtoms = false)
+{
+    std::ifstream fin(filename);
+    if (!fin.is_open())
+    {
+        std::cerr << "Could not open file: " << filename << std::endl;
+        exit(EXIT_FAILURE);
+    }
+    if (containsGhostAtoms)
+    {
+        std::cout << "Reading a file with ghost atoms is not supported." << std::endl;
+        exit(EXIT_FAILURE);
+    }
+    std::string line;
+    std::getline(fin, line);  // comment
+    std::getline(fin, line);
+    int count = 0;
+    if (std::sscanf(line.c_str(), "%d", &count) != 1 || count <= 0)
+    {
+        std::cerr << "Invalid atom count in file " << filename << std::endl;
+        exit(EXIT_FAILURE);
+    }
+    data::HostAtoms h{idx_t(count)};
+    bool noVelocities = false;
+    idx_t n = 0;
+    while (n < count && std::getline(fin, line))
+    {
+        int resNr = 0, atomNr = 0;
+        char resName[6] = {}, atomName[6] = {};
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        const int fields = std::sscanf(line.c_str(), "%5d%5s%5s%5d%8lf%8lf%8lf%8lf%8lf%8lf", &resNr, resName,
+                                       atomName, &atomNr, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+        if (fields != 10)
+        {
+            if (fields != 7)
+            {
+                std::cerr << "Invalid GRO atom line in file " << filename
+                          << ": expected 7 fields for coordinates (or 10 fields including velocities), but parsed "
+                          << fields << " fields from line: \"" << line << "\"" << std::endl;
+                exit(EXIT_FAILURE);
+            }
+            noVelocities = true;
+            v[3] = v[4] = v[5] = 0.0;
+        }
+        for (int d = 0; d < 3; ++d)
+        {
+            h.getPos()(n, d) = v[d];
+            h.getVel()(n, d) = v[3 + d];
+        }
+        h.getMass()(n) = 1_r;
+        h.getRelativeMass()(n) = 1_r;
+        h.getType()(n) = 0;
+        ++n;
+    }
+    Vector3D diam{};
+    fin >> diam[0] >> diam[1] >> diam[2];
+    if (noVelocities)
+        std::cout << "Warning: Some lines in file " << filename
+                  << " do not contain velocities. Respective velocities have been set to zero, but this may lead to "
+                     "unexpected behavior."
+                  << std::endl;
+    subdomain = data::Subdomain(subdomain.minCorner,
+                                {subdomain.minCorner[0] + diam[0], subdomain.minCorner[1] + diam[1],
+                                 subdomain.minCorner[2] + diam[2]},
+                                subdomain.ghostLayerThickness);
+    h.numLocalAtoms = n;
+    h.numGhostAtoms = 0;
+    atoms = data::Atoms(n);
+    data::deep_copy(atoms, h);
+    atoms.numLocalAtoms = n;
+    atoms.numGhostAtoms = 0;
 }
 }  // namespace io
 }  // namespace mrmd

@@ -634,3 +634,22 @@ class AdResSRun:
         self.tf.apply(a)
         lib().or_vv_post(a.h, dt)
         return energy
+
+
+def read_gro(filename):
+    """io::restoreGRO's reading of a GROMACS coordinate file (io/RestoreGRO.cpp:24-147), here
+    by the format's fixed columns: atom lines hold the position in columns 21-44 (three 8-wide
+    fields) and the velocity in columns 45-68; the line after the atoms holds the box edges.
+    Returns (positions, velocities, edges)."""
+    with open(filename) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].strip())
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    for i in range(n):
+        ln = lines[2 + i]
+        pos[i] = [float(ln[20 + 8 * d:28 + 8 * d]) for d in range(3)]
+        if len(ln.rstrip()) > 44:
+            vel[i] = [float(ln[44 + 8 * d:52 + 8 * d]) for d in range(3)]
+    edges = np.array([float(w) for w in lines[2 + n].split()[:3]])
+    return pos, vel, edges

@@ -50,3 +50,49 @@ def test_checkpoint_frame_round_trip(tmp_path):
     np.savez(str(other), x=np.zeros(2))
     with pytest.raises(ValueError):
         checkpoint.read_frame(str(other))
+
+
+def _gro_fixture(tmp_path):
+    import gzip
+    import os
+    import shutil
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lennardJonesNVT_final.gro.gz")
+    dst = tmp_path / "lennardJonesNVT_final.gro"
+    with gzip.open(src, "rb") as f, open(dst, "wb") as g:
+        shutil.copyfileobj(f, g)
+    return dst
+
+
+def test_read_gro_reference_input(tmp_path):
+    """io::restoreGRO's reading (RestoreGRO.cpp:24-147) of the reference's NVE input
+    (examples/02_LennardJones_NVE/lennardJonesNVT_final.gro: 4,096 argon atoms with
+    velocities in a 20^3 box): the library's sscanf-style scan equals the oracle's
+    fixed-column read bit for bit, and the first and last atoms are the file's"""
+    import pyoracle as O
+
+    from mrmd_amd.io import read_gro
+    path = _gro_fixture(tmp_path)
+    pos, vel, edges = read_gro(str(path))
+    opos, ovel, oedges = O.read_gro(str(path))
+    assert pos.shape == (4096, 3) and np.array_equal(edges, [20.0, 20.0, 20.0])
+    assert np.array_equal(pos, opos) and np.array_equal(vel, ovel) and np.array_equal(edges, oedges)
+    assert pos[0].tolist() == [14.587, 5.263, 17.966] and vel[0].tolist() == [-1.0355, 0.7038, -0.6883]
+    assert pos[-1].tolist() == [4.579, 8.706, 3.082] and vel[-1].tolist() == [1.3601, -1.2157, -0.2029]
+
+
+def test_read_gro_without_velocities_and_invalid_lines(tmp_path):
+    """a line of 7 fields gives velocity 0; a line of another field count is refused, as
+    restoreGRO exits on it"""
+    import pytest
+
+    from mrmd_amd.io import read_gro
+    p = tmp_path / "a.gro"
+    p.write_text("title\n2\n    1Argon   Ar    1   1.000   2.000   3.000  0.5000 -0.2500  0.1250\n"
+                 "    2Argon   Ar    2   4.000   5.000   6.000\n   10.0 11.0 12.0\n")
+    pos, vel, edges = read_gro(str(p))
+    assert pos.tolist() == [[1, 2, 3], [4, 5, 6]] and vel.tolist() == [[0.5, -0.25, 0.125], [0, 0, 0]]
+    assert edges.tolist() == [10, 11, 12]
+    bad = tmp_path / "b.gro"
+    bad.write_text("title\n1\n    1Argon   Ar    1   1.000   2.000\n   1 1 1\n")
+    with pytest.raises(ValueError):
+        read_gro(str(bad))
