@@ -198,6 +198,8 @@ struct mrmd_verlet_list
     int block = 256;            ///< workgroup size of the tiled kernels
     int tilesPerCol = 0;
     int64_t ntiles = 0;
+    int rowOrigin[3] = {0, 0, 0};  ///< first cell, per axis, of the box of cells holding rows
+    int rowExtent[3] = {0, 0, 0};  ///< its extent in cells (tiles cover only this box)
     int32_t maxStaged = 0;
     int64_t strideS = 0;        ///< ELL stride over sorted slots (>= nrows, multiple of 64)
     mrmd_hip::DBuf<mrmd_hip::TileInfo> tiles;

@@ -95,8 +95,8 @@ def test_espp_verlet_pairs_sets_energy_forces(M, ctx, espp_positions):
     assert np.abs(Fg - Fc).max() <= F_TOL * np.abs(Fc).max()
 
 
-@pytest.mark.parametrize("columns,cells,block", [(1, 1, 64), (1, 5, 256), (2, 2, 128), (2, 4, 256), (3, 1, 192),
-                                                 (3, 2, 256)])
+@pytest.mark.parametrize("columns,cells,block", [(1, 1, 64), (1, 5, 256), (2, 2, 128), (2, 3, 256), (2, 4, 256),
+                                                 (3, 1, 192)])
 def test_tilings_give_identical_lists_and_forces(M, ctx, espp_positions, columns, cells, block):
     """the tile shape and workgroup size change neither the neighbour rows (set and order)
     nor a single bit of the forces: rows are summed in stencil order whatever the tiling"""
