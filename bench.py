@@ -27,6 +27,7 @@ single-system path, beside the CPU restatement on a bounded sample of the same r
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -39,14 +40,15 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic():
-    """HBM bytes per force-kernel launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (calibrated against a device copy), or None"""
+def measured_traffic(key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (calibrated against a device copy): `key` names the force kernel's line
+    (default) or the list build's, or None"""
     path = os.path.join(ROOT, "profiles", "lj_force_4M_traffic.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None
@@ -63,7 +65,10 @@ def build_system(M, ctx, cells, temperature, seed):
     return s, pos, vel, L
 
 
-def cpu_baseline(pos, vel, L, steps, warm, threads):
+def cpu_baseline(pos, vel, L, steps, warm, reps, threads):
+    """the CPU restatement of the reference algorithm (AoS records, half list with atomic F_j,
+    OpenMP) on the same system: `warm` untimed steps, then `reps` timed runs of `steps` steps
+    each; returns [(atom-steps/s, seconds, rebuilds)] per run"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     O.lib().or_set_threads(threads)
@@ -72,20 +77,71 @@ def cpu_baseline(pos, vel, L, steps, warm, threads):
     prm = O.md_params(nsteps=0, langevin=1, permute=1, shifted=1, gamma=1.0, temperature=1.0, seed=1234)
     md = O.lib().or_md_new(a.h, O.C.byref(box), O.C.byref(prm))
     O.lib().or_md_steps(md, warm)
-    r0 = O.lib().or_md_steps(md, 0)
-    t0 = time.perf_counter()
-    r1 = O.lib().or_md_steps(md, steps)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(reps):
+        r0 = O.lib().or_md_steps(md, 0)
+        t0 = time.perf_counter()
+        r1 = O.lib().or_md_steps(md, steps)
+        dt = time.perf_counter() - t0
+        runs.append((pos.shape[0] * steps / dt, dt, r1 - r0))
     O.lib().or_md_free(md)
-    return pos.shape[0] * steps / dt, dt, r1 - r0
+    return runs
 
 
-def ensemble_bench(ctx, n_rep, steps, serial_steps, tune=True):
+def host_cpu():
+    """the host CPU: model and physical cores from lscpu, and the CPUs this process may use"""
+    info = {"model": None, "physical_cores": None, "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = {k.strip(): v.strip() for k, v in (l.split(":", 1) for l in out.splitlines() if ":" in l)}
+        info["model"] = kv.get("Model name")
+        info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, KeyError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads():
+    """(host_cpu(), the CPU share OMP_NUM_THREADS grants, threads to use): the physical cores,
+    capped by the share and by this process's affinity"""
+    cpu = host_cpu()
+    share = int(os.environ.get("OMP_NUM_THREADS", cpu["affinity_cpus"]))
+    threads = max(1, min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"], share) if x))
+    return cpu, share, threads
+
+
+def cpu_baseline_leg(cells, steps, warm, reps):
+    """runs the CPU sample in a child process, so that the OpenMP placement (threads spread
+    over the cores, one per place) applies from the runtime's start: threads = the physical
+    cores lscpu reports, capped by this process's CPU share (OMP_NUM_THREADS when set, and
+    its affinity).  Returns the cpu_baseline object (median of the timed runs)."""
+    cpu, share, threads = cpu_threads()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="threads")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", "--cells", str(cells),
+                          "--cpu-steps", str(steps), "--cpu-warmup", str(warm), "--cpu-reps", str(reps),
+                          "--cpu-threads", str(threads)], capture_output=True, text=True, env=env, check=True)
+    runs = json.loads(out.stdout.strip().splitlines()[-1])["runs"]
+    rates = sorted(r[0] for r in runs)
+    med = rates[len(rates) // 2]
+    n = 4 * cells ** 3
+    return {"value": med, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"median of {reps} runs of {steps} steps ({reps * steps} timed steps) of the same {n}-atom "
+                      f"system after {warm} warm-up steps; {sum(r[2] for r in runs)} list rebuild(s) in the timed "
+                      f"steps, {sum(r[1] for r in runs):.1f} s timed; OpenMP {threads} threads, "
+                      f"OMP_PROC_BIND=spread, OMP_PLACES=threads",
+            "runs_atom_steps_per_s": [r[0] for r in runs], "cpu_model": cpu["model"],
+            "physical_cores_lscpu": cpu["physical_cores"], "process_cpus": cpu["affinity_cpus"],
+            "cpu_share_omp_num_threads": share}
+
+
+def ensemble_bench(ctx, n_rep, steps, tune=True):
     """256-replica ensemble (atom counts log-uniform in [500, 50000], densities in [0.3, 0.9],
     temperatures in [0.7, 1.5], half AdResS slabs) through the ensemble engine with 10-step
     graphs, against the same replicas run one after another through the single-system path
-    (LJSystem / AdResSSystem).  Both use the capped potential (r_cap = 0.7) that keeps
-    ideal-gas particles entering the hybrid region finite."""
+    (LJSystem / AdResSSystem) over the same number of steps from the same state.  Both use
+    the capped potential (r_cap = 0.7) that keeps ideal-gas particles entering the hybrid
+    region finite.  The launch shape comes from mrmd_amd.autotune.tune_ensemble."""
+    from mrmd_amd.autotune import tune_ensemble
     from mrmd_amd.drivers import AdResSSystem, LJSystem
     from mrmd_amd.ensemble import Ensemble, ensemble_config, relax
     reps, pos, _ = ensemble_config(n_rep=n_rep, seed=2024)
@@ -95,30 +151,17 @@ def ensemble_bench(ctx, n_rep, steps, serial_steps, tune=True):
     e.upload(0, pos)
     e.upload(1, vel)
     e.run(100, 10)
-    tuning = None
-    if tune:
-        best = None
-        for target in (700, 1000, 1400):
-            for block in (128, 256):
-                e.set_tiling(target, block)
-                e.run(10, 10)
-                ctx.synchronize()
-                t0 = time.perf_counter()
-                e.run(20, 10)
-                dt = time.perf_counter() - t0
-                if best is None or dt < best[0]:
-                    best = (dt, target, block)
-        e.set_tiling(best[1], best[2])
-        tuning = {"target_staged": best[1], "block": best[2]}
+    tuning = tune_ensemble(e)[0] if tune else None
     e.run(10, 10)
     ctx.synchronize()
+    P, V = e.positions(), e.velocities()  # the serial runs start from this state too
     t0 = time.perf_counter()
     e.run(steps, 10)
+    ctx.synchronize()
     batched_s = time.perf_counter() - t0
     n_total = e.n_total
     batched = n_total * steps / batched_s
-    P, V = e.positions(), e.velocities()
-    # serial: every replica through the single-system path
+    # serial: every replica through the single-system path, the same steps
     serial_time, serial_atom_steps = 0.0, 0
     for k, r in enumerate(reps):
         common = dict(rc=2.5, skin=0.3, cap=0.7, shifted=True, gamma=r["gamma"], temperature=r["temperature"],
@@ -128,18 +171,21 @@ def ensemble_bench(ctx, n_rep, steps, serial_steps, tune=True):
                                 **common)
         else:
             sysk = LJSystem(ctx, P[k], V[k], r["box"], langevin=True, **common)
-        sysk.run(2, r["dt"])
+        sysk.run(1, r["dt"])  # its first list build, outside the clock like the ensemble's
         ctx.synchronize()
         t0 = time.perf_counter()
-        sysk.run(serial_steps, r["dt"])
+        sysk.run(steps, r["dt"])
         ctx.synchronize()
         serial_time += time.perf_counter() - t0
-        serial_atom_steps += r["num_atoms"] * serial_steps
+        serial_atom_steps += r["num_atoms"] * steps
         del sysk
     serial = serial_atom_steps / serial_time
     return {"replicas": n_rep, "atoms": n_total, "adress_replicas": sum(r["adress"] for r in reps),
             "batched_atom_steps_per_s": batched, "batched_steps": steps, "graph_steps": 10,
-            "serial_atom_steps_per_s": serial, "serial_steps_per_replica": serial_steps,
+            "serial_atom_steps_per_s": serial, "serial_steps_per_replica": steps,
+            "serial_note": "each replica constructed from the ensemble's state before its timed steps and run "
+                           "one untimed step (its first list build, as the ensemble's lists and graphs exist before "
+                           "its clock starts), then the same number of steps timed",
             "speedup": batched / serial, "tiling": tuning}
 
 
@@ -254,14 +300,16 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="FCC cells per edge (100 -> 4M atoms)")
     ap.add_argument("--profile-steps", type=int, default=60,
                     help="untimed steps after the timed ones that time every kernel (HIP events)")
-    ap.add_argument("--cpu-steps", type=int, default=6)
-    ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=17, help="timed steps per CPU run")
+    ap.add_argument("--cpu-warmup", type=int, default=10)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="timed CPU runs (the median is reported)")
+    ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning)")
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint figures")
     ap.add_argument("--ensemble-replicas", type=int, default=256)
-    ap.add_argument("--ensemble-steps", type=int, default=200)
-    ap.add_argument("--serial-steps", type=int, default=10)
+    ap.add_argument("--ensemble-steps", type=int, default=300, help="steps of the batched and of each serial run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--adress-steps", type=int, default=300, help="timed steps of the AdResS slab extra")
@@ -271,6 +319,18 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the ensemble comparison and print its JSON line")
     args = ap.parse_args()
+    if args.cpu_sample:
+        # the CPU leg's child process: the same FCC system and velocities, from the lattice
+        # module alone (the package would load the GPU library)
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("lattice", os.path.join(ROOT, "mrmd_amd", "lattice.py"))
+        lattice = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(lattice)
+        pos, L = lattice.fcc(args.cells, 0.8442)
+        vel = lattice.maxwell_boltzmann(pos.shape[0], 1.0, 1234)
+        runs = cpu_baseline(pos, vel, L, args.cpu_steps, args.cpu_warmup, args.cpu_reps, args.cpu_threads)
+        print(json.dumps({"runs": runs}))
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -281,8 +341,7 @@ def main():
         return
     if args.ensemble_only:
         import mrmd_amd as M
-        res = ensemble_bench(M.Context(0), args.ensemble_replicas, args.ensemble_steps, args.serial_steps,
-                             tune=not args.no_tune)
+        res = ensemble_bench(M.Context(0), args.ensemble_replicas, args.ensemble_steps, tune=not args.no_tune)
         print(json.dumps({"ensemble": res}))
         return
     if args.gpus != 1:
@@ -319,7 +378,7 @@ def main():
     ctx.set_profiling(True)
     ctx.reset_stats()
     s.run(args.profile_steps, dt)
-    stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "post_pre", "langevin_pre", "vv_post", "halo_create",
+    stats = {k: ctx.kernel_stats(k) for k in ("verlet_build", "verlet_tiled", "post_pre", "langevin_pre", "vv_post", "halo_create",
                                               "halo_update", "halo_fold", "atoms_permute", "cell_list_build")}
     nrec = s.atoms.nlocal + s.atoms.nghost
     full_entries = s.vl.info()["total_full"]
@@ -345,23 +404,28 @@ def main():
     bandwidth = {"post_pre": gbs("post_pre", 136.0 * n), "langevin_pre": gbs("langevin_pre", 136.0 * n),
                  "vv_post": gbs("vv_post", 80.0 * n),
                  "verlet_build": gbs("verlet_build", 72.0 * nrec + 2.0 * full_entries + 4.0 * nrec)}
+    # the tiled list kernel alone: each record's position read once (32 B), a row's slot ->
+    # record and record -> cell lookups (8 B), 2 B per list entry and a 4 B count per row
+    list_bytes = 32.0 * nrec + 12.0 * n + 2.0 * full_entries
+    lk, lt = stats["verlet_tiled"]
+    list_gbs = list_bytes / (lt / lk / 1e3) / 1e9 if lk else None
+    roofline_list = {"bound": "hbm", "achieved": list_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": list_gbs / HBM_PEAK_GBS if lk else None,
+                     "traffic": measured_traffic("list_build_traffic_bytes_per_launch") if args.cells == 100 else None,
+                     "kernel": "k_verletTiled (Verlet list build)", "avg_ms": lt / lk if lk else None,
+                     "launches": lk, "bytes_per_launch": list_bytes}
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cv, ct, creb = cpu_baseline(pos, vel, L, args.cpu_steps, args.cpu_warmup, threads)
-        cpu = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {n}-atom system after {args.cpu_warmup} "
-                         f"warm-up step(s), {creb} list rebuild(s) inside, {ct:.1f} s wall"}
+        cpu = cpu_baseline_leg(args.cells, args.cpu_steps, args.cpu_warmup, args.cpu_reps)
 
     extras = None
     if not args.no_extras:
-        extras = {"ensemble": ensemble_bench(ctx, args.ensemble_replicas, args.ensemble_steps, args.serial_steps,
-                                             tune=not args.no_tune),
+        extras = {"ensemble": ensemble_bench(ctx, args.ensemble_replicas, args.ensemble_steps, tune=not args.no_tune),
                   "adjoint": adjoint_ratio(ctx)}
         ad, (P, V, L2) = adress_slab(ctx, args.adress_steps)
         if not args.no_cpu:
-            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            threads = cpu_threads()[2]
             cv, ct, creb = cpu_adress_sample(P, V, L2, args.adress_cpu_steps, threads)
             ad["cpu_baseline"] = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
                                   "sample": f"{args.adress_cpu_steps} steps from the same relaxed state, "
@@ -384,11 +448,12 @@ def main():
         "config": {"workload": "plain LJ NVT (Langevin), FP64, FCC 100^3 x 4", "atoms": n,
                    "density": 0.8442, "rc": 2.5, "skin": 0.3, "dt": dt, "temperature": 1.0, "gamma": 1.0,
                    "rebuilds_in_timed_steps": rebuilds, "tiling": tiling},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic() if args.cells == 100 else None,
                      "kernel": "k_ljTiled (LJ force)", "avg_ms": avg_s * 1e3, "launches": launches,
                      "half_pairs_rn": half_rn, "half_pairs_rc": half_rc, "flop_per_launch": flops},
+        "roofline_list_build": roofline_list,
         "cpu_baseline": cpu,
         "kernel_ms": {k: {"launches": v[0], "total_ms": v[1]} for k, v in stats.items()},
         "kernel_ms_pass": f"{args.profile_steps} untimed steps after the timed ones",

@@ -1,4 +1,4 @@
-"""Autotuner of the tiled kernels' launch shape.
+"""Autotuners of the tiled kernels' launch shape (single systems and ensembles).
 
 The tiled Verlet list and the force kernels have three free parameters: the tile shape
 (a x a cell columns by K cells along z, i.e. the atoms each workgroup owns and the size of
@@ -50,3 +50,36 @@ def tune_lj_system(system, shapes=DEFAULT_SHAPES, blocks=DEFAULT_BLOCKS, force_r
     system.rebuild()
     ctx.set_profiling(was)
     return best, table
+
+
+ENSEMBLE_TARGETS = (700, 1000, 1400)  # staged records per workgroup a replica's tile height aims at
+ENSEMBLE_BLOCKS = (128, 256)
+
+
+def tune_ensemble(ensemble, targets=ENSEMBLE_TARGETS, blocks=ENSEMBLE_BLOCKS, steps=20, graph_steps=10,
+                  verbose=False):
+    """pick the ensemble's launch shape: the target number of staged records per workgroup
+    (which sets every replica's tile height K from its density, i.e. the atoms per workgroup
+    and the cell-tile shape) and the workgroup size.  Each candidate runs `steps` steps of the
+    live ensemble (after a short untimed run that rebuilds its lists and graphs) and the
+    fastest wall time wins; results do not depend on the shape, so tuning advances the
+    ensemble's state like any other steps.  Leaves the best setting applied and returns
+    ({"target_staged", "block"}, table of (target, block, seconds))."""
+    import time
+    ctx = ensemble.ctx
+    table = []
+    for target in targets:
+        for block in blocks:
+            ensemble.set_tiling(target, block)
+            ensemble.run(graph_steps, graph_steps)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ensemble.run(steps, graph_steps)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            table.append((target, block, dt))
+            if verbose:
+                print(f"target_staged={target} block={block} {dt * 1e3 / steps:.3f} ms/step", flush=True)
+    best = min(table, key=lambda r: r[2])
+    ensemble.set_tiling(best[0], best[1])
+    return {"target_staged": best[0], "block": best[1]}, table
