@@ -293,6 +293,18 @@ int mrmd_lj_eval(mrmd_lj* lj, const double* dist_sqr, const int64_t* type_idx, i
 /* LennardJones::apply / apply_if (LennardJones.hpp:136-206): adds pair forces to local atoms.
  * pred == NULL is apply().  Energy and virial stay on the device until requested. */
 int mrmd_lj_apply(mrmd_lj* lj, mrmd_atoms* atoms, mrmd_verlet_list* vl, const mrmd_pred2* pred);
+/* No counterpart in the reference: mrmd_lj_apply (no predicate) for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Adds the pair forces to the local atoms with the same kernel and bits
+ * as mrmd_lj_apply and writes energy and virial to the caller's device memory
+ * energy_virial_dev[0..1] (and to the table, for a later mrmd_lj_energy_virial), without
+ * synchronising.  It calls no hipMalloc / hipFree: the per-workgroup partials go to a static
+ * device array of MRMD_LJ_NOALLOC_MAX_TILES x 2 doubles (1 MiB) that each GPU the library
+ * runs on holds until the process ends and that every call on that GPU shares, so calls that
+ * overlap on one GPU race.  Symmetric tables only (the full-row kernel); a list with more
+ * tiles than MRMD_LJ_NOALLOC_MAX_TILES is refused with MRMD_ERR_UNSUP. */
+#define MRMD_LJ_NOALLOC_MAX_TILES 65536
+int mrmd_lj_apply_noalloc(mrmd_lj* lj, mrmd_atoms* atoms, mrmd_verlet_list* vl, double* energy_virial_dev);
 /* getEnergy / getVirial (LennardJones.cpp:35-36); synchronises */
 int mrmd_lj_energy_virial(mrmd_lj* lj, double* energy, double* virial);
 /* Adjoint of the force map F(x; eps, sigma) (no reference counterpart; for force matching):

@@ -345,6 +345,7 @@ int mrmd_ctx_create(int device, mrmd_ctx** out)
         fail(MRMD_ERR_UNSUP, "this library is built for gfx950, device is %s", prop.gcnArchName);
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->ownStream = true;
+    ljNoAllocSetup(device);
     // coherent: kernels write scalars here for the host to read while the stream runs on
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), 64 * sizeof(double), hipHostMallocCoherent));
     *out = c;

@@ -121,6 +121,10 @@ int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double te
                  uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr);
 // whether mrmd_lj_apply without a predicate uses the full-row kernel (no ghost forces)
 bool ljFullRows(const mrmd_lj* lj);
+// device address of the allocation-free apply's scratch array on `device` (the current
+// device); called at every context creation, looks it up once per device; false when the
+// lookup failed
+bool ljNoAllocSetup(int device);
 // whether the table has the store form (single type, no capping): ljApplyAssign then
 // computes what zeroing the forces of the list rows and mrmd_lj_apply(pred = NULL) leave
 bool ljAssignable(const mrmd_lj* lj);

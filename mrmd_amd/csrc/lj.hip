@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "nlist.hpp"
 #include "physics.hpp"
@@ -764,7 +765,113 @@ bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_p
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------------------
+// Allocation-free apply (mrmd_lj_apply_noalloc): the per-workgroup energy / virial partials
+// of the full-row kernel go to `scratch`, a file-scope device array of LJ_NOALLOC_MAX_TILES
+// x 2 doubles (1 MiB) that each GPU holds until the process ends.  Host code never uses the
+// array's name as a device address: its per-device address comes from
+// hipGetSymbolAddress once per device, when a context is created on it.
+namespace
+{
+constexpr int64_t LJ_NOALLOC_MAX_TILES = MRMD_LJ_NOALLOC_MAX_TILES;
+static __device__ __align__(256) double scratch[2 * LJ_NOALLOC_MAX_TILES];
+constexpr int NOALLOC_DEVICES = 64;
+double* scratchDev[NOALLOC_DEVICES] = {};
+std::mutex scratchMutex;
+
+// sum of the partials into the caller's device memory and the table's own energy / virial
+// (so that a later energy_virial() reads this call's values), one ordered pass
+__global__ void k_sumPartialsTo(const double* partials, int64_t n, double* out, double* ev)
+{
+    __shared__ double red[BLOCK / WAVE];
+    for (int c = 0; c < 2; ++c)
+    {
+        double s = 0.0;
+        for (int64_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k * 2 + c];
+        s = blockSum(s, red);
+        if (threadIdx.x == 0)
+        {
+            out[c] = s;
+            ev[c] = s;
+        }
+    }
+}
+}  // namespace
+
+// called by mrmd_ctx_create with `device` current; a failed lookup leaves the device's
+// entry empty, and mrmd_lj_apply_noalloc then fails with that message
+bool mrmd_hip::ljNoAllocSetup(int device)
+{
+    std::lock_guard<std::mutex> lock(scratchMutex);
+    if (device < 0 || device >= NOALLOC_DEVICES) return false;
+    if (scratchDev[device]) return true;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+    scratchDev[device] = static_cast<double*>(p);
+    return true;
+}
+
 extern "C" {
+
+int mrmd_lj_apply_noalloc(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, double* ev_dev)
+{
+    MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
+    REQUIRE(lj && at && vl && ev_dev && at->ctx == lj->ctx && vl->ctx == lj->ctx, "invalid arguments");
+    if (!lj->symmetric) fail(MRMD_ERR_UNSUP, "the allocation-free apply serves symmetric tables (the full-row kernel) only");
+    REQUIRE(vl->begin == 0 && vl->end <= at->nlocal + at->nghost, "the list rows must start at record 0");
+    REQUIRE(vl->nrows <= at->size, "the list was built for more records than the atoms hold");
+    REQUIRE(at->maxType * lj->typeStride + at->maxType < int64_t(lj->table.size()),
+            "atom type %lld has no entry in the %lld-type table", (long long)at->maxType, (long long)lj->numTypes);
+    if (vl->ntiles > LJ_NOALLOC_MAX_TILES)
+        fail(MRMD_ERR_UNSUP, "the list has %lld tiles, the allocation-free apply's scratch holds %lld (use mrmd_lj_apply)",
+             (long long)vl->ntiles, (long long)LJ_NOALLOC_MAX_TILES);
+    mrmd_ctx* ctx = lj->ctx;
+    double* part = ctx->device >= 0 && ctx->device < NOALLOC_DEVICES ? scratchDev[ctx->device] : nullptr;
+    if (!part)
+        fail(MRMD_ERR_HIP, "hipGetSymbolAddress gave no address for the allocation-free scratch on device %d",
+             ctx->device);
+    const int64_t rows = std::min<int64_t>(at->nlocal, vl->end);
+    const bool fresh = vl->sortedFresh;
+    vl->sortedFresh = false;
+    if (rows > 0)
+    {
+        REQUIRE(vl->ntiles > 0, "the Verlet list has not been built");
+        if (!fresh) vl->refreshSorted(at->pos.p);
+        LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
+                 int(lj->typeStride), lj->rcSqr, mrmd_pred2{}, 0};
+        const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
+                           vl->bin.sortedPos.p, vl->maxStaged};
+        const Pre p1 = lj->table[0];
+        const bool one = lj->table.size() == 1;
+        bool cap = false;
+        for (const auto& q : lj->table) cap = cap || q.capDistSqr > 0.0;
+        const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) * (one ? Staged<true>::bytes : Staged<false>::bytes);
+        const dim3 g(unsigned(vl->ntiles)), b(vl->block);
+        auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, part); };
+        switch ((one ? 2 : 0) | (cap ? 1 : 0))
+        {
+            case 0: launch(k_ljTiled<false, false, false>); break;
+            case 1: launch(k_ljTiled<false, false, true>); break;
+            case 2: launch(k_ljTiled<false, true, false>); break;
+            default: launch(k_ljTiled<false, true, true>); break;
+        }
+        hipLaunchKernelGGL(k_sumPartialsTo, dim3(1), dim3(BLOCK), 0, ctx->stream, part, vl->ntiles, ev_dev, lj->ev.p);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, ctx->stream, ev_dev, int64_t(2));
+        hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(64), 0, ctx->stream, lj->ev.p, int64_t(2));
+    }
+    checkLaunch();
+    lj->nPartials = 0;
+    lj->evValid = false;
+    MRMD_API_END
+}
 
 int mrmd_lj_energy_virial(mrmd_lj* lj, double* e, double* w)
 {
