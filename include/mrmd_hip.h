@@ -271,6 +271,10 @@ int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* atoms, doub
  * 1 when the cell ratio is below 1); block = workgroup size (64, 128, 192 or 256).
  * Applies from the next build; results do not depend on it. */
 int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_columns, int block);
+/* No counterpart in the reference (a test knob): on != 0 builds the list with the 64-bit
+ * entry-offset form of the build kernel, which otherwise serves only lists of 2^31 entries
+ * or more (about 20M atoms); the list is the same either way. */
+int mrmd_verlet_set_wide_offsets(mrmd_verlet_list* vl, int on);
 /* the tiling in use: cells per tile, columns, workgroup size, largest staged neighbourhood */
 int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* tile_columns, int* block,
                            int* max_staged);

@@ -403,6 +403,11 @@ class HalfVerletList:
         workgroup size of the tiled list build and force kernel; applies from the next build"""
         self.ctx.check(_L.mrmd_verlet_set_tiling(self.h, int(tile_cells), int(tile_columns), int(block)))
 
+    def set_wide_offsets(self, on=True):
+        """build with the 64-bit entry-offset form of the list kernel (a test knob: that form
+        otherwise serves only lists of 2^31 entries or more); the list is the same"""
+        self.ctx.check(_L.mrmd_verlet_set_wide_offsets(self.h, 1 if on else 0))
+
     def tiling(self):
         k, a, b, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self.ctx.check(_L.mrmd_verlet_get_tiling(self.h, C.byref(k), C.byref(a), C.byref(b), C.byref(s)))

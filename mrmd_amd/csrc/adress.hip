@@ -234,6 +234,7 @@ struct IGTiled
     int64_t rowsEnd;       ///< molecules [0, rowsEnd) are rows
     int queueCap;          ///< queued non-row partner pairs per workgroup (<= IG_QUEUE)
     int32_t maxStaged;     ///< the sentinel record sits at sentinelSlot(maxStaged)
+    double sentinel;       ///< at (sentinel, sentinel, sentinel)
 };
 
 // sorted-slot record of each molecule's single atom and its modulated lambda (in [0, 1];
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void k_igTiled(IGArgs g, IGTiled tl, Pre p, 
         cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
     }
     stageTile<false>(ti, tl.igPos, st);
-    putSentinel<false>(st, tl.maxStaged, 0.0);
+    putSentinel<false>(st, tl.maxStaged, tl.sentinel, 0.0);
     __syncthreads();
     // a neighbourhood of coarse-grained molecules only (lambda_mod = 0 everywhere) has no
     // pair work (LJ_IdealGas.cpp:95), no drift and no compensation term
@@ -904,7 +905,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
         hipLaunchKernelGGL(k_igGather, dim3(gridFor(vl->nrows)), dim3(BLOCK), 0, ctx->stream, m->view(), a->pos.p,
                            vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
         const IGTiled tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p, s->igPos.p, rows,
-                         s->queueCap, vl->maxStaged};
+                         s->queueCap, vl->maxStaged, vl->sentinel};
         const Pre p = s->table[0];
         const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) * sizeof(double4);
         if (p.capDistSqr > 0.0)

@@ -92,6 +92,7 @@ struct TiledList
     const int32_t* sorted;
     const double4* sortedPos;
     int32_t maxStaged;  ///< the sentinel record sits at sentinelSlot(maxStaged)
+    double sentinel;    ///< at (sentinel, sentinel, sentinel)
 };
 
 // Branch-free pair terms of one chunk of eight entries, written stage by stage over the
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p
         cur = reinterpret_cast<const uint4*>(tl.nbr)[s];
     }
     stageTile<ONE>(ti, tl.sortedPos, st);
-    putSentinel<ONE>(st, tl.maxStaged, 0.0);
+    putSentinel<ONE>(st, tl.maxStaged, tl.sentinel, 0.0);
     __syncthreads();
     double e = 0.0, w = 0.0;
     int n = 0;
@@ -712,7 +713,7 @@ bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_p
             if (!fresh) vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
-                               vl->bin.sortedPos.p, vl->maxStaged};
+                               vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel};
             const Pre p1 = lj->table[0];
             const bool one = lj->table.size() == 1;
             const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) *
@@ -845,7 +846,7 @@ int mrmd_lj_apply_noalloc(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, dou
         LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
                  int(lj->typeStride), lj->rcSqr, mrmd_pred2{}, 0};
         const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
-                           vl->bin.sortedPos.p, vl->maxStaged};
+                           vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel};
         const Pre p1 = lj->table[0];
         const bool one = lj->table.size() == 1;
         bool cap = false;

@@ -79,10 +79,12 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         else
             MD_CALL(mrmd_halo_update(gl, a, box));
         vl->sortedFresh = fresh;
-        if (ljAssignable(lj))
+        if (ljAssignable(lj) && vl->begin == 0 && vl->end == a->nlocal)
         {
             // the force kernel stores the rows' forces (0.0 + F, what zeroing and adding
-            // leave); only the ghost records need their zero
+            // leave); only the ghost records need their zero.  A caller's list whose rows are
+            // not all local atoms (possible before the loop's first rebuild) takes the
+            // fill-and-add path, so that no local atom keeps an old force
             MD_CALL(fillForceRange(a, a->nlocal, a->nlocal + a->nghost));
             MD_CALL(ljApplyAssign(lj, a, vl));
         }

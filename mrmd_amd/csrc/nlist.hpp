@@ -143,15 +143,16 @@ struct Staged<false>
 };
 
 // Rows are padded to whole chunks with a sentinel entry: the LDS slot just past the largest
-// staged neighbourhood of the list, which every tiled force kernel fills with a record far
-// from the box, so a padding pair lies beyond any cutoff and needs no test of its own.
-constexpr double SENTINEL_COORD = 1.0e6;
+// staged neighbourhood of the list, which every tiled force kernel fills with a record at
+// (c, c, c), c = the largest upper grid bound + SENTINEL_GAP + 4 r_n (the list's
+// `sentinel`): every point of the grid lies more than SENTINEL_GAP + 4 r_n from it along each
+// axis, so a padding pair lies beyond any cutoff and needs no test of its own.
+constexpr double SENTINEL_GAP = 1.0e3;
 __host__ __device__ inline int sentinelSlot(int32_t maxStaged) { return maxStaged > 1 ? maxStaged : 1; }
 template <bool XYZ>
-__device__ inline void putSentinel(void* st, int32_t maxStaged, double w)
+__device__ inline void putSentinel(void* st, int32_t maxStaged, double coord, double w)
 {
-    if (threadIdx.x == 0)
-        Staged<XYZ>::put(st, sentinelSlot(maxStaged), make_double4(SENTINEL_COORD, SENTINEL_COORD, SENTINEL_COORD, w));
+    if (threadIdx.x == 0) Staged<XYZ>::put(st, sentinelSlot(maxStaged), make_double4(coord, coord, coord, w));
 }
 
 // copy the tile's neighbourhood (contiguous runs of sorted slots) into LDS; the copy is
@@ -191,6 +192,8 @@ struct mrmd_verlet_list
     int range = 1;
     int32_t capFull = 0;
     int32_t maxFull = 0;
+    double sentinel = 0;        ///< coordinate of the padding entries' record (see putSentinel)
+    bool forceWide = false;     ///< build with 64-bit entry offsets even for small lists
 
     // ---- tiled form (built on the device; what the force kernels read) ----
     int tileK = 0;              ///< cells per tile along z (0 = choose from the density)

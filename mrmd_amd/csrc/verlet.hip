@@ -594,6 +594,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     const double gs = ratio * rn;
     const double delta[3] = {gs, gs, gs};
     const GridD g = makeGrid(gmin, gmax, delta);
+    vl->sentinel = std::max({gmax[0], gmax[1], gmax[2]}) + SENTINEL_GAP + 4.0 * rn;
     vl->bin.build(ctx, pos, 0, nall, g, true, true);
     rowRegion(vl);
     vl->globalValid = false;
@@ -656,7 +657,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                                    vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
                                    vl->capFull, vl->cnt16.p, dmax, stagedCap);
             };
-            const bool wide = int64_t(vl->capFull) * vl->strideS >= (int64_t(1) << 31);
+            const bool wide = vl->forceWide || int64_t(vl->capFull) * vl->strideS >= (int64_t(1) << 31);
             // plain block order: measured faster here than the XCD-contiguous order (1.54 against
             // 1.59 ms at 4M atoms), each XCD's L2 then serves a spread of the grid's columns
             if (R == 1)
@@ -761,6 +762,14 @@ int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_column
     vl->tileK = tile_cells;
     vl->tileCols = tile_columns;
     vl->block = block;
+    MRMD_API_END
+}
+
+int mrmd_verlet_set_wide_offsets(mrmd_verlet_list* vl, int on)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl, "null list");
+    vl->forceWide = on != 0;
     MRMD_API_END
 }
 

@@ -124,6 +124,31 @@ def test_tilings_give_identical_lists_and_forces(M, ctx, espp_positions, columns
     np.testing.assert_array_equal(g.download(M.FORCE, g.nlocal), f_ref)
 
 
+def test_wide_offset_build_equals_narrow(M, ctx, espp_positions):
+    """the 64-bit entry-offset form of the list kernel (which only lists of 2^31 entries or
+    more, ~20M atoms, take by themselves) builds the same rows and gives the same forces"""
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    lists = []
+    for wide in (False, True):
+        vl = M.HalfVerletList(ctx)
+        vl.set_wide_offsets(wide)
+        vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+        lists.append(vl)
+    a, ac = lists[0].full_rows(g.nlocal)
+    b, bc = lists[1].full_rows(g.nlocal)
+    np.testing.assert_array_equal(ac, bc)
+    np.testing.assert_array_equal(a, b)
+    lj = M.LennardJones(ctx, 2.5, 1.0, 1.0)
+    forces = []
+    for vl in lists:
+        g.set_force(0.0)
+        lj.apply(g, vl)
+        forces.append(g.download(M.FORCE, g.nlocal))
+    np.testing.assert_array_equal(forces[0], forces[1])
+
+
 def test_espp_integrators_bit_exact(M, ctx, espp_positions):
     rng = np.random.default_rng(5)
     n = espp_positions.shape[0]

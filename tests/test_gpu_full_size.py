@@ -81,13 +81,25 @@ def test_lj_4m_lists_energy_forces_match_oracle(M, ctx):
     assert np.abs(Fg.sum(axis=0)).max() <= 1e-12 * np.abs(Fg).sum(axis=0).max()
 
 
-def test_lj_4m_nve_conserves_energy(M, ctx):
+@pytest.mark.parametrize("native", [False, True], ids=["step-calls", "native-loop"])
+def test_lj_4m_nve_conserves_energy(M, ctx, native):
+    """200 velocity-Verlet steps at 4M atoms conserve the total energy, through the
+    per-call steps and through the native loop the benchmark times (mrmd_md_run: store-form
+    forces, skipped fold, slot-order scatter, sorted ghost update), read every 10 steps"""
     from mrmd_amd.drivers import LJSystem
     pos, L = _displaced_fcc()
     n = pos.shape[0]
     vel = maxwell_boltzmann(n, 0.72, 2024)
-    s = LJSystem(ctx, pos, vel, L)
-    e = s.run(200, 0.005, energy_every=10)
+    s = LJSystem(ctx, pos, vel, L, permute=native)
+    if native:
+        rec = []
+        for _ in range(20):
+            s.run(10, 0.005)
+            e, w = s.lj.energy_virial()
+            rec.append((e, s.atoms.kinetic_energy()))
+        e = np.array(rec)
+    else:
+        e = s.run(200, 0.005, energy_every=10)
     assert s.rebuilds >= 2
     tot = (e[:, 0] + e[:, 1]) / n
     assert np.isfinite(tot).all()
