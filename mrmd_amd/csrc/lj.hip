@@ -30,10 +30,15 @@ struct mrmd_lj
     bool shifted = false;
     bool symmetric = true;  ///< table(ti*stride+tj) == table(tj*stride+ti) for all types
     std::vector<double> cap, rc;
+    std::vector<double> sigma, eps;  ///< the parameters the table was built from
     std::vector<Pre> table;
     double rcSqr = 0.0;
     DBuf<Pre> dtable;
+    DBuf<double> dParams;  ///< sigma then epsilon per table entry (the adjoint's parameters)
     DBuf<double> ev;  ///< energy, virial on the device
+    /// adjoint work buffers, kept between calls: upstream gradient (component-major over all
+    /// records), dL/dx (component-major), per-workgroup parameter partials, row-major staging
+    DBuf<double> adjG, adjOut, adjPart, adjStage;
     DBuf<double> partials;   ///< per-workgroup (energy, virial) of the last apply
     int64_t nPartials = 0;   ///< > 0: partials not yet summed into ev
     bool evValid = true;
@@ -50,6 +55,10 @@ void mrmd_lj::upload()
 {
     dtable.reserve(ctx, int64_t(table.size()));
     HIPCHECK(hipMemcpyAsync(dtable.p, table.data(), table.size() * sizeof(Pre), hipMemcpyHostToDevice, ctx->stream));
+    const size_t nt2 = table.size();
+    dParams.reserve(ctx, int64_t(2 * nt2));
+    HIPCHECK(hipMemcpyAsync(dParams.p, sigma.data(), nt2 * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(dParams.p + nt2, eps.data(), nt2 * 8, hipMemcpyHostToDevice, ctx->stream));
     sync(ctx);
     symmetric = true;
     for (int64_t a = 0; a < numTypes; ++a)
@@ -94,6 +103,15 @@ struct TiledList
     int32_t maxStaged;  ///< the sentinel record sits at sentinelSlot(maxStaged)
     double sentinel;    ///< at (sentinel, sentinel, sentinel)
 };
+
+// 1/d^2 outside the cutoff made negligible with one select: the high word cleared leaves a
+// value below 2^-1022 whose square underflows to zero, so every term of the pair (f2^3, its
+// products and the force) comes out zero as from a zeroed reciprocal, with the same signs
+__device__ inline double maskReciprocal(double f2, bool in)
+{
+    const long long b = __double_as_longlong(f2);
+    return __hiloint2double(in ? int(b >> 32) : 0, int(b));
+}
 
 // Branch-free pair terms of one chunk of eight entries, written stage by stage over the
 // eight pairs so that the eight dependency chains (LDS read, distance, reciprocal, force)
@@ -152,7 +170,7 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
         {
             bool in = d2[u] <= a.rcSqr;
             if (PRED) in = in && ty[u] >= 0.0;
-            const double f2 = in ? frac2[u] : 0.0;
+            const double f2 = maskReciprocal(frac2[u], in);
             const double f6 = f2 * f2 * f2;
             const double g = f6 * (p1.ff1 * f6 - p1.ff2);
             const double ff = g * f2;
@@ -161,7 +179,8 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
             fz += dz[u] * ff;
             e += f6;
             w += g;
-            n += int(in);
+            unsigned carry;  // the count takes the comparison as a carry (one add)
+            n = int(__builtin_addc(unsigned(n), 0u, unsigned(in), &carry));
         }
         return;
     }
@@ -578,6 +597,8 @@ int mrmd_lj_create(mrmd_ctx* c, const double* cap, const double* rc, const doubl
     const int64_t nt2 = numTypes * numTypes;
     lj->cap.assign(cap, cap + nt2);
     lj->rc.assign(rc, rc + nt2);
+    lj->sigma.assign(sigma, sigma + nt2);
+    lj->eps.assign(eps, eps + nt2);
     buildTable(lj->table, cap, rc, sigma, eps, nt2, shifted != 0);
     double rcMax = rc[0];
     for (int64_t k = 0; k < nt2; ++k) rcMax = std::max(rcMax, rc[k]);
@@ -610,7 +631,10 @@ int mrmd_lj_set_params(mrmd_lj* lj, const double* sigma, const double* eps)
 {
     MRMD_API_BEGIN(lj ? lj->ctx : nullptr)
     REQUIRE(lj && sigma && eps, "invalid arguments");
-    buildTable(lj->table, lj->cap.data(), lj->rc.data(), sigma, eps, int64_t(lj->table.size()), lj->shifted);
+    const int64_t nt2 = int64_t(lj->table.size());
+    buildTable(lj->table, lj->cap.data(), lj->rc.data(), sigma, eps, nt2, lj->shifted);
+    lj->sigma.assign(sigma, sigma + nt2);
+    lj->eps.assign(eps, eps + nt2);
     lj->upload();
     MRMD_API_END
 }
@@ -922,37 +946,26 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
     const int64_t rows = std::min<int64_t>(nl, vl->end);
     vl->ensureGlobal();
     const int nt2 = int(lj->table.size());
-    // recover sigma / epsilon of each pair from the table: ef2 = 4 eps s^6, ef1 = 4 eps s^12
-    std::vector<double> sig(static_cast<size_t>(nt2)), ep(static_cast<size_t>(nt2));
-    for (int k = 0; k < nt2; ++k)
-    {
-        const auto& p = lj->table[size_t(k)];
-        const double s6 = p.ef1 / p.ef2;
-        sig[size_t(k)] = std::pow(s6, 1.0 / 6.0);
-        ep[size_t(k)] = p.ef2 * p.ef2 / (4.0 * p.ef1);
-    }
     const int64_t gstride = std::max<int64_t>(nall, 1);
-    DBuf<double> g, out, pg, sd;
-    g.reserve(ctx, 3 * gstride);
-    out.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
+    const int64_t xs = std::max<int64_t>(nl, 1);
+    lj->adjG.ensure(ctx, 3 * gstride);
+    lj->adjOut.ensure(ctx, 3 * xs);
+    lj->adjStage.ensure(ctx, 3 * xs);
     const unsigned adjGrid = gridFor(std::max<int64_t>(rows, 1) * ADJ_LANES, ADJ_BLOCK);
-    pg.reserve(ctx, int64_t(adjGrid + 1) * 2 * nt2);
-    double* pgSum = pg.p + int64_t(adjGrid) * 2 * nt2;
-    sd.reserve(ctx, 2 * nt2);
-    HIPCHECK(hipMemcpyAsync(sd.p, sig.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpyAsync(sd.p + nt2, ep.data(), size_t(nt2) * 8, hipMemcpyHostToDevice, ctx->stream));
-    // upstream: row-major n x 3 -> component-major with stride gstride
-    {
-        DBuf<double> tmp;
-        tmp.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
-        HIPCHECK(hipMemcpyAsync(tmp.p, dLdF, size_t(3 * nl) * 8,
-                                device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
-        if (nl) hipLaunchKernelGGL(k_rowMajorToSoA, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, tmp.p, g.p, nl, gstride);
-        if (nall > nl)
-            hipLaunchKernelGGL(k_expandGhostUpstream, dim3(gridFor(nall - nl)), dim3(BLOCK), 0, ctx->stream, g.p, gstride,
-                               mrmd_hip::ghostCorrDevice(gl), nl, nall);
-        sync(ctx);
-    }
+    lj->adjPart.ensure(ctx, int64_t(adjGrid + 1) * 2 * nt2);
+    double* const g = lj->adjG.p;
+    double* const out = lj->adjOut.p;
+    double* const stage = lj->adjStage.p;
+    double* const pg = lj->adjPart.p;
+    double* const pgSum = pg + int64_t(adjGrid) * 2 * nt2;
+    // upstream: row-major n x 3 -> component-major with stride gstride (ghost records take
+    // their real atom's)
+    HIPCHECK(hipMemcpyAsync(stage, dLdF, size_t(3 * nl) * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+    if (nl) hipLaunchKernelGGL(k_rowMajorToSoA, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, stage, g, nl, gstride);
+    if (nall > nl)
+        hipLaunchKernelGGL(k_expandGhostUpstream, dim3(gridFor(nall - nl)), dim3(BLOCK), 0, ctx->stream, g, gstride,
+                           mrmd_hip::ghostCorrDevice(gl), nl, nall);
     hipLaunchKernelGGL(k_zeroD, dim3(1), dim3(BLOCK), 0, ctx->stream, pgSum, int64_t(2 * nt2));
     LJArgs a{at->pos.p, nullptr, at->cap, 0, rows, vl->stride, lj->dtable.p, nt2, int(lj->typeStride), lj->rcSqr,
              mrmd_pred2{}, 0};
@@ -962,25 +975,22 @@ extern "C" int mrmd_lj_adjoint(mrmd_lj* lj, const mrmd_atoms* at, mrmd_verlet_li
         bool cap = false;
         for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
         const dim3 grid(adjGrid), blk(ADJ_BLOCK);
-        const int64_t xs = std::max<int64_t>(nl, 1);
+        // sigma and epsilon of each pair from the parameters the table was built from
+        const double* sd = lj->dParams.p;
         auto launch = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, grid, blk, 0, ctx->stream, a, vl->nbr.p, vl->cnt.p, g.p, gstride, sd.p,
-                               sd.p + nt2, out.p, xs, pg.p);
+            hipLaunchKernelGGL(kernel, grid, blk, 0, ctx->stream, a, vl->nbr.p, vl->cnt.p, g, gstride, sd, sd + nt2,
+                               out, xs, pg);
         };
         if (nt2 <= 4)
             cap ? launch(k_ljAdjoint<true, true>) : launch(k_ljAdjoint<true, false>);
         else
             cap ? launch(k_ljAdjoint<false, true>) : launch(k_ljAdjoint<false, false>);
-        hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, pg.p, int64_t(adjGrid), 2 * nt2,
-                           pgSum);
+        hipLaunchKernelGGL(k_sumPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, pg, int64_t(adjGrid), 2 * nt2, pgSum);
     }
     checkLaunch();
-    // outputs
-    DBuf<double> rm;
-    rm.reserve(ctx, 3 * std::max<int64_t>(nl, 1));
-    if (nl) hipLaunchKernelGGL(k_soaToRowMajor, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, out.p, rm.p, nl,
-                               std::max<int64_t>(nl, 1));
-    HIPCHECK(hipMemcpyAsync(dLdx, rm.p, size_t(3 * nl) * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+    // outputs: dL/dx back to row-major, the parameter sums to the host (one synchronisation)
+    if (nl) hipLaunchKernelGGL(k_soaToRowMajor, dim3(gridFor(nl)), dim3(BLOCK), 0, ctx->stream, out, stage, nl, xs);
+    HIPCHECK(hipMemcpyAsync(dLdx, stage, size_t(3 * nl) * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                             ctx->stream));
     std::vector<double> hp(static_cast<size_t>(2 * nt2));
     HIPCHECK(hipMemcpyAsync(hp.data(), pgSum, size_t(2 * nt2) * 8, hipMemcpyDeviceToHost, ctx->stream));
