@@ -275,6 +275,12 @@ int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_column
  * entry-offset form of the build kernel, which otherwise serves only lists of 2^31 entries
  * or more (about 20M atoms); the list is the same either way. */
 int mrmd_verlet_set_wide_offsets(mrmd_verlet_list* vl, int on);
+/* No counterpart in the reference: the most records a tile may stage in LDS (1..2048; 0 =
+ * 2048).  The build shrinks the tile height, then its columns, until every tile fits; when
+ * even one cell column by one cell does not (dense inputs), each tile stages its
+ * neighbourhood in a slice of global memory that the list build and force kernels use in
+ * place of LDS (same lists, same bits).  A small limit forces that path in tests. */
+int mrmd_verlet_set_stage_limit(mrmd_verlet_list* vl, int limit);
 /* the tiling in use: cells per tile, columns, workgroup size, largest staged neighbourhood */
 int mrmd_verlet_get_tiling(const mrmd_verlet_list* vl, int* tile_cells, int* tile_columns, int* block,
                            int* max_staged);

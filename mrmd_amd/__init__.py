@@ -403,6 +403,11 @@ class HalfVerletList:
         workgroup size of the tiled list build and force kernel; applies from the next build"""
         self.ctx.check(_L.mrmd_verlet_set_tiling(self.h, int(tile_cells), int(tile_columns), int(block)))
 
+    def set_stage_limit(self, limit=0):
+        """most records a tile stages in LDS (0 = 2048); a tile that does not fit even at one
+        cell column by one cell stages in global memory (dense inputs)"""
+        self.ctx.check(_L.mrmd_verlet_set_stage_limit(self.h, int(limit)))
+
     def set_wide_offsets(self, on=True):
         """build with the 64-bit entry-offset form of the list kernel (a test knob: that form
         otherwise serves only lists of 2^31 entries or more); the list is the same"""

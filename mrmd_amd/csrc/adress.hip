@@ -882,7 +882,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     // the tiled form serves single-type tables over lists whose molecules are single atoms
     // (checked when the list was built, valid while the molecules' layout is unchanged)
     const bool tiled = s->numTypes == 1 && vl->monoOf == m && vl->monoVersion == m->layoutVersion &&
-                       vl->ntiles > 0 && vl->nrows == m->nlocal + m->nghost;
+                       vl->ntiles > 0 && vl->nrows == m->nlocal + m->nghost && !vl->globalStage;
     if (!tiled) vl->ensureHalf();
     const unsigned grid = tiled ? unsigned(vl->ntiles) : gridFor(rows);
     double* part = static_cast<double*>(ctxScratch(ctx, 1, size_t(grid + 1) * sizeof(double)));

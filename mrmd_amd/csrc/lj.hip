@@ -102,6 +102,8 @@ struct TiledList
     const double4* sortedPos;
     int32_t maxStaged;  ///< the sentinel record sits at sentinelSlot(maxStaged)
     double sentinel;    ///< at (sentinel, sentinel, sentinel)
+    double4* gstage;    ///< the list's per-tile global staging (dense inputs), or null: LDS
+    int64_t gstageRecs; ///< records per tile slice of gstage
 };
 
 // 1/d^2 outside the cutoff made negligible with one select: the high word cleared leaves a
@@ -240,14 +242,17 @@ __device__ inline void ljChunk(const LJArgs& a, const Pre* tab, const Pre& p1, i
 
 // ASSIGN: the forces of the rows are stored as 0.0 + F instead of added (the caller zeroed
 // nothing there; 0.0 + F is what zeroing and adding leave, -0.0 included)
-template <bool PRED, bool ONE, bool CAP, bool ASSIGN = false>
+// GS: the neighbourhood is staged in the tile's slice of the list's global staging buffer
+// (dense inputs whose tiles exceed the LDS) instead of LDS
+template <bool PRED, bool ONE, bool CAP, bool ASSIGN = false, bool GS = false>
 __global__ __launch_bounds__(BLOCK) void k_ljTiled(LJArgs a, TiledList tl, Pre p1, double* partials)
 {
-    extern __shared__ double4 st[];
+    extern __shared__ double4 stLds[];
     __shared__ Pre tab[ONE ? 1 : MAX_TYPES2];
     __shared__ TileInfo ti;
     __shared__ double red[BLOCK / WAVE];
     const unsigned blk = xcdBlock();
+    double4* const st = GS ? tl.gstage + int64_t(blk) * tl.gstageRecs : stLds;
     if (!ONE)
         for (int t = threadIdx.x; t < a.nt2; t += blockDim.x) tab[t] = a.table[t];
     if (threadIdx.x == 0) ti = tl.tiles[blk];
@@ -737,7 +742,8 @@ bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_p
             if (!fresh) vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
-                               vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel};
+                               vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel,
+                               vl->globalStage ? vl->gstage.p : nullptr, vl->gstageRecs};
             const Pre p1 = lj->table[0];
             const bool one = lj->table.size() == 1;
             const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) *
@@ -745,23 +751,28 @@ bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_p
             const dim3 g(grid), b(vl->block);
             bool cap = false;
             for (const auto& p : lj->table) cap = cap || p.capDistSqr > 0.0;
-            auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, partials); };
+            auto launch2 = [&](auto kernel, auto kernelGS) {
+                if (vl->globalStage)
+                    hipLaunchKernelGGL(kernelGS, g, b, 0, ctx->stream, a, tl, p1, partials);
+                else
+                    hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, partials);
+            };
             const int variant = (a.usePred ? 4 : 0) | (one ? 2 : 0) | (cap ? 1 : 0);
             if (assign && variant == 2)
             {
-                launch(k_ljTiled<false, true, false, true>);
+                launch2(k_ljTiled<false, true, false, true, false>, k_ljTiled<false, true, false, true, true>);
                 assigned = true;
             }
             else switch (variant)
             {
-                case 0: launch(k_ljTiled<false, false, false>); break;
-                case 1: launch(k_ljTiled<false, false, true>); break;
-                case 2: launch(k_ljTiled<false, true, false>); break;
-                case 3: launch(k_ljTiled<false, true, true>); break;
-                case 4: launch(k_ljTiled<true, false, false>); break;
-                case 5: launch(k_ljTiled<true, false, true>); break;
-                case 6: launch(k_ljTiled<true, true, false>); break;
-                default: launch(k_ljTiled<true, true, true>); break;
+                case 0: launch2(k_ljTiled<false, false, false, false, false>, k_ljTiled<false, false, false, false, true>); break;
+                case 1: launch2(k_ljTiled<false, false, true, false, false>, k_ljTiled<false, false, true, false, true>); break;
+                case 2: launch2(k_ljTiled<false, true, false, false, false>, k_ljTiled<false, true, false, false, true>); break;
+                case 3: launch2(k_ljTiled<false, true, true, false, false>, k_ljTiled<false, true, true, false, true>); break;
+                case 4: launch2(k_ljTiled<true, false, false, false, false>, k_ljTiled<true, false, false, false, true>); break;
+                case 5: launch2(k_ljTiled<true, false, true, false, false>, k_ljTiled<true, false, true, false, true>); break;
+                case 6: launch2(k_ljTiled<true, true, false, false, false>, k_ljTiled<true, true, false, false, true>); break;
+                default: launch2(k_ljTiled<true, true, true, false, false>, k_ljTiled<true, true, true, false, true>); break;
             }
         }
         else
@@ -870,20 +881,26 @@ int mrmd_lj_apply_noalloc(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, dou
         LJArgs a{at->pos.p, at->force.p, at->cap, 0, rows, 0, lj->dtable.p, int(lj->table.size()),
                  int(lj->typeStride), lj->rcSqr, mrmd_pred2{}, 0};
         const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
-                           vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel};
+                           vl->bin.sortedPos.p, vl->maxStaged, vl->sentinel,
+                               vl->globalStage ? vl->gstage.p : nullptr, vl->gstageRecs};
         const Pre p1 = lj->table[0];
         const bool one = lj->table.size() == 1;
         bool cap = false;
         for (const auto& q : lj->table) cap = cap || q.capDistSqr > 0.0;
         const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) * (one ? Staged<true>::bytes : Staged<false>::bytes);
         const dim3 g(unsigned(vl->ntiles)), b(vl->block);
-        auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, part); };
+        auto launch2 = [&](auto kernel, auto kernelGS) {
+            if (vl->globalStage)
+                hipLaunchKernelGGL(kernelGS, g, b, 0, ctx->stream, a, tl, p1, part);
+            else
+                hipLaunchKernelGGL(kernel, g, b, lds, ctx->stream, a, tl, p1, part);
+        };
         switch ((one ? 2 : 0) | (cap ? 1 : 0))
         {
-            case 0: launch(k_ljTiled<false, false, false>); break;
-            case 1: launch(k_ljTiled<false, false, true>); break;
-            case 2: launch(k_ljTiled<false, true, false>); break;
-            default: launch(k_ljTiled<false, true, true>); break;
+            case 0: launch2(k_ljTiled<false, false, false, false, false>, k_ljTiled<false, false, false, false, true>); break;
+            case 1: launch2(k_ljTiled<false, false, true, false, false>, k_ljTiled<false, false, true, false, true>); break;
+            case 2: launch2(k_ljTiled<false, true, false, false, false>, k_ljTiled<false, true, false, false, true>); break;
+            default: launch2(k_ljTiled<false, true, true, false, false>, k_ljTiled<false, true, true, false, true>); break;
         }
         hipLaunchKernelGGL(k_sumPartialsTo, dim3(1), dim3(BLOCK), 0, ctx->stream, part, vl->ntiles, ev_dev, lj->ev.p);
     }

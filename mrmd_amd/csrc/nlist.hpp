@@ -194,6 +194,14 @@ struct mrmd_verlet_list
     int32_t maxFull = 0;
     double sentinel = 0;        ///< coordinate of the padding entries' record (see putSentinel)
     bool forceWide = false;     ///< build with 64-bit entry offsets even for small lists
+    int stageLimit = mrmd_hip::STAGE_MAX;  ///< staged records a tile may hold in LDS
+    /// tiles whose neighbourhood exceeds stageLimit even at one cell column by one cell
+    /// (dense inputs) stage it in a slice of global memory per tile instead: gstage holds
+    /// ntiles x gstageRecs records of 32 bytes, which the list build and force kernels use
+    /// in place of LDS (same entries, same bits)
+    bool globalStage = false;
+    int64_t gstageRecs = 0;
+    mrmd_hip::DBuf<double4> gstage;
 
     // ---- tiled form (built on the device; what the force kernels read) ----
     int tileK = 0;              ///< cells per tile along z (0 = choose from the density)

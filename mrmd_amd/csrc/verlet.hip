@@ -162,7 +162,7 @@ constexpr int STAGE_PAD = 4;
 // WIDE: list offsets in 64-bit arithmetic (lists of 2^31 entries or more); 32-bit otherwise
 // XCD: tiles in the chip's XCD-contiguous order (neighbouring tiles, which share most of
 // their staged records, on one XCD and its L2); otherwise plain block order
-template <int R, bool WIDE, bool XCD>
+template <int R, bool WIDE, bool XCD, bool GS>
 __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict__ sortedPos,
                                                        const int32_t* __restrict__ sorted,
                                                        const int32_t* __restrict__ cellOf,
@@ -170,19 +170,22 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                                                        const TileInfo* __restrict__ tiles, TileGeom tg, GridD g,
                                                        int64_t rowLo, int64_t rowHi, double rsq, uint16_t* nbr,
                                                        int64_t strideS, int cap, int32_t* cnt, int32_t* maxCnt,
-                                                       int stagedCap)
+                                                       int stagedCap, double4* gstage, int64_t gstageRecs)
 {
     constexpr int S = 2 * R + 1;
     extern __shared__ double4 stRaw[];
-    double* const st = reinterpret_cast<double*>(stRaw);  // {x, y, z} records
     __shared__ TileInfo ti;
+    const int64_t t = XCD ? int64_t(xcdBlock()) : int64_t(blockIdx.x);
+    // the staged {x, y, z} records: in LDS, or (GS, dense inputs) in this tile's slice of
+    // global memory, with the per-cell start table alone in LDS
+    double* const st = GS ? reinterpret_cast<double*>(gstage + t * gstageRecs) : reinterpret_cast<double*>(stRaw);
     // after the staged records and their pad: the LDS slot of the first record of cell
     // kl + z of range r, cstart[r * cz + z] with cz = K + 2R + 1; entry z = nk closes the
     // range (ranges outside the grid are empty).  Sized to the tile height, so that short
     // tiles leave LDS for more workgroups.
     const int cz = tg.K + 2 * R + 1;
-    int* cstart = reinterpret_cast<int*>(st + 3 * (int64_t(stagedCap) + STAGE_PAD));
-    const int64_t t = XCD ? int64_t(xcdBlock()) : int64_t(blockIdx.x);
+    int* cstart = GS ? reinterpret_cast<int*>(stRaw)
+                     : reinterpret_cast<int*>(reinterpret_cast<double*>(stRaw) + 3 * (int64_t(stagedCap) + STAGE_PAD));
     if (threadIdx.x == 0) ti = tiles[t];
     __syncthreads();
     if (ti.nrows == 0) return;
@@ -616,18 +619,30 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
         K = std::max(1, std::min(K, g.n[2]));
     }
     K = std::min(K, MAX_SPAN - 2 * R);
+    vl->globalStage = false;
     for (;;)
     {
         vl->tileCols = a;
         setupTiles(vl, K);
-        if (vl->maxStaged <= STAGE_MAX) break;
+        if (vl->maxStaged <= vl->stageLimit) break;
         if (K > 1)
             K = std::max(1, K / 2);
         else if (a > 1)
             --a;
         else
-            fail(MRMD_ERR_UNSUP, "a tile neighbourhood holds %d records (limit %d): density too high for the tiled list",
-                 vl->maxStaged, STAGE_MAX);
+        {
+            // dense input: even one cell column by one cell exceeds the LDS limit, so each
+            // tile stages its neighbourhood in its own slice of global memory
+            vl->globalStage = true;
+            break;
+        }
+    }
+    if (vl->globalStage)
+    {
+        REQUIRE(vl->maxStaged < 32767, "a tile neighbourhood of %d records exceeds the 15-bit entry slots",
+                vl->maxStaged);
+        vl->gstageRecs = (int64_t(vl->maxStaged) + STAGE_PAD + 1 + 3) / 4 * 4;
+        vl->gstage.ensure(ctx, std::max<int64_t>(vl->ntiles, 1) * vl->gstageRecs);
     }
     vl->strideS = std::max<int64_t>(64, (nall + 63) / 64 * 64);
     vl->cnt16.ensure(ctx, vl->strideS);
@@ -642,7 +657,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     bool mono = false;
     // staged records, then the per-cell start table of (a + 2R)^2 ranges x (K + 2R + 1)
     const int stagedCap = std::max(vl->maxStaged, 1);
-    const size_t lds = size_t(stagedCap + STAGE_PAD) * Staged<true>::bytes +
+    const size_t lds = (vl->globalStage ? 0 : size_t(stagedCap + STAGE_PAD) * Staged<true>::bytes) +
                        size_t((vl->tileCols + 2 * R) * (vl->tileCols + 2 * R) * (vl->tileK + 2 * R + 1)) * sizeof(int);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
@@ -655,15 +670,22 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                 hipLaunchKernelGGL(kernel, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
                                    vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
                                    vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax, stagedCap);
+                                   vl->capFull, vl->cnt16.p, dmax, stagedCap, vl->gstage.p, vl->gstageRecs);
             };
             const bool wide = vl->forceWide || int64_t(vl->capFull) * vl->strideS >= (int64_t(1) << 31);
             // plain block order: measured faster here than the XCD-contiguous order (1.54 against
             // 1.59 ms at 4M atoms), each XCD's L2 then serves a spread of the grid's columns
-            if (R == 1)
-                wide ? launch(k_verletTiled<1, true, false>) : launch(k_verletTiled<1, false, false>);
+            if (vl->globalStage)
+            {
+                if (R == 1)
+                    wide ? launch(k_verletTiled<1, true, false, true>) : launch(k_verletTiled<1, false, false, true>);
+                else
+                    wide ? launch(k_verletTiled<2, true, false, true>) : launch(k_verletTiled<2, false, false, true>);
+            }
+            else if (R == 1)
+                wide ? launch(k_verletTiled<1, true, false, false>) : launch(k_verletTiled<1, false, false, false>);
             else
-                wide ? launch(k_verletTiled<2, true, false>) : launch(k_verletTiled<2, false, false>);
+                wide ? launch(k_verletTiled<2, true, false, false>) : launch(k_verletTiled<2, false, false, false>);
         }
         checkLaunch();
         int32_t hp[2];
@@ -762,6 +784,14 @@ int mrmd_verlet_set_tiling(mrmd_verlet_list* vl, int tile_cells, int tile_column
     vl->tileK = tile_cells;
     vl->tileCols = tile_columns;
     vl->block = block;
+    MRMD_API_END
+}
+
+int mrmd_verlet_set_stage_limit(mrmd_verlet_list* vl, int limit)
+{
+    MRMD_API_BEGIN(vl ? vl->ctx : nullptr)
+    REQUIRE(vl && limit >= 0 && limit <= STAGE_MAX, "the stage limit must lie in [0, %d]", STAGE_MAX);
+    vl->stageLimit = limit > 0 ? limit : STAGE_MAX;
     MRMD_API_END
 }
 

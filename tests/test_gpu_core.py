@@ -124,6 +124,40 @@ def test_tilings_give_identical_lists_and_forces(M, ctx, espp_positions, columns
     np.testing.assert_array_equal(g.download(M.FORCE, g.nlocal), f_ref)
 
 
+def test_global_staging_equals_lds_staging(M, ctx, espp_positions):
+    """tiles whose neighbourhood exceeds the LDS limit even at one cell column by one cell
+    (dense inputs) stage it in global memory: forced here with a stage limit of 64 records,
+    the rows (entries and order) and the forces equal those of LDS staging at the same tile
+    shape bit for bit"""
+    gbox, cbox, g, c = _espp(M, ctx, espp_positions)
+    gl = M.GhostLayer(ctx)
+    gl.create_ghost_atoms(g, gbox)
+    lists = []
+    for limit in (0, 64):
+        vl = M.HalfVerletList(ctx)
+        vl.set_tiling(1, 256, 1)
+        vl.set_stage_limit(limit)
+        vl.build(g, 0, g.nlocal, 2.8, 1.0, [-2.8] * 3, [L_ESPP + 2.8] * 3)
+        t = vl.tiling()
+        assert (t["tile_columns"], t["tile_cells"]) == (1, 1)
+        lists.append(vl)
+    assert lists[1].tiling()["max_staged"] > 64  # only global staging leaves a tile this large
+    assert lists[1].num_pairs() == 1310403
+    a, ac = lists[0].full_rows(g.nlocal)
+    b, bc = lists[1].full_rows(g.nlocal)
+    np.testing.assert_array_equal(ac, bc)
+    np.testing.assert_array_equal(a, b)
+    forces = []
+    for vl in lists:
+        for lj in (M.LennardJones(ctx, 2.5, 1.0, 1.0), M.LennardJones(ctx, 2.5, 1.0, 1.0, capping_distance=0.9)):
+            g.set_force(0.0)
+            lj.apply(g, vl)
+            forces.append((g.download(M.FORCE, g.nlocal), lj.energy_virial()))
+    for k in range(2):
+        np.testing.assert_array_equal(forces[k][0], forces[2 + k][0])
+        assert forces[k][1] == forces[2 + k][1]
+
+
 def test_wide_offset_build_equals_narrow(M, ctx, espp_positions):
     """the 64-bit entry-offset form of the list kernel (which only lists of 2^31 entries or
     more, ~20M atoms, take by themselves) builds the same rows and gives the same forces"""
