@@ -265,7 +265,8 @@ def adress_slab(ctx, steps, warm=50, profile_steps=100):
     elapsed = time.perf_counter() - t0
     rebuilds = s.rebuilds - r0
     names = ("lj_idealgas_tiled", "lj_idealgas", "verlet_build", "post_pre", "langevin_pre", "vv_post", "halo_update",
-             "halo_fold", "update_molecules", "contribute_molecule_force", "tf_sample", "tf_apply")
+             "halo_fold", "update_molecules", "contribute_molecule_force", "tf_sample", "tf_apply", "adress_prepare",
+             "adress_collect")
     ctx.set_profiling(True)
     ctx.reset_stats()
     s.run(profile_steps, 0.005)

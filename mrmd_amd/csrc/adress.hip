@@ -732,6 +732,10 @@ struct mrmd_lj_idealgas
     DBuf<Pre> dtable;
     DBuf<double> hist;  ///< comp | count | mean, each COMP_BINS x numTypes
     DBuf<double4> igPos;  ///< tiled form: sorted-slot atom records
+    /// set when a time loop's fused pass has already written igPos for the next run (for the
+    /// list `igPosList` at its current build); the run then skips its gather
+    bool igPosFresh = false;
+    const mrmd_verlet_list* igPosList = nullptr;
     int queueCap = IG_QUEUE;
     int64_t runCounter = 0;
     int64_t samplingInterval = 200;
@@ -901,9 +905,12 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
     if (rows > 0 && tiled)
     {
         KTimer t(ctx, "lj_idealgas_tiled");
-        s->igPos.ensure(ctx, vl->nrows);
-        hipLaunchKernelGGL(k_igGather, dim3(gridFor(vl->nrows)), dim3(BLOCK), 0, ctx->stream, m->view(), a->pos.p,
-                           vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
+        if (!(s->igPosFresh && s->igPosList == vl))
+        {
+            s->igPos.ensure(ctx, vl->nrows);
+            hipLaunchKernelGGL(k_igGather, dim3(gridFor(vl->nrows)), dim3(BLOCK), 0, ctx->stream, m->view(), a->pos.p,
+                               vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
+        }
         const IGTiled tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p, s->igPos.p, rows,
                          s->queueCap, vl->maxStaged, vl->sentinel};
         const Pre p = s->table[0];
@@ -928,6 +935,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
         hipLaunchKernelGGL(k_updateMeanComp, dim3(gridFor(nh)), dim3(BLOCK), 0, ctx->stream, s->hist.p, s->hist.p + nh,
                            s->hist.p + 2 * nh, nh, 10.0);
     checkLaunch();
+    s->igPosFresh = false;
     ++s->runCounter;
     if (energy)
     {
@@ -1192,3 +1200,178 @@ int mrmd_hist_smoothen(mrmd_ctx* c, double mn, double mx, int64_t nb, int64_t nh
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Fused passes of the AdResS time loop for molecules of one atom each, molecule k = atom k
+// (createMoleculeForEachAtom; checked at every rebuild).  They compute exactly what the
+// separate actions compute, in the same expressions:
+//  * adressPrepare: UpdateGhostAtoms on the ghost records (non-rebuild steps),
+//    UpdateMolecules with the slab weighting, the zero atom and molecule forces, and the
+//    sorted-slot records the tiled LJ_IdealGas kernel reads;
+//  * adressCollectGhosts: ContributeMoleculeForceToAtoms and AccumulateForce (fold) for the
+//    ghost records (the local atoms' molecule forces are added in the fused post/pre step,
+//    or by adressContributeLocal on a loop's last step).
+namespace
+{
+__global__ void k_zeroOne(int32_t* p)
+{
+    if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ void k_notIdentity(const int64_t* offset, const int64_t* numAtoms, int64_t n, int32_t* out)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n && (offset[k] != k || numAtoms[k] != 1)) atomicAdd(out, 1);
+}
+
+struct PrepArgs
+{
+    MolView m;
+    AtomsView a;
+    int64_t nlocal, nall;
+    const int32_t* corr;  ///< ghost update from the real atoms, or null (rebuild step)
+    double diam[3];
+    mrmd_slab slab;
+    double4* igPos;        ///< sorted-slot records for the tiled kernel, or null
+    const int32_t* slotOf;
+    int64_t rowsEnd;       ///< molecules [0, rowsEnd) are rows of the list
+};
+
+__global__ void k_adressPrepare(PrepArgs g)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= g.nall) return;
+    double4 p = g.a.pos[r];
+    if (g.corr && r >= g.nlocal)
+    {
+        // UpdateGhostAtoms.cpp:31-68 (the expressions of k_ghostUpdate)
+        const double4 q = g.a.pos[g.corr[r]];
+        const double dx[3] = {p.x - q.x, p.y - q.y, p.z - q.z};
+        double x[3] = {q.x, q.y, q.z};
+        for (int d = 0; d < 3; ++d)
+            if (dx[d] > +(0.1 * g.diam[d])) x[d] += g.diam[d];
+        for (int d = 0; d < 3; ++d)
+            if (dx[d] < -(0.1 * g.diam[d])) x[d] -= g.diam[d];
+        p.x = x[0];
+        p.y = x[1];
+        p.z = x[2];
+        g.a.pos[r] = p;
+    }
+    // UpdateMolecules::update for one atom (k_updateMolecules): the centre is relMass x the
+    // atom position (summed from zero), then the slab weighting
+    const double rm = g.a.relmass[r];
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    X += p.x * rm;
+    Y += p.y * rm;
+    Z += p.z * rm;
+    double4 q = g.m.pos[r];
+    q.x = X;
+    q.y = Y;
+    q.z = Z;
+    g.m.pos[r] = q;
+    double l, md, gr;
+    slabEval(g.slab, X, l, md, gr);
+    g.m.lambda[r] = l;
+    g.m.modLambda[r] = md;
+    g.m.grad[r] = gr;
+    g.m.grad[g.m.cap + r] = 0.0;
+    g.m.grad[2 * g.m.cap + r] = 0.0;
+    for (int d = 0; d < 3; ++d)
+    {
+        g.a.force[d * g.a.cap + r] = 0.0;
+        g.m.force[d * g.m.cap + r] = 0.0;
+    }
+    // the record k_igGather writes (|lambda_mod|, negated for non-rows)
+    if (g.igPos)
+    {
+        const double la = fabs(md);
+        g.igPos[g.slotOf[r]] = make_double4(p.x, p.y, p.z, r < g.rowsEnd ? la : -la);
+    }
+}
+
+// ghost records: molecule force to the ghost atom, then the fold into its real atom
+// (k_contribute, then k_ghostFold: only non-zero components move)
+__global__ void k_adressCollectGhosts(MolView m, AtomsView a, const int32_t* corr, int64_t begin, int64_t end)
+{
+    const int64_t i = begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const double rm = a.relmass[i];
+    double f[3];
+    for (int d = 0; d < 3; ++d) f[d] = a.force[d * a.cap + i] + rm * m.force[d * m.cap + i];
+    const int32_t real = corr[i];
+    if (real == -1)
+    {
+        for (int d = 0; d < 3; ++d) a.force[d * a.cap + i] = f[d];
+        return;
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+        if (f[d] != 0.0) atomicAddD(&a.force[d * a.cap + real], f[d]);
+        a.force[d * a.cap + i] = 0.0;
+    }
+}
+}  // namespace
+
+bool mrmd_hip::adressIdentityLayout(const mrmd_molecules* m, const mrmd_atoms* a)
+{
+    mrmd_ctx* ctx = m->ctx;
+    const int64_t n = m->nlocal + m->nghost;
+    if (n != a->nlocal + a->nghost || m->nlocal != a->nlocal) return false;
+    if (n == 0) return true;
+    int32_t* cnt = static_cast<int32_t*>(ctxScratch(ctx, 2, 64)) + 24;
+    hipLaunchKernelGGL(k_zeroOne, dim3(1), dim3(64), 0, ctx->stream, cnt);
+    hipLaunchKernelGGL(k_notIdentity, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, m->offset.p, m->numAtoms.p, n, cnt);
+    checkLaunch();
+    int32_t h[1];
+    readbackI32(ctx, cnt, 1, h);
+    return h[0] == 0;
+}
+
+void mrmd_hip::adressPrepare(mrmd_atoms* a, mrmd_molecules* m, const mrmd_ghost_layer* gl, const mrmd_box* box,
+                             const mrmd_slab* slab, mrmd_lj_idealgas* ig, mrmd_verlet_list* vl)
+{
+    mrmd_ctx* ctx = a->ctx;
+    const int64_t nall = a->nlocal + a->nghost;
+    KTimer t(ctx, "adress_prepare");
+    PrepArgs g{m->view(), a->view(), a->nlocal, nall, gl ? ghostCorrDevice(gl) : nullptr, {0, 0, 0}, *slab, nullptr,
+               nullptr, std::min<int64_t>(m->nlocal, vl->end)};
+    for (int d = 0; d < 3; ++d) g.diam[d] = box->max[d] - box->min[d];
+    // the tiled LJ_IdealGas kernel's records, when that kernel will run on this list
+    const bool tiled = ig->numTypes == 1 && vl->monoOf == m && vl->monoVersion == m->layoutVersion &&
+                       vl->ntiles > 0 && vl->nrows == nall && !vl->globalStage;
+    if (tiled)
+    {
+        ig->igPos.ensure(ctx, vl->nrows);
+        g.igPos = ig->igPos.p;
+        g.slotOf = vl->ensureSlotOf();
+    }
+    if (nall > 0) hipLaunchKernelGGL(k_adressPrepare, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, g);
+    checkLaunch();
+    ig->igPosFresh = tiled;
+    ig->igPosList = vl;
+}
+
+void mrmd_hip::adressCollectGhosts(mrmd_atoms* a, mrmd_molecules* m, const mrmd_ghost_layer* gl)
+{
+    mrmd_ctx* ctx = a->ctx;
+    KTimer t(ctx, "adress_collect");
+    if (a->nghost > 0)
+        hipLaunchKernelGGL(k_adressCollectGhosts, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, ctx->stream, m->view(),
+                           a->view(), ghostCorrDevice(gl), a->nlocal, a->nlocal + a->nghost);
+    checkLaunch();
+}
+
+void mrmd_hip::adressContributeLocal(mrmd_atoms* a, mrmd_molecules* m)
+{
+    mrmd_ctx* ctx = a->ctx;
+    if (a->nlocal > 0)
+        hipLaunchKernelGGL(k_contribute, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, ctx->stream, m->view(), a->view(),
+                           a->nlocal);
+    checkLaunch();
+}
+
+void mrmd_hip::thermoForceTable(const mrmd_thermo_force* tf, const double** force, HistD* hd)
+{
+    *force = tf->force.p;
+    *hd = tf->hd;
+}

@@ -117,14 +117,30 @@ __device__ inline double coord(const double4& p, int axis)
 // (velocity Verlet, or Langevin without a predicate), bit-identical to the two calls;
 // *maxDisp = sqrt(max |dx|^2) of the pre-force part (synchronises); sortedPos (if not
 // null) receives each atom's new position at its sorted slot slotOf[i]
+struct HistD;
+// the fused post/pre step of the time loops; with adrMol (the AdResS loop, molecule k = atom
+// k) each local atom's molecule force (ContributeMoleculeForceToAtoms) and the thermodynamic
+// force of its x-bin (ThermodynamicForce::apply) are added to its force first
 int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
-                 uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr);
+                 uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr,
+                 const mrmd_molecules* adrMol = nullptr, const double* tfForce = nullptr,
+                 const HistD* tfHd = nullptr);
 // whether mrmd_lj_apply without a predicate uses the full-row kernel (no ghost forces)
 bool ljFullRows(const mrmd_lj* lj);
 // device address of the allocation-free apply's scratch array on `device` (the current
 // device); called at every context creation, looks it up once per device; false when the
 // lookup failed
 bool ljNoAllocSetup(int device);
+const int32_t* ghostCorrDevice(const mrmd_ghost_layer* gl);
+int64_t ghostCorrSize(const mrmd_ghost_layer* gl);
+// fused passes of the AdResS time loop (molecule k = atom k; see adress.hip)
+bool adressIdentityLayout(const mrmd_molecules* m, const mrmd_atoms* a);
+void adressPrepare(mrmd_atoms* a, mrmd_molecules* m, const mrmd_ghost_layer* gl, const mrmd_box* box,
+                   const mrmd_slab* slab, mrmd_lj_idealgas* ig, mrmd_verlet_list* vl);
+void adressCollectGhosts(mrmd_atoms* a, mrmd_molecules* m, const mrmd_ghost_layer* gl);
+void adressContributeLocal(mrmd_atoms* a, mrmd_molecules* m);
+// the thermodynamic force's device table and histogram geometry
+void thermoForceTable(const mrmd_thermo_force* tf, const double** force, HistD* hd);
 // whether the table has the store form (single type, no capping): ljApplyAssign then
 // computes what zeroing the forces of the list rows and mrmd_lj_apply(pred = NULL) leave
 bool ljAssignable(const mrmd_lj* lj);

@@ -8,6 +8,7 @@
 
 #include "containers.hpp"
 #include "detmath.hpp"
+#include "physics.hpp"
 
 using namespace mrmd_hip;
 
@@ -155,10 +156,22 @@ __global__ __launch_bounds__(BLOCK) void k_langevinPre(AtomsView a, int64_t n, d
 // bit-identical, with one pass over the arrays instead of two.  LANGEVIN: the pre-force
 // part is k_langevinPre<false, UNI>, otherwise k_vvPre.  sortedPos (if not null): each new
 // position also goes to the atom's sorted slot slotOf[i] of the Verlet list.
-template <bool LANGEVIN, bool UNI>
+// ADR (the AdResS loop, molecule k = atom k): the atom's molecule force times its relative
+// mass is added to F first (ContributeMoleculeForceToAtoms, ContributeMoleculeForceToAtoms.cpp:
+// 23-48), then the thermodynamic force of its x-bin to F_x (ThermodynamicForce::apply,
+// ThermodynamicForce.hpp:98-122), the sums those actions leave in the force array
+struct AdrFold
+{
+    const double* molForce;  ///< [3][molCap]
+    int64_t molCap;
+    const double* tfForce;
+    HistD hd;
+};
+
+template <bool LANGEVIN, bool UNI, bool ADR = false>
 __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
                                                    uint64_t seed, uint64_t call, LangevinFactors uf, double* partials,
-                                                   double4* sortedPos, const int32_t* slotOf)
+                                                   double4* sortedPos, const int32_t* slotOf, AdrFold adr)
 {
     __shared__ double red[BLOCK / WAVE];
     __shared__ WaveDraws draws[LANGEVIN ? BLOCK / WAVE : 1];
@@ -171,7 +184,17 @@ __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, doubl
     LangevinFactors f = uf;
     if (live)
     {
-        const double fx = a.force[i], fy = a.force[a.cap + i], fz = a.force[2 * a.cap + i];
+        double fx = a.force[i], fy = a.force[a.cap + i], fz = a.force[2 * a.cap + i];
+        if (ADR)
+        {
+            const double rm = a.relmass[i];
+            fx += rm * adr.molForce[i];
+            fy += rm * adr.molForce[adr.molCap + i];
+            fz += rm * adr.molForce[2 * adr.molCap + i];
+            const double4 q = a.pos[i];
+            const int64_t bin = getBin(q.x, adr.hd.min, adr.hd.invBinSize, adr.hd.numBins);
+            if (bin != -1) fx += adr.tfForce[bin * adr.hd.numHist + int64_t(q.w)];
+        }
         vx = a.vel[i];
         vy = a.vel[a.cap + i];
         vz = a.vel[2 * a.cap + i];
@@ -331,7 +354,8 @@ double* dispHost(mrmd_atoms* a, const double* maxDisp)
 
 // the fused post/pre step of time loops (k_postPre); synchronises for the displacement
 int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
-                           uint64_t call, double* maxDisp, double4* sortedPos, const int32_t* slotOf)
+                           uint64_t call, double* maxDisp, double4* sortedPos, const int32_t* slotOf,
+                           const mrmd_molecules* adrMol, const double* tfForce, const HistD* tfHd)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && maxDisp, "invalid arguments");
@@ -346,11 +370,20 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
         KTimer t(ctx, "post_pre");
         const bool uni = a->massUniform && a->massValue > 0.0;
         const LangevinFactors uf = uni ? langevinFactors(a->massValue, dt, zeta, temperature) : LangevinFactors{};
+        const AdrFold adr{adrMol ? adrMol->force.p : nullptr, adrMol ? adrMol->cap : 0, tfForce,
+                          tfHd ? *tfHd : HistD{}};
         auto launch = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, zeta,
-                               temperature, seed, call, uf, part, sortedPos, slotOf);
+                               temperature, seed, call, uf, part, sortedPos, slotOf, adr);
         };
-        if (langevin)
+        if (adrMol)
+        {
+            REQUIRE(langevin && tfForce && tfHd && a->maxType < tfHd->numHist && adrMol->nlocal == a->nlocal,
+                    "the fused AdResS step needs Langevin steps, a molecule per local atom and a type column per "
+                    "atom type");
+            uni ? launch(k_postPre<true, true, true>) : launch(k_postPre<true, false, true>);
+        }
+        else if (langevin)
             uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);
         else
             uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);

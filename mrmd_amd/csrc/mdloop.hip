@@ -12,6 +12,7 @@
 
 #include "containers.hpp"
 #include "nlist.hpp"
+#include "physics.hpp"
 
 using namespace mrmd_hip;
 
@@ -146,11 +147,20 @@ int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, m
     // post-force integration fused with the next step's pre-force integration, as above
     double d = 0.0;
     if (nsteps > 0) MD_CALL(mrmd_langevin_pre(a, s->dt, s->zeta, s->temperature, s->seed, s->call, nullptr, &d));
+    // molecules of one atom each, molecule k = atom k (checked now and at every rebuild):
+    // the per-record actions around LJ_IdealGas run as fused passes (adressPrepare before
+    // it, adressCollectGhosts after it, the local molecule forces and the thermodynamic force
+    // inside the post/pre step)
+    bool fused = nsteps > 0 && adressIdentityLayout(m, a);
+    const double* tfForce = nullptr;
+    HistD tfHd{};
+    thermoForceTable(tf, &tfForce, &tfHd);
     for (int64_t k = 0; k < nsteps; ++k)
     {
         ++s->call;
         s->max_disp += d;
-        if (s->max_disp >= s->skin * 0.5)
+        const bool rebuild = s->max_disp >= s->skin * 0.5;
+        if (rebuild)
         {
             s->max_disp = 0.0;
             MD_CALL(mrmd_molecules_update(m, a, nullptr));
@@ -159,24 +169,41 @@ int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, m
             MD_CALL(mrmd_molecules_update(m, a, nullptr));
             MD_CALL(mrmd_verlet_build_molecules(vl, m, 0, m->nlocal, rn, 1.0, gridMin, gridMax, 0));
             ++s->rebuilds;
+            fused = adressIdentityLayout(m, a);
         }
+        if (fused)
+            adressPrepare(a, m, rebuild ? nullptr : gl, box, slab, ig, vl);
         else
-            MD_CALL(mrmd_halo_update(gl, a, box));
-        MD_CALL(mrmd_molecules_update(m, a, slab));
-        MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
-        MD_CALL(mrmd_molecules_fill(m, MRMD_MOL_FORCE, 0.0));
+        {
+            if (!rebuild) MD_CALL(mrmd_halo_update(gl, a, box));
+            MD_CALL(mrmd_molecules_update(m, a, slab));
+            MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
+            MD_CALL(mrmd_molecules_fill(m, MRMD_MOL_FORCE, 0.0));
+        }
         MD_CALL(mrmd_ljig_run(ig, m, vl, a, k + 1 == nsteps ? &s->energy : nullptr));
-        MD_CALL(mrmd_molecules_contribute_force(m, a));
-        MD_CALL(mrmd_halo_fold_forces(gl, a));
+        if (fused)
+            adressCollectGhosts(a, m, gl);
+        else
+        {
+            MD_CALL(mrmd_molecules_contribute_force(m, a));
+            MD_CALL(mrmd_halo_fold_forces(gl, a));
+        }
         MD_CALL(mrmd_tf_sample(tf, a));
         ++s->steps;
         if (s->tf_every && s->steps % s->tf_every == 0)
             MD_CALL(mrmd_tf_update(tf, s->tf_sigma, s->tf_range, &s->tf_pred));
-        MD_CALL(mrmd_tf_apply(tf, a, nullptr));
-        if (k + 1 < nsteps)
-            MD_CALL(fusedPostPre(a, s->dt, true, s->zeta, s->temperature, s->seed, s->call, &d));
+        if (k + 1 < nsteps && fused)
+            MD_CALL(fusedPostPre(a, s->dt, true, s->zeta, s->temperature, s->seed, s->call, &d, nullptr, nullptr, m,
+                                 tfForce, &tfHd));
         else
-            MD_CALL(mrmd_vv_post(a, s->dt));
+        {
+            if (fused) adressContributeLocal(a, m);
+            MD_CALL(mrmd_tf_apply(tf, a, nullptr));
+            if (k + 1 < nsteps)
+                MD_CALL(fusedPostPre(a, s->dt, true, s->zeta, s->temperature, s->seed, s->call, &d));
+            else
+                MD_CALL(mrmd_vv_post(a, s->dt));
+        }
     }
 #undef MD_CALL
     MRMD_API_END
