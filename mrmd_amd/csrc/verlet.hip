@@ -157,7 +157,7 @@ constexpr int STAGE_PAD = 4;
 // One lane per row.  Every row scans its stencil columns in Cabana's i, j, k order with the
 // min-distance prune (the surviving cells of a column as one slot range, cut by binary
 // search to the z window of the row, the records being z-sorted), 32 candidates per pass in
-// groups of four (independent LDS reads and distances, hit bits shifted in with the carry),
+// groups of four (independent LDS reads and distances, hit bits shifted in by a funnel shift),
 // then the hits appended in slot order.
 // WIDE: list offsets in 64-bit arithmetic (lists of 2^31 entries or more); 32-bit otherwise
 // XCD: tiles in the chip's XCD-contiguous order (neighbouring tiles, which share most of
@@ -340,7 +340,11 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     // pad), their bits are cleared below.  hits (and, for the columns of the
                     // row's own x index, the sign bits of dx and the dx == 0 ties) take one bit
                     // per candidate, shifted in from the right (first candidate highest)
-                    uint32_t hits = 0, neg = 0, tie = 0;
+                    // A bit is shifted in by one funnel shift of the sign word of a double:
+                    // rsq - d^2 is negative exactly when d^2 > rsq (a rounded difference keeps
+                    // the sign of the exact one and is +0 only for equal operands), and dx
+                    // is negative exactly when its sign bit is set (xp - xn is never -0)
+                    uint32_t miss = 0, neg = 0, tie = 0;
                     const int nq = min(32, hi - qb);
                     const double* q = st + 3 * qb;
                     int m = 0;
@@ -352,11 +356,12 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                             const double dx = xpx - q[3 * (m + u)];
                             const double dy = xpy - q[3 * (m + u) + 1];
                             const double dz = xpz - q[3 * (m + u) + 2];
-                            unsigned carry;
-                            hits = __builtin_addc(hits, hits, unsigned(dx * dx + dy * dy + dz * dz <= rsq), &carry);
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            miss = __builtin_amdgcn_alignbit(miss, uint32_t(__double2hiint(rsq - r2)), 31u);
                             if (da == 0)
                             {
-                                neg = __builtin_addc(neg, neg, unsigned(dx < 0.0), &carry);
+                                unsigned carry;
+                                neg = __builtin_amdgcn_alignbit(neg, uint32_t(__double2hiint(dx)), 31u);
                                 tie = __builtin_addc(tie, tie, unsigned(dx == 0.0), &carry);
                             }
                         }
@@ -364,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     // align: candidate u at bit 31 - u; drop the candidates past the run and
                     // the row itself
                     const int sh = 32 - m;  // m is 4..32
-                    hits <<= sh;
+                    uint32_t hits = ~miss << sh;
                     hits &= ~((1u << (32 - nq)) - 1u);
                     if (qSelf >= qb && qSelf < qb + nq) hits &= ~(0x80000000u >> (qSelf - qb));
                     if (da == 0)
@@ -377,27 +382,33 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                     // yp && zn > zp)))): columns of a larger x index hold larger x (binning is
                     // monotone in x), smaller index smaller x; in the row's own x index the
                     // sign of dx decides, and a tie dx == 0 compares y and z
-                    const bool fits = c + __builtin_popcount(hits) <= cap;
-                    while (hits)
-                    {
-                        const int u = __builtin_clz(hits);
-                        hits &= 0x7fffffffu >> u;
-                        uint32_t half = da > 0 ? 1u : 0u;
-                        if (da == 0)
+                    auto appendHits = [&](auto checked) {
+                        while (hits)
                         {
-                            half = (neg << u) >> 31;
-                            if ((tie << u) & 0x80000000u)
+                            const int u = __builtin_clz(hits);
+                            hits ^= 0x80000000u >> u;
+                            uint32_t half = da > 0 ? 1u : 0u;
+                            if (da == 0)
                             {
-                                const double ny = q[3 * u + 1], nz = q[3 * u + 2];
-                                half = uint32_t((ny > xpy) | ((ny == xpy) & (nz > xpz)));
+                                half = (neg << u) >> 31;
+                                if ((tie << u) & 0x80000000u)
+                                {
+                                    const double ny = q[3 * u + 1], nz = q[3 * u + 2];
+                                    half = uint32_t((ny > xpy) | ((ny == xpy) & (nz > xpz)));
+                                }
                             }
+                            const uint16_t v = uint16_t((qb + u) | (half << 15));
+                            if (!decltype(checked)::value || c < cap)
+                                append(v);
+                            else
+                                ++c;
                         }
-                        const uint16_t v = uint16_t((qb + u) | (half << 15));
-                        if (fits || c < cap)
-                            append(v);
-                        else
-                            ++c;
-                    }
+                    };
+                    // a row that overflows the capacity counts its further hits without storing
+                    if (c + __builtin_popcount(hits) <= cap)
+                        appendHits(std::false_type{});
+                    else
+                        appendHits(std::true_type{});
                 }
             }
         }
