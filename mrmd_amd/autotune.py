@@ -52,7 +52,7 @@ def tune_lj_system(system, shapes=DEFAULT_SHAPES, blocks=DEFAULT_BLOCKS, force_r
     return best, table
 
 
-ENSEMBLE_TARGETS = (700, 1000, 1400)  # staged records per workgroup a replica's tile height aims at
+ENSEMBLE_TARGETS = (1000, 1400, 1800)  # staged records per workgroup a replica's tile height aims at
 ENSEMBLE_BLOCKS = (128, 256)
 
 

@@ -8,12 +8,15 @@
 // another: the step's decisions (rebuild or not, sampling, thermodynamic-force updates) are
 // taken on the device per replica, so a block of steps is captured once as a hipGraph.
 //
-// Periodicity without ghost records.  A tile (a run of K cells along z in one (i, j)
-// column) stages the 3 x 3 neighbouring columns x (K + 2) cells in LDS; each staged piece
-// carries the periodic image shift that brings it next to the tile, applied as x + L or
-// x - L, i.e. exactly the position a ghost record would have, so pair distances equal those
-// of the ghost-layer path bit for bit.  Two cells per axis are enough: the two images of a
-// column then come from different stencil offsets.
+// Periodicity without ghost records.  A tile (a run of K cells along z in up to 2 x 2
+// neighbouring (i, j) columns) stages the (2 + 2) x (2 + 2) columns around it x (K + 2)
+// cells in LDS; each staged piece carries the periodic image shift that brings it next to
+// the tile, applied as x + L or x - L, i.e. exactly the position a ghost record would have,
+// so pair distances equal those of the ghost-layer path bit for bit.  Two cells per axis
+// are enough: the images of a column then come from different stencil offsets.  Records
+// are z-sorted inside each cell, so a staged column run is z-sorted and the list build cuts
+// each row's candidates in a column to a z window by binary search, as the single-system
+// build does.
 //
 // Determinism.  Per-replica sums are ordered (per-tile partials, then one pass per replica);
 // the compensation-energy samples of LJ_IdealGas are accumulated in 64-bit fixed point
@@ -32,8 +35,12 @@ using namespace mrmd_hip;
 
 namespace
 {
-constexpr int B_RANGES = 27;       // 3 x 3 columns, up to three image pieces each
+constexpr int B_COLS = 2;          // tile columns per axis
+constexpr int B_STEN = B_COLS + 2; // staged columns per axis
+constexpr int B_RANGES = B_STEN * B_STEN * 3;  // staged columns, up to three image pieces each
+constexpr int B_SELF = B_COLS * B_COLS;
 constexpr int B_SPAN = 18;         // cells of a tile's column run (K + 2), so K <= 16
+constexpr int B_PAD = 4;           // staged records past the neighbourhood a candidate group may read
 constexpr int COMP_BINS = 200;     // LJ_IdealGas compensation bins (LJ_IdealGas.hpp:54)
 constexpr double COMP_SCALE = 4294967296.0;
 constexpr int64_t SAMPLING_INTERVAL = 200;    // LJ_IdealGas defaults (LJ_IdealGas.hpp:56-57)
@@ -56,20 +63,31 @@ struct RepD
     int32_t tfDelta, tfSym, tfPeriodic, cmp0;
 };
 
-// static geometry of a tile
+// static geometry of a tile: columns i0 .. i0 + ni - 1 by j0 .. j0 + nj - 1, cells k0 .. k1 - 1
 struct TileG
 {
-    int32_t rep, i, j, k0, k1;
+    int32_t rep, i0, j0, ni, nj, k0, k1;
 };
 
-// slot ranges of a tile, rebuilt with the replica's binning
+// slot ranges of a tile, rebuilt with the replica's binning: the staged pieces (column by
+// column, (ni + 2) x (nj + 2) columns, i-major), and the rows, self column after self column
 struct TileB
 {
-    int32_t rowBegin, rowEnd, nr, staged, selfPiece;
+    int32_t nr, staged, nself, nrows;
     int32_t begin[B_RANGES], base[B_RANGES];
     int8_t sx[B_RANGES], sy[B_RANGES], sz[B_RANGES];
-    int32_t colBase[9];
+    int32_t colBase[B_STEN * B_STEN];
+    int32_t selfPiece[B_SELF], selfBegin[B_SELF], selfPrefix[B_SELF + 1];
 };
+
+// sorted slot of the t-th row of a tile, and its self column
+__device__ inline int bTileRow(const TileB& ti, int t, int& sc)
+{
+    int r = 0;
+    while (r + 1 < ti.nself && ti.selfPrefix[r + 1] <= t) ++r;
+    sc = r;
+    return ti.selfBegin[r] + (t - ti.selfPrefix[r]);
+}
 
 struct BS
 {
@@ -94,7 +112,7 @@ struct BS
     uint16_t* nbr;
     int32_t* cnt;
     int64_t strideS;
-    int32_t capList, stageCap;
+    int32_t capList, stageCap;  ///< list width; staged records that fit the launch's LDS
     double* dispStep;
     double* dispAcc;
     int32_t* flag;
@@ -110,6 +128,7 @@ struct BS
     int32_t* err;
     Pre lj;
     double rcSqr, rnSqr;
+    int32_t tileBase;  ///< first tile of the launch (the force launches take plain and AdResS tiles apart)
 };
 
 __device__ inline int cellIndex(const RepD& R, int ix, int iy, int iz) { return R.c0 + (ix * R.nc[1] + iy) * R.nc[2] + iz; }
@@ -293,27 +312,75 @@ __global__ void k_bFill(BS b)
     b.sorted[b.offsets[c] + atomicAdd(&b.fill[c], 1)] = int32_t(i);
 }
 
-// ascending atom index inside each cell (the fill order of the atomics is arbitrary)
-__global__ void k_bSortCells(BS b)
+// ascending z inside each cell, the atom index breaking ties (the fill order of the atomics
+// is arbitrary).  A workgroup takes 128 consecutive cells (one contiguous run of slots,
+// replicas being contiguous) when one of them belongs to a flagged replica, loads the run's
+// keys into LDS, and every lane of a flagged cell insertion-sorts its cell there; a run
+// longer than the buffer (crowded cells) sorts in place in global memory.
+constexpr int B_SORT_CELLS = 128;
+constexpr int B_SORT_RUN = 4096;
+__global__ __launch_bounds__(B_SORT_CELLS) void k_bSortCells(BS b)
 {
-    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= b.ncells || !b.flag[b.cellRep[c]]) return;
-    int32_t* s = b.sorted + b.offsets[c];
-    const int n = b.counts[c];
-    for (int k = 1; k < n; ++k)
+    __shared__ int32_t idx[B_SORT_RUN];
+    __shared__ double key[B_SORT_RUN];
+    const int64_t c0 = int64_t(blockIdx.x) * B_SORT_CELLS;
+    const int64_t c1 = min(c0 + int64_t(B_SORT_CELLS), b.ncells);
+    const int64_t c = c0 + int64_t(threadIdx.x);
+    const bool mine = c < c1 && b.flag[b.cellRep[c]];
+    if (!__syncthreads_or(mine)) return;
+    const int r0 = b.offsets[c0], r1 = b.offsets[c1 - 1] + b.counts[c1 - 1];
+    auto less = [](double zx, int32_t x, double zy, int32_t y) { return zx < zy || (zx == zy && x < y); };
+    if (r1 - r0 <= B_SORT_RUN)
     {
-        const int32_t v = s[k];
-        int j = k - 1;
-        while (j >= 0 && s[j] > v)
+        for (int k = threadIdx.x; k < r1 - r0; k += B_SORT_CELLS)
         {
-            s[j + 1] = s[j];
+            const int32_t v = b.sorted[r0 + k];
+            idx[k] = v;
+            key[k] = b.pos[v].z;
+        }
+        __syncthreads();
+        if (mine)
+        {
+            const int o = b.offsets[c] - r0, n = b.counts[c];
+            for (int i = 1; i < n; ++i)
+            {
+                const int32_t v = idx[o + i];
+                const double zv = key[o + i];
+                int j = i - 1;
+                while (j >= 0 && less(zv, v, key[o + j], idx[o + j]))
+                {
+                    idx[o + j + 1] = idx[o + j];
+                    key[o + j + 1] = key[o + j];
+                    --j;
+                }
+                idx[o + j + 1] = v;
+                key[o + j + 1] = zv;
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < r1 - r0; k += B_SORT_CELLS) b.sorted[r0 + k] = idx[k];
+        return;
+    }
+    if (!mine) return;
+    int32_t* sl = b.sorted + b.offsets[c];
+    const int n = b.counts[c];
+    for (int i = 1; i < n; ++i)
+    {
+        const int32_t v = sl[i];
+        const double zv = b.pos[v].z;
+        int j = i - 1;
+        while (j >= 0 && less(zv, v, b.pos[sl[j]].z, sl[j]))
+        {
+            sl[j + 1] = sl[j];
             --j;
         }
-        s[j + 1] = v;
+        sl[j + 1] = v;
     }
 }
 
-// slot ranges, image shifts and LDS bases of every tile of a flagged replica
+// slot ranges, image shifts and LDS bases of every tile of a flagged replica: the staged
+// columns i-major, each the pieces of its z-run [k0 - 1, k1 + 1) (below 0 the image -1,
+// from nz the image +1), and the rows of the self columns (their unshifted pieces)
 __global__ void k_bTiles(BS b)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -323,20 +390,18 @@ __global__ void k_bTiles(BS b)
     const RepD& R = b.reps[G.rep];
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     TileB& ti = b.tiles[t];
-    const int first = cellIndex(R, G.i, G.j, G.k0), last = cellIndex(R, G.i, G.j, G.k1 - 1);
-    ti.rowBegin = b.offsets[first];
-    ti.rowEnd = b.offsets[last] + b.counts[last];
-    int nr = 0, staged = 0;
-    for (int di = -1; di <= 1; ++di)
-        for (int dj = -1; dj <= 1; ++dj)
+    const int spanJ = G.nj + 2;
+    int nr = 0, staged = 0, nself = 0, nrows = 0;
+    for (int sa = 0; sa < G.ni + 2; ++sa)
+        for (int sb = 0; sb < spanJ; ++sb)
         {
-            int ii = G.i + di, jj = G.j + dj, sx = 0, sy = 0;
+            int ii = G.i0 - 1 + sa, jj = G.j0 - 1 + sb, sx = 0, sy = 0;
             if (ii < 0) ii += nx, sx = -1;
             else if (ii >= nx) ii -= nx, sx = 1;
             if (jj < 0) jj += ny, sy = -1;
             else if (jj >= ny) jj -= ny, sy = 1;
-            ti.colBase[(di + 1) * 3 + (dj + 1)] = staged;
-            // pieces of the z-run [k0 - 1, k1 + 1): below 0 (image -1), inside, from nz (image +1)
+            ti.colBase[sa * spanJ + sb] = staged;
+            const bool self = sa >= 1 && sa <= G.ni && sb >= 1 && sb <= G.nj;
             const int zs = G.k0 - 1, ze = G.k1 + 1;
             const int lo[3] = {zs < 0 ? nz - 1 : 0, zs < 0 ? 0 : zs, 0};
             const int hi[3] = {zs < 0 ? nz : 0, ze < nz ? ze : nz, ze > nz ? ze - nz : 0};
@@ -350,13 +415,24 @@ __global__ void k_bTiles(BS b)
                 ti.sx[nr] = int8_t(sx);
                 ti.sy[nr] = int8_t(sy);
                 ti.sz[nr] = int8_t(sh[pc]);
-                if (di == 0 && dj == 0 && sh[pc] == 0) ti.selfPiece = nr;
+                if (self && sh[pc] == 0)
+                {
+                    const int rf = cellIndex(R, ii, jj, G.k0), rl = cellIndex(R, ii, jj, G.k1 - 1);
+                    ti.selfPiece[nself] = nr;
+                    ti.selfBegin[nself] = b.offsets[rf];
+                    ti.selfPrefix[nself] = nrows;
+                    nrows += b.offsets[rl] + b.counts[rl] - b.offsets[rf];
+                    ++nself;
+                }
                 staged += b.offsets[cl] + b.counts[cl] - b.offsets[cf];
                 ++nr;
             }
         }
+    ti.selfPrefix[nself] = nrows;
     ti.nr = nr;
     ti.staged = staged;
+    ti.nself = nself;
+    ti.nrows = nrows;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -389,8 +465,10 @@ __global__ void k_bGather(BS b)
     if (valid && key >= 0 && r.head == int(threadIdx.x) % WAVE) atomicAddD(&b.tfDensity[key], double(r.len));
 }
 
-// copy a tile's neighbourhood into LDS, shifting each piece by its periodic image
-__device__ inline void stageShifted(const TileB& ti, const double4* __restrict__ sp, double4* st, const RepD& R)
+// copy a tile's neighbourhood into LDS, shifting each piece by its periodic image: {x, y, z}
+// records (XYZ) or whole double4 records (the AdResS forces read the slab weight in w)
+template <bool XYZ>
+__device__ inline void stageShifted(const TileB& ti, const double4* __restrict__ sp, double* st, const RepD& R)
 {
     int r = 0;
     for (int q = threadIdx.x; q < ti.staged; q += blockDim.x)
@@ -400,7 +478,14 @@ __device__ inline void stageShifted(const TileB& ti, const double4* __restrict__
         v.x += double(ti.sx[r]) * R.L[0];
         v.y += double(ti.sy[r]) * R.L[1];
         v.z += double(ti.sz[r]) * R.L[2];
-        st[q] = v;
+        if (XYZ)
+        {
+            st[3 * q] = v.x;
+            st[3 * q + 1] = v.y;
+            st[3 * q + 2] = v.z;
+        }
+        else
+            reinterpret_cast<double4*>(st)[q] = v;
     }
 }
 
@@ -408,7 +493,12 @@ __device__ inline void stageShifted(const TileB& ti, const double4* __restrict__
 // straight from the slot-ordered records with the same periodic shift (the same arithmetic,
 // so the same bits) -- a slower path that keeps tiles of crowded cells (lattice starts,
 // density fluctuations) exact instead of dropping them.
-struct LdsFetch
+struct LdsFetch3
+{
+    const double* st;
+    __device__ double4 operator()(int q) const { return make_double4(st[3 * q], st[3 * q + 1], st[3 * q + 2], 1.0); }
+};
+struct LdsFetch4
 {
     const double4* st;
     __device__ double4 operator()(int q) const { return st[q]; }
@@ -420,6 +510,7 @@ struct GlobalFetch
     double L0, L1, L2;
     __device__ double4 operator()(int q) const
     {
+        q = min(q, ti->staged - 1);  // a candidate group may read up to three records past a run
         int r = 0;
         while (r + 1 < ti->nr && ti->base[r + 1] <= q) ++r;
         double4 v = sp[ti->begin[r] + (q - ti->base[r])];
@@ -430,69 +521,122 @@ struct GlobalFetch
     }
 };
 
-// squared distance from p to the (unwrapped) cell (ix, iy, iz) of the replica grid
-__device__ inline double cellDistSqr(const RepD& R, double px, double py, double pz, int ix, int iy, int iz)
-{
-    const double p[3] = {px, py, pz};
-    const int c[3] = {ix, iy, iz};
-    double r2 = 0.0;
-    for (int d = 0; d < 3; ++d)
-    {
-        const double centre = (double(c[d]) + 0.5) * R.dxc[d];
-        double r = fabs(p[d] - centre) - 0.5 * R.dxc[d];
-        r = r > 0.0 ? r : 0.0;
-        r2 += r * r;
-    }
-    return r2;
-}
-
-// rows of one tile of a flagged replica (entries: staged slots, rows padded to whole chunks
-// with the row's own slot)
+// Full rows of one tile of a flagged replica, one lane per row, as the single-system build
+// forms them: per stencil column the cells that Cabana's min-distance prune keeps (one run
+// in z), cut by binary search to the z window of the row (records are z-sorted in a staged
+// column), 32 candidates per pass in groups of four with the hit bits shifted in by a
+// funnel shift, then the hits appended in slot order.  Entries are staged slots; rows are
+// padded to whole chunks with the row's own slot (distance 0, which the forces skip).
 template <class Fetch>
 __device__ void bBuildRows(const BS& b, const TileB& ti, const TileG& G, const RepD& R, const int* cstart,
                            const Fetch& fetch)
 {
+    const int spanJ = G.nj + 2;
+    const int64_t chunkStride = b.strideS * CHUNK;
+    const double rsq = b.rnSqr;
     int localMax = 0;
-    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
-        const int qSelf = ti.base[ti.selfPiece] + (s - ti.begin[ti.selfPiece]);
+        int sc;
+        const int s = bTileRow(ti, t, sc);
+        const int pr = ti.selfPiece[sc];
+        const int qSelf = ti.base[pr] + (s - ti.begin[pr]);
         const double4 xp = fetch(qSelf);
         const int kz = (b.cellOf[b.sorted[s]] - R.c0) % R.nc[2];
-        // entry k of this row: rowBase[(k / 8) * chunkStride + k % 8] (entryIndex)
+        const int sa = sc / G.nj, sb = sc % G.nj;
+        const int i = G.i0 + sa, j = G.j0 + sb;
+        // per-axis distances from the row to the stencil cells' boxes (unwrapped indices) and
+        // the lateral terms of the z window, shrunk by 1e-9 of a cell
+        double rx2[3], ry2[3], wx2[3], wy2[3], rz2[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            const double cx = (double(i - 1 + a) + 0.5) * R.dxc[0];
+            const double cy = (double(j - 1 + a) + 0.5) * R.dxc[1];
+            const double cz = (double(kz - 1 + a) + 0.5) * R.dxc[2];
+            double r = fabs(xp.x - cx) - 0.5 * R.dxc[0];
+            r = r > 0.0 ? r : 0.0;
+            rx2[a] = r * r;
+            r = fabs(xp.y - cy) - 0.5 * R.dxc[1];
+            r = r > 0.0 ? r : 0.0;
+            ry2[a] = r * r;
+            r = fabs(xp.z - cz) - 0.5 * R.dxc[2];
+            r = r > 0.0 ? r : 0.0;
+            rz2[a] = r * r;
+            double e = fabs(xp.x - cx) - 0.5 * R.dxc[0] - 1e-9 * R.dxc[0];
+            e = e > 0.0 ? e : 0.0;
+            wx2[a] = e * e;
+            e = fabs(xp.y - cy) - 0.5 * R.dxc[1] - 1e-9 * R.dxc[1];
+            e = e > 0.0 ? e : 0.0;
+            wy2[a] = e * e;
+        }
         uint16_t* const rowBase = b.nbr + int64_t(s) * CHUNK;
-        const int64_t chunkStride = b.strideS * CHUNK;
         int c = 0;
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj)
-                for (int dk = -1; dk <= 1; ++dk)
-                {
-                    if (cellDistSqr(R, xp.x, xp.y, xp.z, G.i + di, G.j + dj, kz + dk) > b.rnSqr) continue;
-                    const int* cs = cstart + ((di + 1) * 3 + (dj + 1)) * (B_SPAN + 1) + (kz - G.k0 + 1 + dk);
-                    const int q1 = cs[1];
-                    for (int q = cs[0]; q < q1; q += 4)
+#pragma unroll
+        for (int da = -1; da <= 1; ++da)
+#pragma unroll
+            for (int db = -1; db <= 1; ++db)
+            {
+                // z index of the staged column: cell k of the run is entry k - (k0 - 1)
+                const int* cs = cstart + ((sa + 1 + da) * spanJ + (sb + 1 + db)) * (B_SPAN + 1) - (G.k0 - 1);
+                const double lat = rx2[da + 1] + ry2[db + 1];
+                int kLo = INT_MAX, kHi = INT_MIN;
+#pragma unroll
+                for (int dc = 0; dc < 3; ++dc)
+                    if (lat + rz2[dc] <= rsq)
                     {
-                        // reads stay inside the cell's run; the bits past its end and the
-                        // row's own slot are cleared after the distances
-                        uint32_t hits = 0;
+                        kLo = min(kLo, kz - 1 + dc);
+                        kHi = kz - 1 + dc;
+                    }
+                if (kLo > kHi) continue;
+                const double zsq = rsq - (wx2[da + 1] + wy2[db + 1]) + 1e-12 * rsq;
+                int lo = cs[kLo], hi = cs[kLo + 1];
+                while (lo < hi)
+                {
+                    const int m = (lo + hi) >> 1;
+                    const double dz = xp.z - fetch(m).z;
+                    if (dz > 0.0 && dz * dz > zsq) lo = m + 1;
+                    else hi = m;
+                }
+                const int wlo = lo;
+                lo = max(wlo, cs[kHi]);
+                hi = cs[kHi + 1];
+                while (lo < hi)
+                {
+                    const int m = (lo + hi) >> 1;
+                    const double dz = xp.z - fetch(m).z;
+                    if (dz < 0.0 && dz * dz > zsq) hi = m;
+                    else lo = m + 1;
+                }
+                const int whi = lo;
+                for (int qb = wlo; qb < whi; qb += 32)
+                {
+                    uint32_t miss = 0;
+                    const int nq = min(32, whi - qb);
+                    int m = 0;
+                    for (; m < nq; m += 4)
+                    {
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
                         {
-                            const double4 xn = fetch(min(q + u, q1 - 1));
+                            const double4 xn = fetch(qb + m + u);
                             const double dx = xp.x - xn.x, dy = xp.y - xn.y, dz = xp.z - xn.z;
-                            hits |= uint32_t(dx * dx + dy * dy + dz * dz <= b.rnSqr) << u;
-                        }
-                        const int nq = min(4, q1 - q);
-                        hits &= (1u << nq) - 1u;
-                        if (qSelf >= q && qSelf < q + nq) hits &= ~(1u << (qSelf - q));
-                        while (hits)
-                        {
-                            const int u = __builtin_ctz(hits);
-                            hits &= hits - 1;
-                            if (c < b.capList) rowBase[int64_t(c >> 3) * chunkStride + (c & 7)] = uint16_t(q + u);
-                            ++c;
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            miss = __builtin_amdgcn_alignbit(miss, uint32_t(__double2hiint(rsq - r2)), 31u);
                         }
                     }
+                    uint32_t hits = ~miss << (32 - m);
+                    hits &= ~((1u << (32 - nq)) - 1u);
+                    if (qSelf >= qb && qSelf < qb + nq) hits &= ~(0x80000000u >> (qSelf - qb));
+                    while (hits)
+                    {
+                        const int u = __builtin_clz(hits);
+                        hits ^= 0x80000000u >> u;
+                        if (c < b.capList) rowBase[int64_t(c >> 3) * chunkStride + (c & 7)] = uint16_t(qb + u);
+                        ++c;
+                    }
                 }
+            }
         for (int k = c; k < ((c + CHUNK - 1) / CHUNK) * CHUNK && k < b.capList; ++k)
             rowBase[int64_t(k >> 3) * chunkStride + (k & 7)] = uint16_t(qSelf);
         b.cnt[s] = min(c, b.capList);  // an overflow is reported through err; reads stay in the list
@@ -504,9 +648,9 @@ __device__ void bBuildRows(const BS& b, const TileB& ti, const TileG& G, const R
 // full-row list of every tile of a flagged replica
 __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
 {
-    extern __shared__ double4 st[];
+    extern __shared__ double stb[];
     __shared__ TileB ti;
-    __shared__ int cstart[9 * (B_SPAN + 1)];
+    __shared__ int cstart[B_STEN * B_STEN * (B_SPAN + 1)];
     // plain block order: when only some replicas rebuild, their tiles are contiguous, and the
     // hardware's round-robin of consecutive workgroups over the XCDs spreads them over all
     // eight (grouping consecutive tiles on one XCD would put a whole replica on one)
@@ -515,30 +659,31 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
     if (!b.flag[G.rep]) return;
     if (threadIdx.x == 0) ti = b.tiles[t];
     __syncthreads();
-    if (ti.rowEnd <= ti.rowBegin) return;
+    if (ti.nrows == 0) return;
     const RepD R = b.reps[G.rep];
     const bool fits = ti.staged <= b.stageCap;
-    if (fits) stageShifted(ti, b.sortedPos, st, R);
+    if (fits) stageShifted<true>(ti, b.sortedPos, stb, R);
+    // LDS slot of the first record of cell k0 - 1 + z of each staged column
     const int nzr = G.k1 - G.k0 + 2;
-    if (threadIdx.x < 9)
+    const int spanJ = G.nj + 2, ncol = (G.ni + 2) * spanJ;
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x)
     {
-        const int c = threadIdx.x, di = c / 3 - 1, dj = c % 3 - 1;
-        int ii = G.i + di, jj = G.j + dj;
+        int ii = G.i0 - 1 + col / spanJ, jj = G.j0 - 1 + col % spanJ;
         ii = ii < 0 ? ii + R.nc[0] : (ii >= R.nc[0] ? ii - R.nc[0] : ii);
         jj = jj < 0 ? jj + R.nc[1] : (jj >= R.nc[1] ? jj - R.nc[1] : jj);
-        int run = ti.colBase[c];
+        int run = ti.colBase[col];
         for (int z = 0; z < nzr; ++z)
         {
             int kz = G.k0 - 1 + z;
             kz = kz < 0 ? kz + R.nc[2] : (kz >= R.nc[2] ? kz - R.nc[2] : kz);
-            cstart[c * (B_SPAN + 1) + z] = run;
+            cstart[col * (B_SPAN + 1) + z] = run;
             run += b.counts[cellIndex(R, ii, jj, kz)];
         }
-        cstart[c * (B_SPAN + 1) + nzr] = run;
+        cstart[col * (B_SPAN + 1) + nzr] = run;
     }
     __syncthreads();
     if (fits)
-        bBuildRows(b, ti, G, R, cstart, LdsFetch{st});
+        bBuildRows(b, ti, G, R, cstart, LdsFetch3{stb});
     else
         bBuildRows(b, ti, G, R, cstart, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
 }
@@ -547,7 +692,9 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
 // forces: weighted LJ (LJ_IdealGas.cpp:80-200 with one atom per molecule; weight 1 for
 // plain replicas, which reduces to LennardJones.hpp:168-197), drift and compensation terms
 
-template <bool CAP, class Fetch>
+// W: weighted (AdResS replicas: slab weights in w, the unweighted energy e0 for the drift);
+// plain replicas have weight 1 and need neither
+template <bool CAP, bool W, class Fetch>
 __device__ inline void chunkW(const BS& b, const double4& pi, const Fetch& fetch, const uint4 cur, double& fx,
                               double& fy, double& fz, double& ew, double& e0)
 {
@@ -568,15 +715,28 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const Fetch& fetch
     for (int u = 0; u < CHUNK; ++u) frac2[u] = recipNewton(d2[u]);
     const Pre& p = b.lj;
     auto accumulate = [&](int u, double ff, double en) {
-        // both partners coarse-grained: no interaction (LJ_IdealGas.cpp:95)
-        const bool in = d2[u] <= b.rcSqr && d2[u] > 0.0 && !(pi.w <= 0.0 && lj[u] <= 0.0);
-        const double w = 0.5 * (pi.w + lj[u]);
-        const double ffw = in ? ff * w : 0.0;
-        fx += dx[u] * ffw;
-        fy += dy[u] * ffw;
-        fz += dz[u] * ffw;
-        ew += in ? en * w : 0.0;
-        e0 += in ? en : 0.0;
+        if constexpr (W)
+        {
+            // both partners coarse-grained: no interaction (LJ_IdealGas.cpp:95)
+            const bool in = d2[u] <= b.rcSqr && d2[u] > 0.0 && !(pi.w <= 0.0 && lj[u] <= 0.0);
+            const double w = 0.5 * (pi.w + lj[u]);
+            const double ffw = in ? ff * w : 0.0;
+            fx += dx[u] * ffw;
+            fy += dy[u] * ffw;
+            fz += dz[u] * ffw;
+            ew += in ? en * w : 0.0;
+            e0 += in ? en : 0.0;
+        }
+        else
+        {
+            // padding entries are the row itself (distance 0)
+            const bool in = d2[u] <= b.rcSqr && d2[u] > 0.0;
+            const double ffw = in ? ff : 0.0;
+            fx += dx[u] * ffw;
+            fy += dy[u] * ffw;
+            fz += dz[u] * ffw;
+            ew += in ? en : 0.0;
+        }
     };
     if constexpr (!CAP)
     {
@@ -619,14 +779,16 @@ __device__ inline void chunkW(const BS& b, const double4& pi, const Fetch& fetch
 
 // the rows of one tile: weighted pair forces, drift and compensation terms; returns the
 // lane's energy sum
-template <bool CAP, class Fetch>
+template <bool CAP, bool W, class Fetch>
 __device__ double bForceRows(const BS& b, const TileB& ti, const RepD& R, const Fetch& fetch)
 {
     const int64_t step = *b.step;
-    const bool sampling = R.adress && step % SAMPLING_INTERVAL == 0;
+    const bool sampling = W && R.adress && step % SAMPLING_INTERVAL == 0;
     double eTile = 0.0;
-    for (int s = ti.rowBegin + threadIdx.x; s < ti.rowEnd; s += blockDim.x)
+    for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
     {
+        int sc;
+        const int s = bTileRow(ti, t, sc);
         const int c = b.cnt[s];
         const int32_t i = b.sorted[s];
         const double4 pi = b.sortedPos[s];
@@ -638,10 +800,10 @@ __device__ double bForceRows(const BS& b, const TileB& ti, const RepD& R, const 
         {
             uint4 nxt = cur;
             if (ch + 1 < nch) nxt = chunk[int64_t(ch + 1) * b.strideS];
-            chunkW<CAP>(b, pi, fetch, cur, fx, fy, fz, ew, e0);
+            chunkW<CAP, W>(b, pi, fetch, cur, fx, fy, fz, ew, e0);
             cur = nxt;
         }
-        if (R.adress && pi.w > 0.0 && pi.w < 1.0)
+        if (W && R.adress && pi.w > 0.0 && pi.w < 1.0)
         {
             // hybrid atom: drift -1/2 sum_j E_ij grad(lambda) and the mean compensation
             // energy (LJ_IdealGas.cpp:150-170, 208-224)
@@ -666,22 +828,29 @@ __device__ double bForceRows(const BS& b, const TileB& ti, const RepD& R, const 
     return eTile;
 }
 
-template <bool CAP>
+// tiles [tileBase, tileBase + gridDim.x): the plain replicas' tiles (W = false, {x, y, z}
+// staged) or the AdResS replicas' (W = true, whole records staged)
+template <bool CAP, bool W>
 __global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
 {
-    extern __shared__ double4 st[];
+    extern __shared__ double stf[];
     __shared__ TileB ti;
     __shared__ double red[BLOCK / WAVE];
-    const unsigned t = xcdBlock();
+    const unsigned t = unsigned(b.tileBase) + xcdBlock();
     const TileG G = b.tg[t];
     if (threadIdx.x == 0) ti = b.tiles[t];
     __syncthreads();
     const bool fits = ti.staged <= b.stageCap;
     const RepD R = b.reps[G.rep];
-    if (fits) stageShifted(ti, b.sortedPos, st, R);
+    if (fits) stageShifted<!W>(ti, b.sortedPos, stf, R);
     __syncthreads();
-    const double eTile = fits ? bForceRows<CAP>(b, ti, R, LdsFetch{st})
-                              : bForceRows<CAP>(b, ti, R, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
+    double eTile;
+    if (!fits)
+        eTile = bForceRows<CAP, W>(b, ti, R, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
+    else if constexpr (W)
+        eTile = bForceRows<CAP, W>(b, ti, R, LdsFetch4{reinterpret_cast<const double4*>(stf)});
+    else
+        eTile = bForceRows<CAP, W>(b, ti, R, LdsFetch3{stf});
     const double e = blockSum(0.5 * eTile, red);
     if (threadIdx.x == 0) b.ePart[t] = e;
 }
@@ -828,8 +997,10 @@ struct mrmd_batch
     double rc = 2.5, skin = 0.3;
     Pre lj{};
     bool cap_ = false;
-    int targetStaged = 1000, block = 256, stageCap = 1600;
+    int targetStaged = 1400, block = 256;
+    int stageCapPlain = 1600, stageCapAdress = 1600, stageCapBuild = 1600;  ///< staged records in LDS per launch
     int stageCapLimit = 0;  ///< 0: the stage area follows the expected neighbourhood size
+    int64_t plainTiles = 0;  ///< tiles of the plain replicas come first, then the AdResS replicas' tiles
     int32_t capList = 0;
     int64_t steps = 0;
 
@@ -890,7 +1061,8 @@ BS mrmd_batch::state()
     b.cnt = cnt.p;
     b.strideS = cap;
     b.capList = capList;
-    b.stageCap = stageCap;
+    b.stageCap = stageCapBuild;
+    b.tileBase = 0;
     b.dispStep = dispStep.p;
     b.dispAcc = dispAcc.p;
     b.flag = flag.p;
@@ -910,32 +1082,50 @@ BS mrmd_batch::state()
     return b;
 }
 
-// per-replica tile height from the target staged size, tile geometry, LDS capacity
+// per-replica tile height from the target staged size, tile geometry, LDS capacity; the
+// plain replicas' tiles first, then the AdResS replicas' (their force launches stage
+// different records)
 void mrmd_batch::layoutTiles()
 {
     tg.clear();
-    double maxStage = 0.0;
+    double maxStage[2] = {0.0, 0.0};
     int64_t t0 = 0;
-    for (auto& R : reps)
+    for (int adress = 0; adress < 2; ++adress)
     {
-        const double perCell = double(R.n) / double(R.nc[0] * R.nc[1] * R.nc[2]);
-        int K = int(std::lround(double(targetStaged) / (9.0 * std::max(perCell, 1e-3)))) - 2;
-        K = std::max(1, std::min({K, R.nc[2], B_SPAN - 2}));
-        R.K = K;
-        R.tilesPerCol = (R.nc[2] + K - 1) / K;
-        R.t0 = int32_t(t0);
-        R.ntiles = R.nc[0] * R.nc[1] * R.tilesPerCol;
-        for (int i = 0; i < R.nc[0]; ++i)
-            for (int j = 0; j < R.nc[1]; ++j)
-                for (int kb = 0; kb < R.tilesPerCol; ++kb)
-                    tg.push_back(TileG{int32_t(&R - reps.data()), i, j, kb * K, std::min((kb + 1) * K, R.nc[2])});
-        t0 += R.ntiles;
-        maxStage = std::max(maxStage, 9.0 * (K + 2) * perCell);
+        if (adress == 1) plainTiles = t0;
+        for (auto& R : reps)
+        {
+            if (R.adress != adress) continue;
+            const double perCell = double(R.n) / double(R.nc[0] * R.nc[1] * R.nc[2]);
+            const double cols = double(B_STEN * B_STEN);
+            int K = int(std::lround(double(targetStaged) / (cols * std::max(perCell, 1e-3)))) - 2;
+            K = std::max(1, std::min({K, R.nc[2], B_SPAN - 2}));
+            R.K = K;
+            R.tilesPerCol = (R.nc[2] + K - 1) / K;
+            R.t0 = int32_t(t0);
+            const int bi = (R.nc[0] + B_COLS - 1) / B_COLS, bj = (R.nc[1] + B_COLS - 1) / B_COLS;
+            R.ntiles = bi * bj * R.tilesPerCol;
+            for (int ib = 0; ib < bi; ++ib)
+                for (int jb = 0; jb < bj; ++jb)
+                    for (int kb = 0; kb < R.tilesPerCol; ++kb)
+                        tg.push_back(TileG{int32_t(&R - reps.data()), ib * B_COLS, jb * B_COLS,
+                                           std::min(B_COLS, R.nc[0] - ib * B_COLS),
+                                           std::min(B_COLS, R.nc[1] - jb * B_COLS), kb * K,
+                                           std::min((kb + 1) * K, R.nc[2])});
+            t0 += R.ntiles;
+            maxStage[adress] = std::max(maxStage[adress], cols * (K + 2) * perCell);
+        }
     }
     ntiles = t0;
     // LDS capacity: the expected largest neighbourhood with room for density fluctuations
-    stageCap = int(std::min(double(STAGE_MAX), std::ceil(1.6 * maxStage + 64.0)));
-    if (stageCapLimit > 0) stageCap = std::min(stageCap, stageCapLimit);
+    // (a tile beyond it reads its records from global memory, exactly)
+    auto capFor = [&](double m) {
+        int v = int(std::min(double(STAGE_MAX), std::ceil(1.25 * m + 64.0)));
+        return stageCapLimit > 0 ? std::min(v, stageCapLimit) : v;
+    };
+    stageCapPlain = capFor(maxStage[0]);
+    stageCapAdress = capFor(maxStage[1]);
+    stageCapBuild = std::max(stageCapPlain, stageCapAdress);
     mrmd_ctx* c = ctx;
     dreps.reserve(c, int64_t(reps.size()));
     HIPCHECK(hipMemcpyAsync(dreps.p, reps.data(), reps.size() * sizeof(RepD), hipMemcpyHostToDevice, c->stream));
@@ -957,8 +1147,8 @@ void mrmd_batch::enqueueStep()
     mrmd_ctx* c = ctx;
     const BS b = state();
     const unsigned ga = gridFor(natoms), gc = gridFor(ncells), gt = gridFor(ntiles);
+    const unsigned gs = unsigned((ncells + B_SORT_CELLS - 1) / B_SORT_CELLS);
     const unsigned nr = unsigned(reps.size());
-    const size_t lds = size_t(stageCap) * sizeof(double4);
     hipLaunchKernelGGL(k_bPre, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bCheck, dim3(gridFor(nr)), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bWrap, dim3(ga), dim3(BLOCK), 0, c->stream, b);
@@ -966,14 +1156,28 @@ void mrmd_batch::enqueueStep()
     hipLaunchKernelGGL(k_bCount, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bSortCells, dim3(gc), dim3(BLOCK), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(B_SORT_CELLS), 0, c->stream, b);
     hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,
+                       b);
+    auto force = [&](auto kernel, int64_t t0, int64_t t1, int capRecs, size_t bytes) {
+        if (t1 <= t0) return;
+        BS f = b;
+        f.tileBase = int32_t(t0);
+        f.stageCap = capRecs;
+        hipLaunchKernelGGL(kernel, dim3(unsigned(t1 - t0)), dim3(block), size_t(capRecs) * bytes, c->stream, f);
+    };
     if (cap_)
-        hipLaunchKernelGGL(k_bForce<true>, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    {
+        force(k_bForce<true, false>, 0, plainTiles, stageCapPlain, 24);
+        force(k_bForce<true, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
+    }
     else
-        hipLaunchKernelGGL(k_bForce<false>, dim3(unsigned(ntiles)), dim3(block), lds, c->stream, b);
+    {
+        force(k_bForce<false, false>, 0, plainTiles, stageCapPlain, 24);
+        force(k_bForce<false, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
+    }
     hipLaunchKernelGGL(k_bCompUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bTFUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bPost, dim3(ga), dim3(BLOCK), 0, c->stream, b);

@@ -251,3 +251,23 @@ def test_checkpoint_resume_bit_identical(ctx, relaxed, tmp_path):
     pa, ka = a.energies()
     pb, kb = b.energies()
     assert np.array_equal(pa, pb) and np.array_equal(ka, kb)
+
+
+def test_tiling_does_not_change_the_bits(ctx, relaxed):
+    """two launch shapes (tile height target and workgroup size) give bit-identical runs: a
+    row's entries follow its stencil columns and, in each, the z order of the records,
+    whatever the tile that stages them"""
+    from mrmd_amd.ensemble import Ensemble
+    reps, pos, vel = relaxed
+    out = []
+    for target, block in ((1000, 128), (1800, 256)):
+        e = Ensemble(ctx, reps, RC, SKIN)
+        e.set_lj(cap=CAP)
+        e.set_tiling(target, block)
+        e.upload(0, pos)
+        e.upload(1, vel)
+        e.run(60, 10)
+        out.append((e.positions(), e.velocities()))
+    for k in range(len(reps)):
+        np.testing.assert_array_equal(out[0][0][k], out[1][0][k])
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
