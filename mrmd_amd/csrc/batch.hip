@@ -312,70 +312,13 @@ __global__ void k_bFill(BS b)
     b.sorted[b.offsets[c] + atomicAdd(&b.fill[c], 1)] = int32_t(i);
 }
 
-// ascending z inside each cell, the atom index breaking ties (the fill order of the atomics
-// is arbitrary).  A workgroup takes 128 consecutive cells (one contiguous run of slots,
-// replicas being contiguous) when one of them belongs to a flagged replica, loads the run's
-// keys into LDS, and every lane of a flagged cell insertion-sorts its cell there; a run
-// longer than the buffer (crowded cells) sorts in place in global memory.
-constexpr int B_SORT_CELLS = 128;
-constexpr int B_SORT_RUN = 4096;
-__global__ __launch_bounds__(B_SORT_CELLS) void k_bSortCells(BS b)
+// ascending z inside each cell of a flagged replica, the atom index breaking ties (the fill
+// order of the atomics is arbitrary): one group of 32 lanes per cell (sortCellByZ)
+__global__ __launch_bounds__(BLOCK) void k_bSortCells(BS b)
 {
-    __shared__ int32_t idx[B_SORT_RUN];
-    __shared__ double key[B_SORT_RUN];
-    const int64_t c0 = int64_t(blockIdx.x) * B_SORT_CELLS;
-    const int64_t c1 = min(c0 + int64_t(B_SORT_CELLS), b.ncells);
-    const int64_t c = c0 + int64_t(threadIdx.x);
-    const bool mine = c < c1 && b.flag[b.cellRep[c]];
-    if (!__syncthreads_or(mine)) return;
-    const int r0 = b.offsets[c0], r1 = b.offsets[c1 - 1] + b.counts[c1 - 1];
-    auto less = [](double zx, int32_t x, double zy, int32_t y) { return zx < zy || (zx == zy && x < y); };
-    if (r1 - r0 <= B_SORT_RUN)
-    {
-        for (int k = threadIdx.x; k < r1 - r0; k += B_SORT_CELLS)
-        {
-            const int32_t v = b.sorted[r0 + k];
-            idx[k] = v;
-            key[k] = b.pos[v].z;
-        }
-        __syncthreads();
-        if (mine)
-        {
-            const int o = b.offsets[c] - r0, n = b.counts[c];
-            for (int i = 1; i < n; ++i)
-            {
-                const int32_t v = idx[o + i];
-                const double zv = key[o + i];
-                int j = i - 1;
-                while (j >= 0 && less(zv, v, key[o + j], idx[o + j]))
-                {
-                    idx[o + j + 1] = idx[o + j];
-                    key[o + j + 1] = key[o + j];
-                    --j;
-                }
-                idx[o + j + 1] = v;
-                key[o + j + 1] = zv;
-            }
-        }
-        __syncthreads();
-        for (int k = threadIdx.x; k < r1 - r0; k += B_SORT_CELLS) b.sorted[r0 + k] = idx[k];
-        return;
-    }
-    if (!mine) return;
-    int32_t* sl = b.sorted + b.offsets[c];
-    const int n = b.counts[c];
-    for (int i = 1; i < n; ++i)
-    {
-        const int32_t v = sl[i];
-        const double zv = b.pos[v].z;
-        int j = i - 1;
-        while (j >= 0 && less(zv, v, b.pos[sl[j]].z, sl[j]))
-        {
-            sl[j + 1] = sl[j];
-            --j;
-        }
-        sl[j + 1] = v;
-    }
+    const int64_t c = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SORT_GROUP;
+    if (c >= b.ncells || !b.flag[b.cellRep[c]]) return;
+    sortCellByZ(b.sorted + b.offsets[c], b.counts[c], b.pos, int(threadIdx.x % SORT_GROUP));
 }
 
 // slot ranges, image shifts and LDS bases of every tile of a flagged replica: the staged
@@ -1147,7 +1090,7 @@ void mrmd_batch::enqueueStep()
     mrmd_ctx* c = ctx;
     const BS b = state();
     const unsigned ga = gridFor(natoms), gc = gridFor(ncells), gt = gridFor(ntiles);
-    const unsigned gs = unsigned((ncells + B_SORT_CELLS - 1) / B_SORT_CELLS);
+    const unsigned gs = unsigned((ncells * SORT_GROUP + BLOCK - 1) / BLOCK);
     const unsigned nr = unsigned(reps.size());
     hipLaunchKernelGGL(k_bPre, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bCheck, dim3(gridFor(nr)), dim3(BLOCK), 0, c->stream, b);
@@ -1156,7 +1099,7 @@ void mrmd_batch::enqueueStep()
     hipLaunchKernelGGL(k_bCount, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(B_SORT_CELLS), 0, c->stream, b);
+    hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,

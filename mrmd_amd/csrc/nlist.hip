@@ -60,88 +60,52 @@ __global__ void k_binFill(const int32_t* cellOf, int64_t begin, int64_t n, const
     if (valid) sorted[offsets[c] + base + (lane - r.head)] = int32_t(begin + k);
 }
 
-// order the records inside each cell: ascending record index, or (Z) ascending z with the
-// record index breaking ties.  A workgroup takes 128 consecutive cells, whose records are
-// one contiguous run of the sorted array, and loads the run's keys into LDS once.  By index,
-// every record's place in its cell is its rank there, the count of the cell's records with
-// a smaller index (one lane per record; the lanes of one cell read the same record at a
-// time), so the records go straight to their places; by z, every lane insertion-sorts its
-// cell in LDS (fewer comparisons, n^2 / 4 per cell against n^2 for the ranks, which the
-// larger z keys make the cheaper form) and the run is written back.  A run longer than the
-// LDS buffer (pathological crowding) sorts in place in global memory, one lane per cell,
-// with a heap sort for crowded cells.
+// order the records inside each cell by ascending record index.  A workgroup takes 128
+// consecutive cells, whose records are one contiguous run of the sorted array, and loads the
+// run into LDS once; every record's place in its cell is its rank there, the count of the
+// cell's records with a smaller index (one lane per record; the lanes of one cell read the
+// same record at a time), so the records go straight to their places.  A run longer than
+// the LDS buffer (pathological crowding) sorts in place in global memory, one lane per cell,
+// with a heap sort for crowded cells.  (The z order of the list build is k_sortCellsZ's.)
 constexpr int SORT_RUN = 4096;
 constexpr int SORT_CELLS = 128;  // cells per workgroup: ~20 records each fill a run of ~2500
 
-template <bool Z>
 __global__ __launch_bounds__(SORT_CELLS) void k_binSortCells(const int32_t* offsets, const int32_t* counts, int32_t* sorted,
-                                                        int ncell, const double4* __restrict__ pos)
+                                                        int ncell)
 {
     __shared__ int32_t idx[SORT_RUN];
-    __shared__ double key[Z ? SORT_RUN : 1];
-    __shared__ uint8_t cellOfRec[Z ? 1 : SORT_RUN];
+    __shared__ uint8_t cellOfRec[SORT_RUN];
     __shared__ int32_t cellStart[SORT_CELLS], cellCount[SORT_CELLS];
     const int c0 = int(blockIdx.x) * SORT_CELLS;
     const int c1 = min(c0 + SORT_CELLS, ncell);
     const int r0 = offsets[c0], r1 = offsets[c1 - 1] + counts[c1 - 1];
     const int c = c0 + int(threadIdx.x);
-    auto less = [](double zx, int32_t x, double zy, int32_t y) { return Z ? (zx < zy || (zx == zy && x < y)) : x < y; };
     if (r1 - r0 <= SORT_RUN)
     {
-        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS)
-        {
-            const int32_t v = sorted[r0 + k];
-            idx[k] = v;
-            if (Z) key[k] = pos[v].z;
-        }
-        if constexpr (!Z)
-        {
-            if (c < c1)
-            {
-                const int o = offsets[c] - r0, n = counts[c];
-                cellStart[threadIdx.x] = o;
-                cellCount[threadIdx.x] = n;
-                for (int i = 0; i < n; ++i) cellOfRec[o + i] = uint8_t(threadIdx.x);
-            }
-            __syncthreads();
-            for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS)
-            {
-                const int32_t v = idx[k];
-                const int cl = cellOfRec[k];
-                const int o = cellStart[cl], n = cellCount[cl];
-                int rank = 0;
-                for (int j = o; j < o + n; ++j) rank += idx[j] < v ? 1 : 0;
-                sorted[r0 + o + rank] = v;
-            }
-            return;
-        }
-        __syncthreads();
+        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS) idx[k] = sorted[r0 + k];
         if (c < c1)
         {
             const int o = offsets[c] - r0, n = counts[c];
-            for (int i = 1; i < n; ++i)
-            {
-                const int32_t v = idx[o + i];
-                const double zv = Z ? key[o + i] : 0.0;
-                int j = i - 1;
-                while (j >= 0 && less(zv, v, Z ? key[o + j] : 0.0, idx[o + j]))
-                {
-                    idx[o + j + 1] = idx[o + j];
-                    if (Z) key[o + j + 1] = key[o + j];
-                    --j;
-                }
-                idx[o + j + 1] = v;
-                if (Z) key[o + j + 1] = zv;
-            }
+            cellStart[threadIdx.x] = o;
+            cellCount[threadIdx.x] = n;
+            for (int i = 0; i < n; ++i) cellOfRec[o + i] = uint8_t(threadIdx.x);
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS) sorted[r0 + k] = idx[k];
+        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS)
+        {
+            const int32_t v = idx[k];
+            const int cl = cellOfRec[k];
+            const int o = cellStart[cl], n = cellCount[cl];
+            int rank = 0;
+            for (int j = o; j < o + n; ++j) rank += idx[j] < v ? 1 : 0;
+            sorted[r0 + o + rank] = v;
+        }
         return;
     }
     if (c >= c1) return;
     int32_t* a = sorted + offsets[c];
     const int n = counts[c];
-    auto lessRec = [&](int32_t x, int32_t y) { return less(Z ? pos[x].z : 0.0, x, Z ? pos[y].z : 0.0, y); };
+    auto lessRec = [](int32_t x, int32_t y) { return x < y; };
     if (n <= 96)
     {
         for (int i = 1; i < n; ++i)
@@ -186,6 +150,15 @@ __global__ __launch_bounds__(SORT_CELLS) void k_binSortCells(const int32_t* offs
     }
 }
 
+// (z, index) order inside each cell: one group of 32 lanes per cell (sortCellByZ)
+__global__ __launch_bounds__(BLOCK) void k_sortCellsZ(const int32_t* offsets, const int32_t* counts, int32_t* sorted,
+                                                      int ncell, const double4* __restrict__ pos)
+{
+    const int64_t c = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SORT_GROUP;
+    if (c >= ncell) return;
+    sortCellByZ(sorted + offsets[c], counts[c], pos, int(threadIdx.x % SORT_GROUP));
+}
+
 __global__ void k_gatherPos(const double4* pos, const int32_t* sorted, int64_t n, double4* out)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -223,11 +196,11 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
         hipLaunchKernelGGL(k_binFill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, cellOf.p, b, n, offsets.p,
                            fill.p, sorted.p);
         if (zOrder)
-            hipLaunchKernelGGL(k_binSortCells<true>, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
-                               ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
+            hipLaunchKernelGGL(k_sortCellsZ, dim3(unsigned((int64_t(ncell) * SORT_GROUP + BLOCK - 1) / BLOCK)),
+                               dim3(BLOCK), 0, ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
         else
-            hipLaunchKernelGGL(k_binSortCells<false>, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
-                               ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
+            hipLaunchKernelGGL(k_binSortCells, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
+                               ctx->stream, offsets.p, counts.p, sorted.p, ncell);
     }
     if (withPos && n > 0)
     {

@@ -155,6 +155,53 @@ __device__ inline void putSentinel(void* st, int32_t maxStaged, double coord, do
     if (threadIdx.x == 0) Staged<XYZ>::put(st, sentinelSlot(maxStaged), make_double4(coord, coord, coord, w));
 }
 
+// Sort the records of one cell by (z, record index) in a group of 32 lanes (half a wave):
+// lane l holds record l of the cell (lanes past the count hold +infinity), a bitonic network
+// of 15 compare-exchange stages over lane shuffles orders them, and they are written back in
+// place.  A cell of more than 32 records (rare crowding) is insertion-sorted by lane 0 of the
+// group alone, in global memory.  `n` is the same for the 32 lanes of a group.
+constexpr int SORT_GROUP = 32;
+__device__ inline void sortCellByZ(int32_t* recs, int n, const double4* __restrict__ pos, int lane)
+{
+    if (n <= 1) return;
+    if (n <= SORT_GROUP)
+    {
+        int32_t v = lane < n ? recs[lane] : INT32_MAX;
+        double z = lane < n ? pos[v].z : HUGE_VAL;
+        for (int k = 2; k <= SORT_GROUP; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                const double zo = __shfl_xor(z, j, SORT_GROUP);
+                const int32_t vo = __shfl_xor(v, j, SORT_GROUP);
+                // the lower lane of a pair keeps the smaller record in an ascending block
+                // (lane & k == 0) and the larger in a descending one, the upper lane the other
+                const bool wantMin = ((lane & k) == 0) != ((lane & j) != 0);
+                const bool otherLess = zo < z || (zo == z && vo < v);
+                const bool selfLess = z < zo || (z == zo && v < vo);
+                if (wantMin ? otherLess : selfLess)
+                {
+                    z = zo;
+                    v = vo;
+                }
+            }
+        if (lane < n) recs[lane] = v;
+        return;
+    }
+    if (lane != 0) return;
+    for (int i = 1; i < n; ++i)
+    {
+        const int32_t v = recs[i];
+        const double zv = pos[v].z;
+        int j = i - 1;
+        while (j >= 0 && (zv < pos[recs[j]].z || (zv == pos[recs[j]].z && v < recs[j])))
+        {
+            recs[j + 1] = recs[j];
+            --j;
+        }
+        recs[j + 1] = v;
+    }
+}
+
 // copy the tile's neighbourhood (contiguous runs of sorted slots) into LDS; the copy is
 // unrolled so that four independent loads per lane are in flight
 template <bool XYZ>
