@@ -26,6 +26,17 @@ PRED_SLAB_AT, PRED_SLAB_HY, PRED_SLAB_CG, PRED_GREATER, PRED_LESS = 4, 5, 6, 7, 
 PAIR_ALWAYS, PAIR_AND, PAIR_OR = 0, 1, 2
 
 
+def _vec3(v):
+    """a per-axis triple of doubles (a scalar applies to all three axes); the library reads
+    exactly three"""
+    a = np.asarray(v, dtype=np.float64)
+    if a.size == 1:
+        a = np.full(3, float(a.reshape(-1)[0]))
+    if a.shape != (3,):
+        raise ValueError(f"expected 3 values per axis, got shape {a.shape}")
+    return np.ascontiguousarray(a)
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -346,9 +357,8 @@ class LinkedCellList:
     def build(self, atoms, begin, end, delta, gmin, gmax):
         """rebuild in place (buffers are reused)"""
         self.begin, self.end = begin, end
-        d = np.ascontiguousarray(np.broadcast_to(delta, 3), dtype=np.float64)
-        lo = np.ascontiguousarray(gmin, dtype=np.float64)
-        hi = np.ascontiguousarray(gmax, dtype=np.float64)
+        d = _vec3(delta)
+        lo, hi = _vec3(gmin), _vec3(gmax)
         self.ctx.check(_L.mrmd_cell_list_build(self.h, atoms.h, begin, end, _p(d), _p(lo), _p(hi)))
 
     def __del__(self):
@@ -387,8 +397,7 @@ class HalfVerletList:
             self.h = None
 
     def build(self, atoms, begin, end, r_n, cell_ratio, gmin, gmax, max_neigh=0, molecules=False):
-        lo = np.ascontiguousarray(gmin, dtype=np.float64)
-        hi = np.ascontiguousarray(gmax, dtype=np.float64)
+        lo, hi = _vec3(gmin), _vec3(gmax)
         f = _L.mrmd_verlet_build_molecules if molecules else _L.mrmd_verlet_build
         self.ctx.check(f(self.h, atoms.h, begin, end, float(r_n), float(cell_ratio), _p(lo), _p(hi), max_neigh))
 
