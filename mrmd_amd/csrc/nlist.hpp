@@ -156,35 +156,91 @@ __device__ inline void putSentinel(void* st, int32_t maxStaged, double coord, do
 }
 
 // Sort the records of one cell by (z, record index) in a group of 32 lanes (half a wave):
-// lane l holds record l of the cell (lanes past the count hold +infinity), a bitonic network
-// of 15 compare-exchange stages over lane shuffles orders them, and they are written back in
-// place.  A cell of more than 32 records (rare crowding) is insertion-sorted by lane 0 of the
-// group alone, in global memory.  `n` is the same for the 32 lanes of a group.
+// lane l holds records l, l + 32, ... (E per lane, E = 1 for cells of up to 32 records, 4 for
+// up to 128; the places past the count hold +infinity), a bitonic network orders them (pairs
+// 32 or more places apart inside a lane, nearer pairs across lanes by shuffles), and they are
+// written back in place.  A cell of more than 128 records (pathological crowding) is
+// insertion-sorted by lane 0 of the group alone, in global memory.  `n` is the same for the
+// 32 lanes of a group.
 constexpr int SORT_GROUP = 32;
+template <int E>
+__device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict__ pos, int lane)
+{
+    double z[E];
+    int32_t v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+    {
+        const int e = r * SORT_GROUP + lane;
+        v[r] = e < n ? recs[e] : INT32_MAX;
+        z[r] = e < n ? pos[v[r]].z : HUGE_VAL;
+    }
+    auto less = [](double za, int32_t va, double zb, int32_t vb) { return za < zb || (za == zb && va < vb); };
+#pragma unroll
+    for (int k = 2; k <= E * SORT_GROUP; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            if (j >= SORT_GROUP)
+            {
+                // partners in one lane: register r against r ^ (j / 32)
+                const int rj = j / SORT_GROUP;
+#pragma unroll
+                for (int r = 0; r < E; ++r)
+                {
+                    const int rp = r ^ rj;
+                    if (rp < r) continue;
+                    const int e = r * SORT_GROUP + lane;  // the lower place of the pair
+                    const bool asc = (e & k) == 0;
+                    if (asc == less(z[rp], v[rp], z[r], v[r]))
+                    {
+                        const double tz = z[r];
+                        const int32_t tv = v[r];
+                        z[r] = z[rp];
+                        v[r] = v[rp];
+                        z[rp] = tz;
+                        v[rp] = tv;
+                    }
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int r = 0; r < E; ++r)
+                {
+                    const int e = r * SORT_GROUP + lane;
+                    const double zo = __shfl_xor(z[r], j, SORT_GROUP);
+                    const int32_t vo = __shfl_xor(v[r], j, SORT_GROUP);
+                    // the lower place of a pair keeps the smaller record in an ascending block
+                    // (e & k == 0) and the larger in a descending one, the upper place the other
+                    const bool wantMin = ((e & k) == 0) != ((e & j) != 0);
+                    if (wantMin ? less(zo, vo, z[r], v[r]) : less(z[r], v[r], zo, vo))
+                    {
+                        z[r] = zo;
+                        v[r] = vo;
+                    }
+                }
+            }
+        }
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+    {
+        const int e = r * SORT_GROUP + lane;
+        if (e < n) recs[e] = v[r];
+    }
+}
+
 __device__ inline void sortCellByZ(int32_t* recs, int n, const double4* __restrict__ pos, int lane)
 {
     if (n <= 1) return;
     if (n <= SORT_GROUP)
     {
-        int32_t v = lane < n ? recs[lane] : INT32_MAX;
-        double z = lane < n ? pos[v].z : HUGE_VAL;
-        for (int k = 2; k <= SORT_GROUP; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1)
-            {
-                const double zo = __shfl_xor(z, j, SORT_GROUP);
-                const int32_t vo = __shfl_xor(v, j, SORT_GROUP);
-                // the lower lane of a pair keeps the smaller record in an ascending block
-                // (lane & k == 0) and the larger in a descending one, the upper lane the other
-                const bool wantMin = ((lane & k) == 0) != ((lane & j) != 0);
-                const bool otherLess = zo < z || (zo == z && vo < v);
-                const bool selfLess = z < zo || (z == zo && v < vo);
-                if (wantMin ? otherLess : selfLess)
-                {
-                    z = zo;
-                    v = vo;
-                }
-            }
-        if (lane < n) recs[lane] = v;
+        sortGroup<1>(recs, n, pos, lane);
+        return;
+    }
+    if (n <= 4 * SORT_GROUP)
+    {
+        sortGroup<4>(recs, n, pos, lane);
         return;
     }
     if (lane != 0) return;
