@@ -410,9 +410,11 @@ __global__ void k_bGather(BS b)
 
 // copy a tile's neighbourhood into LDS, shifting each piece by its periodic image: {x, y, z}
 // records (XYZ) or whole double4 records (the AdResS forces read the slab weight in w)
+// Returns whether a record this lane staged has a weight w > 0 (whole records only).
 template <bool XYZ>
-__device__ inline void stageShifted(const TileB& ti, const double4* __restrict__ sp, double* st, const RepD& R)
+__device__ inline bool stageShifted(const TileB& ti, const double4* __restrict__ sp, double* st, const RepD& R)
 {
+    bool weighted = false;
     int r = 0;
     for (int q = threadIdx.x; q < ti.staged; q += blockDim.x)
     {
@@ -428,8 +430,12 @@ __device__ inline void stageShifted(const TileB& ti, const double4* __restrict__
             st[3 * q + 2] = v.z;
         }
         else
+        {
             reinterpret_cast<double4*>(st)[q] = v;
+            weighted = weighted || v.w > 0.0;
+        }
     }
+    return weighted;
 }
 
 // A tile's records by staged slot: from LDS when its neighbourhood fits the stage area, else
@@ -785,8 +791,24 @@ __global__ __launch_bounds__(BLOCK) void k_bForce(BS b)
     __syncthreads();
     const bool fits = ti.staged <= b.stageCap;
     const RepD R = b.reps[G.rep];
-    if (fits) stageShifted<!W>(ti, b.sortedPos, stf, R);
-    __syncthreads();
+    const bool weighted = fits ? stageShifted<!W>(ti, b.sortedPos, stf, R) : true;
+    // an AdResS tile whose neighbourhood is all coarse-grained (every weight 0) has no pair
+    // (LJ_IdealGas.cpp:95), hence zero forces, drift and energy: the rows get the +0.0 that
+    // the pair loop would leave, without it
+    if (W && !__syncthreads_or(weighted))
+    {
+        for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
+        {
+            int sc;
+            const int32_t i = b.sorted[bTileRow(ti, t, sc)];
+            b.force[i] = 0.0;
+            b.force[b.cap + i] = 0.0;
+            b.force[2 * b.cap + i] = 0.0;
+        }
+        if (threadIdx.x == 0) b.ePart[t] = 0.0;
+        return;
+    }
+    if (!W) __syncthreads();
     double eTile;
     if (!fits)
         eTile = bForceRows<CAP, W>(b, ti, R, GlobalFetch{&ti, b.sortedPos, R.L[0], R.L[1], R.L[2]});
