@@ -473,6 +473,9 @@ __device__ inline unsigned xcdBlock()
 // exclusive scan of int64 counts on the device (small helper used by binning/compaction)
 void scanExclusiveI64(mrmd_ctx* ctx, const int32_t* in, int64_t* out, int64_t n, int64_t* totalDev);
 void scanExclusiveI32(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int32_t* totalDev);
+// `count` arrays of n values each, consecutive in memory (array k at in + k * n), scanned
+// independently; totals to totalsDev[0, count)
+void scanExclusiveI32Multi(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int count, int32_t* totalsDev);
 
 }  // namespace mrmd_hip
 

@@ -69,9 +69,49 @@ __global__ void k_tileScan(const int32_t* in, int64_t n, const TO* sums, TO* out
         }
 }
 
+// one workgroup per array (array k at in + k * n): exclusive scan in runs of TILE with a
+// carried sum, for arrays short enough that one launch beats the three of the tiled form
+constexpr int64_t SMALL_SCAN = 4 * TILE;
+template <class TO>
+__global__ __launch_bounds__(BLOCK) void k_scanSmall(const int32_t* in, TO* out, int64_t n, TO* total)
+{
+    __shared__ TO lds[BLOCK / WAVE];
+    in += int64_t(blockIdx.x) * n;
+    out += int64_t(blockIdx.x) * n;
+    TO carry = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += TILE)
+    {
+        const int64_t base = t0 + int64_t(threadIdx.x) * 4;
+        TO v[4];
+        TO s = 0;
+        for (int k = 0; k < 4; ++k)
+        {
+            v[k] = (base + k < n) ? TO(in[base + k]) : TO(0);
+            s += v[k];
+        }
+        TO tot;
+        TO ex = blockExclusiveScan<TO>(s, lds, &tot) + carry;
+        for (int k = 0; k < 4; ++k)
+            if (base + k < n)
+            {
+                out[base + k] = ex;
+                ex += v[k];
+            }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) total[blockIdx.x] = carry;
+}
+
 template <class TO>
 void scanImpl(mrmd_ctx* ctx, const int32_t* in, TO* out, int64_t n, TO* totalDev)
 {
+    if (n <= SMALL_SCAN)
+    {
+        hipLaunchKernelGGL(k_scanSmall<TO>, dim3(1), dim3(BLOCK), 0, ctx->stream, in, out, n, totalDev);
+        checkLaunch();
+        return;
+    }
     const int64_t tiles = std::max<int64_t>(1, (n + TILE - 1) / TILE);
     TO* sums = static_cast<TO*>(ctxScratch(ctx, 0, size_t(tiles) * sizeof(TO)));
     hipLaunchKernelGGL(k_tileSums<TO>, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, in, n, sums);
@@ -90,6 +130,17 @@ void scanExclusiveI64(mrmd_ctx* ctx, const int32_t* in, int64_t* out, int64_t n,
 void scanExclusiveI32(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int32_t* totalDev)
 {
     scanImpl<int32_t>(ctx, in, out, n, totalDev);
+}
+void scanExclusiveI32Multi(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int count, int32_t* totalsDev)
+{
+    if (n <= SMALL_SCAN)
+    {
+        hipLaunchKernelGGL(k_scanSmall<int32_t>, dim3(unsigned(count)), dim3(BLOCK), 0, ctx->stream, in, out, n,
+                           totalsDev);
+        checkLaunch();
+        return;
+    }
+    for (int k = 0; k < count; ++k) scanImpl<int32_t>(ctx, in + k * n, out + k * n, n, totalsDev + k);
 }
 
 void waitHostSeq(mrmd_ctx* ctx, int slot, double want)

@@ -416,8 +416,7 @@ void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int ax
     hipLaunchKernelGGL(k_ghostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->pos.p, total, axis,
                        s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, gl->tileCnt.p + tiles);
     checkLaunch();
-    scanExclusiveI32(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, tot);
-    scanExclusiveI32(ctx, gl->tileCnt.p + tiles, gl->tileOff.p + tiles, tiles, tot + 1);
+    scanExclusiveI32Multi(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, 2, tot);
     int32_t h[2];
     readbackI32(ctx, tot, 2, h);
     const int64_t nP = h[0], nN = h[1];
@@ -450,7 +449,7 @@ void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const
     hipLaunchKernelGGL(k_molGhostCount, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(), totM, axis,
                        s.minInner[axis], s.maxInner[axis], gl->tileCnt.p, tiles);
     checkLaunch();
-    for (int k = 0; k < 4; ++k) scanExclusiveI32(ctx, gl->tileCnt.p + k * tiles, gl->tileOff.p + k * tiles, tiles, tot + k);
+    scanExclusiveI32Multi(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, 4, tot);
     int32_t h[4];
     readbackI32(ctx, tot, 4, h);
     const int64_t newM = totM + h[0] + h[2], newA = totA + h[1] + h[3];
