@@ -235,6 +235,7 @@ struct IGTiled
     int queueCap;          ///< queued non-row partner pairs per workgroup (<= IG_QUEUE)
     int32_t maxStaged;     ///< the sentinel record sits at sentinelSlot(maxStaged)
     double sentinel;       ///< at (sentinel, sentinel, sentinel)
+    const int8_t* skip;    ///< per tile: 1 = no record of its neighbourhood can be weighted (or null)
 };
 
 // sorted-slot record of each molecule's single atom and its modulated lambda (in [0, 1];
@@ -430,6 +431,12 @@ __global__ __launch_bounds__(BLOCK) void k_igTiled(IGArgs g, IGTiled tl, Pre p, 
     // atomistic and hybrid slab, one contiguous range of tile indices, which an
     // XCD-contiguous remap would hand to two or three XCDs only
     const unsigned blk = blockIdx.x;
+    // a tile that the banded list build found out of reach of the weighted region
+    if (tl.skip && tl.skip[blk])
+    {
+        if (threadIdx.x == 0) partials[blk] = 0.0;
+        return;
+    }
     if (threadIdx.x == 0)
     {
         ti = tl.tiles[blk];
@@ -912,7 +919,7 @@ int mrmd_ljig_run(mrmd_lj_idealgas* s, mrmd_molecules* m, mrmd_verlet_list* vl, 
                                vl->bin.sorted.p, vl->nrows, rows, s->igPos.p);
         }
         const IGTiled tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p, s->igPos.p, rows,
-                         s->queueCap, vl->maxStaged, vl->sentinel};
+                         s->queueCap, vl->maxStaged, vl->sentinel, vl->banded ? vl->tileSkip.p : nullptr};
         const Pre p = s->table[0];
         const size_t lds = size_t(sentinelSlot(vl->maxStaged) + 1) * sizeof(double4);
         if (p.capDistSqr > 0.0)

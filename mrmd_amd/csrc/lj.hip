@@ -739,6 +739,8 @@ bool ljApplyImpl(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, const mrmd_p
         if (full)
         {
             REQUIRE(vl->ntiles > 0, "the Verlet list has not been built");
+            REQUIRE(!vl->banded, "the list holds only the rows within reach of the slab's weighted region (built "
+                                 "inside the AdResS time loop); rebuild it for LennardJones");
             if (!fresh) vl->refreshSorted(at->pos.p);
             KTimer t(ctx, "lj_force");
             const TiledList tl{vl->nbr16.p, vl->cnt16.p, vl->strideS, vl->tiles.p, vl->bin.sorted.p,
@@ -861,6 +863,8 @@ int mrmd_lj_apply_noalloc(mrmd_lj* lj, mrmd_atoms* at, mrmd_verlet_list* vl, dou
     if (!lj->symmetric) fail(MRMD_ERR_UNSUP, "the allocation-free apply serves symmetric tables (the full-row kernel) only");
     REQUIRE(vl->begin == 0 && vl->end <= at->nlocal + at->nghost, "the list rows must start at record 0");
     REQUIRE(vl->nrows <= at->size, "the list was built for more records than the atoms hold");
+    REQUIRE(!vl->banded, "the list holds only the rows within reach of the slab's weighted region (built inside "
+                         "the AdResS time loop); rebuild it for LennardJones");
     REQUIRE(at->maxType * lj->typeStride + at->maxType < int64_t(lj->table.size()),
             "atom type %lld has no entry in the %lld-type table", (long long)at->maxType, (long long)lj->numTypes);
     if (vl->ntiles > LJ_NOALLOC_MAX_TILES)

@@ -167,9 +167,26 @@ int mrmd_adress_md_run(mrmd_atoms* a, mrmd_molecules* m, mrmd_ghost_layer* gl, m
             MD_CALL(mrmd_multires_wrap(m, a, box));
             MD_CALL(mrmd_multires_halo_create(gl, m, a, box));
             MD_CALL(mrmd_molecules_update(m, a, nullptr));
-            MD_CALL(mrmd_verlet_build_molecules(vl, m, 0, m->nlocal, rn, 1.0, gridMin, gridMax, 0));
+            // monatomic molecules (the fused layout): rows only where a pair can carry a weight
+            // before the next rebuild, the x band of the slab's atomistic and hybrid regions
+            // widened by the displacement that triggers a rebuild (a molecule's move since
+            // the build stays below skin / 2); a layout that is no longer monatomic builds
+            // again in full
+            if (fused && slab)
+            {
+                const double reach = 0.5 * slab->atomistic_diameter + slab->hybrid_diameter + 0.5 * s->skin;
+                vl->bandLo = slab->center[0] - reach;
+                vl->bandHi = slab->center[0] + reach;
+            }
+            const int built = mrmd_verlet_build_molecules(vl, m, 0, m->nlocal, rn, 1.0, gridMin, gridMax, 0);
+            vl->bandLo = 1.0;
+            vl->bandHi = -1.0;
+            MD_CALL(built);
             ++s->rebuilds;
+            const bool wasBanded = vl->banded;
             fused = adressIdentityLayout(m, a);
+            if (wasBanded && !fused)
+                MD_CALL(mrmd_verlet_build_molecules(vl, m, 0, m->nlocal, rn, 1.0, gridMin, gridMax, 0));
         }
         if (fused)
             adressPrepare(a, m, rebuild ? nullptr : gl, box, slab, ig, vl);

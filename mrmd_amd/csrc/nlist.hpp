@@ -321,6 +321,14 @@ struct mrmd_verlet_list
     /// half flag (bit 15)
     mrmd_hip::DBuf<uint16_t> nbr16;
     mrmd_hip::DBuf<int32_t> cnt16;  ///< count per sorted slot (0 for non-row records)
+    /// The AdResS time loop builds only the rows that can interact before the next rebuild:
+    /// with bandLo <= bandHi, a tile whose neighbourhood lies wholly outside the x band
+    /// [bandLo, bandHi] (records that may carry a slab weight before the next rebuild) gets
+    /// empty rows and tileSkip[t] = 1, which the tiled LJ_IdealGas kernel skips.  Such a list
+    /// (`banded`) serves that kernel only; every other use asks for a full build.
+    double bandLo = 1.0, bandHi = -1.0;
+    bool banded = false;
+    mrmd_hip::DBuf<int8_t> tileSkip;
 
     // ---- global-index form (rows = records begin+r), converted on demand ----
     bool globalValid = false;

@@ -170,7 +170,8 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
                                                        const TileInfo* __restrict__ tiles, TileGeom tg, GridD g,
                                                        int64_t rowLo, int64_t rowHi, double rsq, uint16_t* nbr,
                                                        int64_t strideS, int cap, int32_t* cnt, int32_t* maxCnt,
-                                                       int stagedCap, double4* gstage, int64_t gstageRecs)
+                                                       int stagedCap, double4* gstage, int64_t gstageRecs,
+                                                       double bandLo, double bandHi, int8_t* tileSkip)
 {
     constexpr int S = 2 * R + 1;
     extern __shared__ double4 stRaw[];
@@ -191,6 +192,25 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
     if (ti.nrows == 0) return;
     int i0, j0, k0, k1;
     tileCoords(tg, t, i0, j0, k0, k1);
+    if (bandLo <= bandHi)
+    {
+        // x range of the neighbourhood's cells (the edge cells take every record beyond
+        // the grid; widened by 1e-9 of a cell for records binned across a face by rounding)
+        const int ia = max(i0 - R, 0), ib = min(i0 + tg.cols - 1 + R, tg.n0 - 1);
+        const double xlo = ia == 0 ? -HUGE_VAL : g.min[0] + double(ia) * g.dx[0] - 1e-9 * g.dx[0];
+        const double xhi = ib == tg.n0 - 1 ? HUGE_VAL : g.min[0] + double(ib + 1) * g.dx[0] + 1e-9 * g.dx[0];
+        const bool skip = xhi < bandLo || xlo > bandHi;
+        if (threadIdx.x == 0) tileSkip[t] = skip ? 1 : 0;
+        if (skip)
+        {
+            for (int row = threadIdx.x; row < ti.nrows; row += blockDim.x)
+            {
+                int self;
+                cnt[tileRow(ti, row, self)] = 0;
+            }
+            return;
+        }
+    }
     const int span = tg.cols + 2 * R;
     const int kl = max(k0 - R, 0), nk = min(k1 + R, tg.n2) - kl;
     // the first row's dependent loads (slot -> record -> cell) are issued before the staging
@@ -613,6 +633,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     rowRegion(vl);
     vl->globalValid = false;
     vl->halfValid = false;
+    vl->banded = vl->bandLo <= vl->bandHi;
     vl->slotOfValid = false;
     vl->sortedFresh = false;
     // tile shape: a x a columns (a <= 3; 1 when the stencil reaches two cells), height K
@@ -655,6 +676,7 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
         vl->gstageRecs = (int64_t(vl->maxStaged) + STAGE_PAD + 1 + 3) / 4 * 4;
         vl->gstage.ensure(ctx, std::max<int64_t>(vl->ntiles, 1) * vl->gstageRecs);
     }
+    if (vl->banded) vl->tileSkip.ensure(ctx, std::max<int64_t>(vl->ntiles, 1));
     vl->strideS = std::max<int64_t>(64, (nall + 63) / 64 * 64);
     vl->cnt16.ensure(ctx, vl->strideS);
     hipLaunchKernelGGL(k_zeroI32v, dim3(gridFor(nall)), dim3(BLOCK), 0, ctx->stream, vl->cnt16.p, nall);
@@ -681,7 +703,8 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
                 hipLaunchKernelGGL(kernel, dim3(unsigned(vl->ntiles)), dim3(vl->block), lds, ctx->stream,
                                    vl->bin.sortedPos.p, vl->bin.sorted.p, vl->bin.cellOf.p, vl->bin.offsets.p,
                                    vl->tiles.p, geom(vl), g, begin, end, rn * rn, vl->nbr16.p, vl->strideS,
-                                   vl->capFull, vl->cnt16.p, dmax, stagedCap, vl->gstage.p, vl->gstageRecs);
+                                   vl->capFull, vl->cnt16.p, dmax, stagedCap, vl->gstage.p, vl->gstageRecs,
+                                   vl->bandLo, vl->bandHi, vl->tileSkip.p);
             };
             const bool wide = vl->forceWide || int64_t(vl->capFull) * vl->strideS >= (int64_t(1) << 31);
             // plain block order: measured faster here than the XCD-contiguous order (1.54 against
@@ -737,6 +760,9 @@ const int32_t* mrmd_verlet_list::ensureSlotOf()
 
 void mrmd_verlet_list::ensureGlobal()
 {
+    if (banded)
+        fail(MRMD_ERR_STATE, "the list holds only the rows within reach of the slab's weighted region (built "
+                             "inside the AdResS time loop); rebuild it for this use");
     if (globalValid) return;
     const int64_t rows = end - begin;
     stride = std::max<int64_t>(64, (rows + 63) / 64 * 64);
@@ -930,6 +956,7 @@ int mrmd_verlet_count_within(mrmd_verlet_list* vl, const mrmd_atoms* a, double r
 int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* maxFull, int64_t* totalFull, int32_t* dims)
 {
     if (!vl) return MRMD_ERR_ARG;
+    MRMD_API_BEGIN(vl->ctx)
     if (rows) *rows = vl->end - vl->begin;
     if (maxFull) *maxFull = vl->maxFull;
     if (totalFull)
@@ -939,15 +966,9 @@ int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* maxFull
         std::vector<int32_t> h(static_cast<size_t>(r));
         if (r)
         {
-            try
-            {
-                v->ensureGlobal();
-            }
-            catch (...)
-            {
-                return MRMD_ERR_HIP;
-            }
-            if (hipMemcpy(h.data(), v->cnt.p, size_t(r) * 4, hipMemcpyDeviceToHost) != hipSuccess) return MRMD_ERR_HIP;
+            v->ensureGlobal();
+            HIPCHECK(hipMemcpyAsync(h.data(), v->cnt.p, size_t(r) * 4, hipMemcpyDeviceToHost, v->ctx->stream));
+            sync(v->ctx);
         }
         int64_t t = 0;
         for (auto c : h) t += c;
@@ -955,7 +976,7 @@ int mrmd_verlet_info(const mrmd_verlet_list* vl, int64_t* rows, int32_t* maxFull
     }
     if (dims)
         for (int d = 0; d < 3; ++d) dims[d] = vl->bin.grid.n[d];
-    return MRMD_OK;
+    MRMD_API_END
 }
 
 }  // extern "C"

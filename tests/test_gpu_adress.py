@@ -342,3 +342,30 @@ def test_adress_native_loop_equals_step_calls(M, ctx):
     d -= np.asarray(L) * np.round(d / np.asarray(L))
     assert np.abs(d).max() < 1e-9, np.abs(d).max()
     assert np.abs(a.tf.get_force() - b.tf.get_force()).max() < 1e-9
+
+
+def test_adress_loop_builds_rows_near_the_weighted_region_only(M, ctx):
+    """The native AdResS loop builds list rows only in tiles within reach of the slab's
+    atomistic and hybrid regions (x band widened by half the skin): on a long slab most
+    tiles are coarse-grained and skipped.  Its trajectory equals the step-by-step calls,
+    whose lists are full, and the banded list refuses uses other than LJ_IdealGas."""
+    from mrmd_amd.drivers import AdResSSystem
+    from mrmd_amd.lattice import maxwell_boltzmann
+    L = [80.0, 12.0, 12.0]
+    g = np.random.default_rng(5)
+    grid = np.stack(np.meshgrid(np.arange(64) * 1.25, np.arange(10) * 1.2, np.arange(10) * 1.2, indexing="ij"), -1)
+    pos = np.mod(grid.reshape(-1, 3) + 0.3 + g.uniform(-0.05, 0.05, (6400, 3)), L)
+    vel = maxwell_boltzmann(pos.shape[0], 1.0, 9)
+    kw = dict(cap=0.7, seed=13, tf_every=10)
+    a = AdResSSystem(ctx, pos, vel, L, 8.0, 5.0, 1, **kw)
+    b = AdResSSystem(ctx, pos, vel, L, 8.0, 5.0, 1, **kw)
+    for _ in range(30):
+        a.step(0.005)
+    b.run(30, 0.005)
+    assert a.rebuilds == b.rebuilds and a.rebuilds >= 2
+    d = a.local(M.POS) - b.local(M.POS)
+    d -= np.asarray(L) * np.round(d / np.asarray(L))
+    assert np.abs(d).max() < 1e-9, np.abs(d).max()
+    assert a.energy == pytest.approx(b.energy, rel=1e-10)
+    with pytest.raises(M.MrmdError, match="weighted region"):
+        b.vl.info()
