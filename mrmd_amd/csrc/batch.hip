@@ -610,6 +610,33 @@ __global__ __launch_bounds__(BLOCK) void k_bBuild(BS b)
     __syncthreads();
     if (ti.nrows == 0) return;
     const RepD R = b.reps[G.rep];
+    if (R.adress)
+    {
+        // an AdResS tile whose staged columns all lie out of reach of the weighted region
+        // (the slab's atomistic and hybrid x band widened by the displacement that triggers
+        // a rebuild; weights come from the records' own x in [0, L)) keeps no pair before its
+        // next rebuild: empty rows, and the force kernel finds every weight 0
+        const double reach = 0.5 * R.slab.atomistic_diameter + R.slab.hybrid_diameter + R.halfSkin;
+        const double lo = R.slab.center[0] - reach, hi = R.slab.center[0] + reach;
+        bool near = false;
+        for (int sa = 0; sa < G.ni + 2; ++sa)
+        {
+            int ii = G.i0 - 1 + sa;
+            ii = ii < 0 ? ii + R.nc[0] : (ii >= R.nc[0] ? ii - R.nc[0] : ii);
+            const double xlo = ii == 0 ? -HUGE_VAL : double(ii) * R.dxc[0] - 1e-9 * R.dxc[0];
+            const double xhi = ii == R.nc[0] - 1 ? HUGE_VAL : double(ii + 1) * R.dxc[0] + 1e-9 * R.dxc[0];
+            near = near || !(xhi < lo || xlo > hi);
+        }
+        if (!near)
+        {
+            for (int t = threadIdx.x; t < ti.nrows; t += blockDim.x)
+            {
+                int sc;
+                b.cnt[bTileRow(ti, t, sc)] = 0;
+            }
+            return;
+        }
+    }
     const bool fits = ti.staged <= b.stageCap;
     if (fits) stageShifted<true>(ti, b.sortedPos, stb, R);
     // LDS slot of the first record of cell k0 - 1 + z of each staged column
