@@ -143,7 +143,11 @@ __device__ inline int locate(double x, double rdx, int n)
 // ---------------------------------------------------------------------------------------
 // integration
 
-// VelocityVerletLangevinThermostat::preForceIntegrate (VelocityVerletLangevinThermostat.hpp:63-136)
+// VelocityVerletLangevinThermostat::preForceIntegrate (VelocityVerletLangevinThermostat.hpp:63-136);
+// POST: the previous step's post-force integration (k_bPost) first, in the same pass: the
+// thermodynamic force added to F_x, the kick, then this step's kick with the same force,
+// the expressions and order of the two kernels
+template <bool POST>
 __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
 {
     __shared__ WaveDraws draws[BLOCK / WAVE];
@@ -164,9 +168,23 @@ __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
         vy = b.vel[cap + i];
         vz = b.vel[2 * cap + i];
         const double dtfm = dtHalf / m;
-        vx += dtfm * b.force[i];
-        vy += dtfm * b.force[cap + i];
-        vz += dtfm * b.force[2 * cap + i];
+        double fx = b.force[i];
+        const double fy = b.force[cap + i], fz = b.force[2 * cap + i];
+        if (POST)
+        {
+            if (R->adress)
+            {
+                const int64_t bin = getBin(p.x, 0.0, R->invBinWidth, R->nbins);
+                if (bin != -1) fx += b.tfForce[R->h0 + bin];
+            }
+            const double dtfmPost = (0.5 * R->dt) / R->mass;
+            vx += dtfmPost * fx;
+            vy += dtfmPost * fy;
+            vz += dtfmPost * fz;
+        }
+        vx += dtfm * fx;
+        vy += dtfm * fy;
+        vz += dtfm * fz;
         ox = p.x;
         oy = p.y;
         oz = p.z;
@@ -1022,7 +1040,7 @@ struct mrmd_batch
     }
     BS state();
     void layoutTiles();
-    void enqueueStep();
+    void enqueueSteps(int64_t n);
 };
 
 BS mrmd_batch::state()
@@ -1134,46 +1152,55 @@ void mrmd_batch::layoutTiles()
     dropGraph();
 }
 
-void mrmd_batch::enqueueStep()
+// n steps; a step's post-force integration and the next step's pre-force integration run
+// as one pass (k_bPre<true>), the first pre-force and the last post-force pass alone
+void mrmd_batch::enqueueSteps(int64_t n)
 {
+    if (n <= 0) return;
     mrmd_ctx* c = ctx;
     const BS b = state();
     const unsigned ga = gridFor(natoms), gc = gridFor(ncells), gt = gridFor(ntiles);
     const unsigned gs = unsigned((ncells * SORT_GROUP + BLOCK - 1) / BLOCK);
     const unsigned nr = unsigned(reps.size());
-    hipLaunchKernelGGL(k_bPre, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bCheck, dim3(gridFor(nr)), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bWrap, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bZeroCells, dim3(gc), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bCount, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,
-                       b);
-    auto force = [&](auto kernel, int64_t t0, int64_t t1, int capRecs, size_t bytes) {
-        if (t1 <= t0) return;
-        BS f = b;
-        f.tileBase = int32_t(t0);
-        f.stageCap = capRecs;
-        hipLaunchKernelGGL(kernel, dim3(unsigned(t1 - t0)), dim3(block), size_t(capRecs) * bytes, c->stream, f);
-    };
-    if (cap_)
+    hipLaunchKernelGGL(k_bPre<false>, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+    for (int64_t k = 0; k < n; ++k)
     {
-        force(k_bForce<true, false>, 0, plainTiles, stageCapPlain, 24);
-        force(k_bForce<true, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
+        hipLaunchKernelGGL(k_bCheck, dim3(gridFor(nr)), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bWrap, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bZeroCells, dim3(gc), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bCount, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,
+                           b);
+        auto force = [&](auto kernel, int64_t t0, int64_t t1, int capRecs, size_t bytes) {
+            if (t1 <= t0) return;
+            BS f = b;
+            f.tileBase = int32_t(t0);
+            f.stageCap = capRecs;
+            hipLaunchKernelGGL(kernel, dim3(unsigned(t1 - t0)), dim3(block), size_t(capRecs) * bytes, c->stream, f);
+        };
+        if (cap_)
+        {
+            force(k_bForce<true, false>, 0, plainTiles, stageCapPlain, 24);
+            force(k_bForce<true, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
+        }
+        else
+        {
+            force(k_bForce<false, false>, 0, plainTiles, stageCapPlain, 24);
+            force(k_bForce<false, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
+        }
+        hipLaunchKernelGGL(k_bCompUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bTFUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bStepEnd, dim3(1), dim3(64), 0, c->stream, b);
+        if (k + 1 < n)
+            hipLaunchKernelGGL(k_bPre<true>, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        else
+            hipLaunchKernelGGL(k_bPost, dim3(ga), dim3(BLOCK), 0, c->stream, b);
     }
-    else
-    {
-        force(k_bForce<false, false>, 0, plainTiles, stageCapPlain, 24);
-        force(k_bForce<false, true>, plainTiles, ntiles, stageCapAdress, sizeof(double4));
-    }
-    hipLaunchKernelGGL(k_bCompUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bTFUpdate, dim3(nr), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bPost, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-    hipLaunchKernelGGL(k_bStepEnd, dim3(1), dim3(64), 0, c->stream, b);
 }
 
 // Checkpoint blob of the ensemble's dynamic state: a header of eight int64 (magic, version,
@@ -1509,14 +1536,14 @@ int mrmd_batch_run(mrmd_batch* b, int64_t nsteps, int graphSteps)
         {
             b->dropGraph();
             HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int s = 0; s < graphSteps; ++s) b->enqueueStep();
+            b->enqueueSteps(graphSteps);
             HIPCHECK(hipStreamEndCapture(c->stream, &b->graph));
             HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             b->graphSteps = graphSteps;
         }
         for (; done + graphSteps <= nsteps; done += graphSteps) HIPCHECK(hipGraphLaunch(b->exec, c->stream));
     }
-    for (; done < nsteps; ++done) b->enqueueStep();
+    if (done < nsteps) b->enqueueSteps(nsteps - done);
     checkLaunch();
     int32_t e = 0;
     HIPCHECK(hipMemcpyAsync(&e, b->err.p, 4, hipMemcpyDeviceToHost, c->stream));
