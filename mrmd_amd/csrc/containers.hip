@@ -133,7 +133,9 @@ void scanExclusiveI32(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n,
 }
 void scanExclusiveI32Multi(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, int count, int32_t* totalsDev)
 {
-    if (n <= SMALL_SCAN)
+    // one workgroup per array, looping over runs of TILE: one launch instead of three per
+    // array while the runs stay few
+    if (n <= (count > 1 ? 16 : 1) * SMALL_SCAN)
     {
         hipLaunchKernelGGL(k_scanSmall<int32_t>, dim3(unsigned(count)), dim3(BLOCK), 0, ctx->stream, in, out, n,
                            totalsDev);

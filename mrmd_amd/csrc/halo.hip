@@ -15,8 +15,9 @@ struct mrmd_ghost_layer
     mrmd_hip::CtxPtr ctx;
     DBuf<int32_t> corr;  ///< correspondingRealAtom, -1 for real atoms
     int64_t corrSize = 0;
-    DBuf<int32_t> tileCnt;   ///< 4 x tiles counts
-    DBuf<int32_t> tileOff;   ///< 4 x tiles exclusive offsets
+    DBuf<int32_t> tileCnt;   ///< per-tile counts (2 arrays per axis, 26 for the three-axis pass)
+    DBuf<int32_t> tileOff;   ///< their exclusive offsets
+    DBuf<int32_t> blocks;    ///< three-axis pass: 26 totals, 27 bases by block code, the ghost count
 };
 
 namespace mrmd_hip
@@ -176,6 +177,178 @@ __global__ void k_ghostScatter(AtomsView a, int32_t* corr, int64_t total, int ax
         }
         runP += tP;
         runN += tN;
+    }
+}
+
+// The three axes of GhostExchange::createGhostAtoms (GhostExchange.cpp:59-169) in one pass
+// over the real atoms.  Running axis by axis, the exchange copies every record (real atoms,
+// then the ghosts made so far, in index order) with coordinate < minInner by +L and >=
+// maxInner by -L; a ghost keeps its source's other coordinates, so whether it is copied
+// again along a later axis is its real atom's test along that axis.  Every ghost is thus
+// one real atom i and a shift code b = sx + 3 sy + 9 sz (per axis 0: not copied, 1: +L,
+// 2: -L; b != 0), and the exchange's output is 26 blocks, each the atoms of its code in
+// index order, in the order ghostBlockOrder gives: x: +, -; y: + of (real, x+, x-), then
+// - of the same; z: + of (real, x+, x-, y+ of the three, y- of the three), then - of the
+// same.  Positions take each axis's shift once, as the axis-by-axis copies do.
+constexpr int TILE3 = BLOCK * 4;
+constexpr int GHOST_BLOCKS = 26;
+
+__host__ __device__ inline int ghostBlockOrder(int b)
+{
+    const int sx = b % 3, sy = (b / 3) % 3, sz = b / 9;
+    if (sz) return 8 + (sz - 1) * 9 + sy * 3 + sx;
+    if (sy) return 2 + (sy - 1) * 3 + sx;
+    return sx - 1;
+}
+
+// bit 2d: coordinate d < minInner (copied +L along d), bit 2d + 1: >= maxInner (-L)
+__device__ inline unsigned ghostFlags(const double4& p, const SubD& s)
+{
+    return unsigned(p.x < s.minInner[0]) | (unsigned(p.x >= s.maxInner[0]) << 1) |
+           (unsigned(p.y < s.minInner[1]) << 2) | (unsigned(p.y >= s.maxInner[1]) << 3) |
+           (unsigned(p.z < s.minInner[2]) << 4) | (unsigned(p.z >= s.maxInner[2]) << 5);
+}
+
+// the codes an atom's flags put it in: bit b for every b != 0 with, per axis, s = 0 or the
+// flag of s set
+__device__ inline unsigned ghostCodes(unsigned f)
+{
+    const unsigned mx = 1u | ((f & 3u) << 1), my = 1u | (((f >> 2) & 3u) << 1), mz = 1u | (((f >> 4) & 3u) << 1);
+    unsigned mxy = 0, m = 0;
+    for (int k = 0; k < 3; ++k)
+        if ((my >> k) & 1u) mxy |= mx << (3 * k);
+    for (int k = 0; k < 3; ++k)
+        if ((mz >> k) & 1u) m |= mxy << (9 * k);
+    return f ? m & ~1u : 0u;
+}
+
+// the codes present in the wave
+__device__ inline unsigned waveCodes(unsigned m)
+{
+    for (int off = WAVE / 2; off > 0; off >>= 1) m |= __shfl_xor(m, off, WAVE);
+    return m;
+}
+
+// per tile of TILE3 atoms: atoms of each code, cnt[(b - 1) * tiles + tile].  Each lane
+// loads its ROUNDS positions first; a wave then counts only the codes present in it.
+constexpr int ROUNDS = TILE3 / BLOCK;
+__global__ __launch_bounds__(BLOCK) void k_ghostCount3(const double4* pos, int64_t n, SubD s, int32_t* cnt,
+                                                       int64_t tiles)
+{
+    __shared__ int sc[BLOCK / WAVE][GHOST_BLOCKS];
+    const int64_t base = int64_t(blockIdx.x) * TILE3;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    unsigned codes[ROUNDS], all = 0;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        codes[r] = e < n ? ghostCodes(ghostFlags(pos[e], s)) : 0u;
+        all |= codes[r];
+    }
+    int acc = 0;  // lane b - 1: this wave's atoms of code b
+    for (unsigned present = __ballot(all != 0) ? waveCodes(all) : 0u; present; present &= present - 1)
+    {
+        const int b = __builtin_ctz(present);
+        int c = 0;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) c += __popcll(__ballot((codes[r] >> b) & 1u));
+        if (lane == b - 1) acc = c;
+    }
+    if (lane < GHOST_BLOCKS) sc[w][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < GHOST_BLOCKS)
+    {
+        int t = 0;
+        for (int q = 0; q < BLOCK / WAVE; ++q) t += sc[q][threadIdx.x];
+        cnt[int64_t(threadIdx.x) * tiles + blockIdx.x] = t;
+    }
+}
+
+// block bases in output order from the 26 totals (tot[b - 1]); bases[b] by code, *count = sum
+__global__ void k_ghostBases(const int32_t* tot, int32_t* bases, int32_t* count)
+{
+    if (threadIdx.x != 0) return;
+    int byOrder[GHOST_BLOCKS];
+    for (int b = 1; b <= GHOST_BLOCKS; ++b) byOrder[ghostBlockOrder(b)] = tot[b - 1];
+    int start[GHOST_BLOCKS], acc = 0;
+    for (int o = 0; o < GHOST_BLOCKS; ++o)
+    {
+        start[o] = acc;
+        acc += byOrder[o];
+    }
+    bases[0] = 0;
+    for (int b = 1; b <= GHOST_BLOCKS; ++b) bases[b] = start[ghostBlockOrder(b)];
+    *count = acc;
+}
+
+// every real atom writes its ghosts: ghost index n + bases[b] + the tile's offset + its rank
+// in the tile among the atoms of code b (records of the tile in index order: round by round,
+// wave by wave, lane by lane).  Waves without a flagged atom only take part in the counts.
+__global__ __launch_bounds__(BLOCK) void k_ghostScatter3(AtomsView a, int32_t* corr, int64_t n, SubD s,
+                                                         const int32_t* off, int64_t tiles, const int32_t* bases)
+{
+    __shared__ int wc[BLOCK / WAVE][GHOST_BLOCKS];
+    const int64_t base = int64_t(blockIdx.x) * TILE3;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    // lane b - 1: ghosts of code b of this tile written before this round
+    int run = lane < GHOST_BLOCKS ? bases[lane + 1] + off[int64_t(lane) * tiles + blockIdx.x] : 0;
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+        unsigned codes = 0;
+        if (e < n)
+        {
+            p = a.pos[e];
+            codes = ghostCodes(ghostFlags(p, s));
+        }
+        const unsigned present = __ballot(codes != 0) ? waveCodes(codes) : 0u;
+        int mine = 0;
+        for (unsigned pr = present; pr; pr &= pr - 1)
+        {
+            const int b = __builtin_ctz(pr);
+            const int c = __popcll(__ballot((codes >> b) & 1u));
+            if (lane == b - 1) mine = c;
+        }
+        if (lane < GHOST_BLOCKS) wc[w][lane] = mine;
+        __syncthreads();
+        int before = 0, total = 0;
+        if (lane < GHOST_BLOCKS)
+            for (int q = 0; q < BLOCK / WAVE; ++q)
+            {
+                const int c = wc[q][lane];
+                before += q < w ? c : 0;
+                total += c;
+            }
+        __syncthreads();
+        for (unsigned pr = present; pr; pr &= pr - 1)
+        {
+            const int b = __builtin_ctz(pr);
+            const bool m = (codes >> b) & 1u;
+            const unsigned long long mask = __ballot(m);
+            const int st = __shfl(run + before, b - 1, WAVE);
+            if (m)
+            {
+                const int64_t g = n + st + waveRank(mask);
+                const int sx = b % 3, sy = (b / 3) % 3, sz = b / 9;
+                double4 q = p;
+                if (sx) q.x += sx == 1 ? s.diam[0] : -s.diam[0];
+                if (sy) q.y += sy == 1 ? s.diam[1] : -s.diam[1];
+                if (sz) q.z += sz == 1 ? s.diam[2] : -s.diam[2];
+                a.pos[g] = q;
+                for (int d = 0; d < 3; ++d)
+                {
+                    a.vel[d * a.cap + g] = a.vel[d * a.cap + e];
+                    a.force[d * a.cap + g] = a.force[d * a.cap + e];
+                }
+                a.mass[g] = a.mass[e];
+                a.charge[g] = a.charge[e];
+                a.relmass[g] = a.relmass[e];
+                corr[g] = int32_t(e);
+            }
+        }
+        run += total;
     }
 }
 
@@ -436,6 +609,43 @@ void createAxis(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, int ax
     a->nghost += nP + nN;
 }
 
+// the three axes in one pass over the real atoms (no ghosts yet): the same ghosts in the same
+// order as createAxis for x, y and z
+void createAll(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
+{
+    mrmd_ctx* ctx = gl->ctx;
+    const SubD s = makeSub(box);
+    const int64_t n = a->nlocal;
+    if (n >= (int64_t(1) << 31) - 1) fail(MRMD_ERR_UNSUP, "more than 2^31 records");
+    const int64_t tiles = std::max<int64_t>(1, (n + TILE3 - 1) / TILE3);
+    gl->tileCnt.ensure(ctx, GHOST_BLOCKS * tiles);
+    gl->tileOff.ensure(ctx, GHOST_BLOCKS * tiles);
+    gl->blocks.ensure(ctx, 64);
+    int32_t* tot = gl->blocks.p;
+    int32_t* bases = tot + 32;
+    int32_t* count = bases + 27;
+    hipLaunchKernelGGL(k_ghostCount3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->pos.p, n, s,
+                       gl->tileCnt.p, tiles);
+    checkLaunch();
+    scanExclusiveI32Multi(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, GHOST_BLOCKS, tot);
+    hipLaunchKernelGGL(k_ghostBases, dim3(1), dim3(64), 0, ctx->stream, tot, bases, count);
+    checkLaunch();
+    int32_t h[1];
+    readbackI32(ctx, count, 1, h);
+    const int64_t newTotal = n + h[0];
+    a->ensureCap(newTotal);
+    a->size = newTotal;
+    if (gl->corr.n < newTotal) gl->corr.reserve(ctx, growCap(newTotal), gl->corrSize);
+    if (h[0] > 0)
+    {
+        hipLaunchKernelGGL(k_ghostScatter3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->view(), gl->corr.p,
+                           n, s, gl->tileOff.p, tiles, bases);
+        checkLaunch();
+    }
+    gl->corrSize = newTotal;
+    a->nghost = h[0];
+}
+
 void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box, int axis)
 {
     mrmd_ctx* ctx = gl->ctx;
@@ -541,7 +751,7 @@ int mrmd_halo_create(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, i
     KTimer t(gl->ctx, "halo_create");
     resetCorr(gl, a->nlocal);
     a->nghost = 0;
-    for (int d = 0; d < 3; ++d) createAxis(gl, a, box, d);
+    createAll(gl, a, box);
     if (nghost) *nghost = a->nghost;
     MRMD_API_END
 }
