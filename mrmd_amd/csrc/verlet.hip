@@ -239,7 +239,6 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
     }
     __syncthreads();
     using Off = std::conditional_t<WIDE, int64_t, uint32_t>;
-    const Off chunkStride = Off(strideS) * CHUNK;
     int localMax = 0;
     for (int row = threadIdx.x; row < ti.nrows; row += blockDim.x)
     {
@@ -284,17 +283,19 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
             e = e > 0.0 ? e : 0.0;
             wy2[a] = e * e;
         }
-        uint32_t idx = 0;  // element offset of entry c in this row (entryIndex(c, s) - s * CHUNK)
-        uint16_t* const rowBase = nbr + int64_t(s) * CHUNK;
+        // entries are collected eight at a time in four registers (each append shifts the
+        // eight 16-bit places down by one and puts the entry in the top place, so the first
+        // entry ends in the lowest) and stored as one 16-byte chunk: chunk k of slot s is
+        // chunk k * strideS + s of the list
+        uint4* const chunkBase = reinterpret_cast<uint4*>(nbr) + s;
+        uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         int c = 0;
         auto append = [&](uint16_t v) {
-            if (WIDE)
-                rowBase[Off(c >> 3) * chunkStride + (c & 7)] = v;
-            else
-            {
-                rowBase[idx] = v;
-                idx += (c & 7) == 7 ? uint32_t(chunkStride) - 7u : 1u;
-            }
+            b0 = __builtin_amdgcn_alignbit(b1, b0, 16u);
+            b1 = __builtin_amdgcn_alignbit(b2, b1, 16u);
+            b2 = __builtin_amdgcn_alignbit(b3, b2, 16u);
+            b3 = __builtin_amdgcn_alignbit(uint32_t(v), b3, 16u);
+            if ((c & 7) == 7) chunkBase[Off(c >> 3) * Off(strideS)] = make_uint4(b0, b1, b2, b3);
             ++c;
         };
 #pragma unroll
