@@ -63,7 +63,8 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
             if (cells)
             {
                 MD_CALL(mrmd_cell_list_build(cells, a, 0, a->nlocal, delta, box->min, box->max));
-                MD_CALL(mrmd_atoms_permute(a, cells));
+                // the ghosts are made again and every force computed again before either is read
+                permuteAtoms(a, cells, false);
             }
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));

@@ -230,7 +230,8 @@ __global__ void k_copyRecords(AtomsView src, AtomsView dst, int64_t lo, int64_t 
     dst.relmass[i] = src.relmass[i];
 }
 
-// Cabana::permute: new[begin + k] = old[perm[k]] for every field
+// Cabana::permute: new[begin + k] = old[perm[k]] for every field (FORCE: forces too)
+template <bool FORCE>
 __global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* perm, int64_t n, int64_t begin)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -240,7 +241,7 @@ __global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* per
     for (int c = 0; c < 3; ++c)
     {
         dst.vel[c * dst.cap + d] = src.vel[c * src.cap + s];
-        dst.force[c * dst.cap + d] = src.force[c * src.cap + s];
+        if (FORCE) dst.force[c * dst.cap + d] = src.force[c * src.cap + s];
     }
     dst.mass[d] = src.mass[s];
     dst.charge[d] = src.charge[s];
@@ -254,6 +255,55 @@ __global__ void k_copyBack(T* dst, const T* src, int64_t n)
     if (k < n) dst[k] = src[k];
 }
 }  // namespace
+
+namespace mrmd_hip
+{
+// Atoms::permute by a cell list's order; full = false (the time loop, right before the ghosts
+// are made again and the forces computed again) moves neither the forces of the permuted
+// records nor the records outside the list's range, whose values are rewritten before use
+void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full)
+{
+    const int64_t n = cl->bin.end - cl->bin.begin;
+    REQUIRE(cl->bin.end <= a->size, "cell list covers records beyond the atoms");
+    if (n == 0) return;
+    mrmd_ctx* ctx = a->ctx;
+    KTimer t(ctx, "atoms_permute");
+    // gather into the spare buffers (same capacity), then swap them in; the old buffers
+    // become the spares of the next permutation
+    a->sparePos.reserve(ctx, a->cap);
+    a->spareVel.reserve(ctx, 3 * a->cap);
+    a->spareForce.reserve(ctx, 3 * a->cap);
+    a->spareMass.reserve(ctx, a->cap);
+    a->spareCharge.reserve(ctx, a->cap);
+    a->spareRelmass.reserve(ctx, a->cap);
+    if (full)
+        hipLaunchKernelGGL(k_permuteGather<true>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin);
+    else
+        hipLaunchKernelGGL(k_permuteGather<false>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin);
+    // records outside [begin, end) are unchanged: copy them across
+    const int64_t b = cl->bin.begin, e = cl->bin.end, s = a->size;
+    auto copyRange = [&](int64_t lo, int64_t hi)
+    {
+        if (hi <= lo) return;
+        hipLaunchKernelGGL(k_copyRecords, dim3(gridFor(hi - lo)), dim3(BLOCK), 0, ctx->stream, a->view(),
+                           a->spareView(), lo, hi);
+    };
+    if (full)
+    {
+        copyRange(0, b);
+        copyRange(e, s);
+    }
+    checkLaunch();
+    a->pos.swap(a->sparePos);
+    a->vel.swap(a->spareVel);
+    a->force.swap(a->spareForce);
+    a->mass.swap(a->spareMass);
+    a->charge.swap(a->spareCharge);
+    a->relmass.swap(a->spareRelmass);
+}
+}  // namespace mrmd_hip
 
 extern "C" {
 
@@ -319,38 +369,7 @@ int mrmd_atoms_permute(mrmd_atoms* a, const mrmd_cell_list* cl)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && cl, "invalid arguments");
-    const int64_t n = cl->bin.end - cl->bin.begin;
-    REQUIRE(cl->bin.end <= a->size, "cell list covers records beyond the atoms");
-    if (n == 0) return MRMD_OK;
-    mrmd_ctx* ctx = a->ctx;
-    KTimer t(ctx, "atoms_permute");
-    // gather into the spare buffers (same capacity), then swap them in; the old buffers
-    // become the spares of the next permutation
-    a->sparePos.reserve(ctx, a->cap);
-    a->spareVel.reserve(ctx, 3 * a->cap);
-    a->spareForce.reserve(ctx, 3 * a->cap);
-    a->spareMass.reserve(ctx, a->cap);
-    a->spareCharge.reserve(ctx, a->cap);
-    a->spareRelmass.reserve(ctx, a->cap);
-    hipLaunchKernelGGL(k_permuteGather, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(), a->spareView(),
-                       cl->bin.sorted.p, n, cl->bin.begin);
-    // records outside [begin, end) are unchanged: copy them across
-    const int64_t b = cl->bin.begin, e = cl->bin.end, s = a->size;
-    auto copyRange = [&](int64_t lo, int64_t hi)
-    {
-        if (hi <= lo) return;
-        hipLaunchKernelGGL(k_copyRecords, dim3(gridFor(hi - lo)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                           a->spareView(), lo, hi);
-    };
-    copyRange(0, b);
-    copyRange(e, s);
-    checkLaunch();
-    a->pos.swap(a->sparePos);
-    a->vel.swap(a->spareVel);
-    a->force.swap(a->spareForce);
-    a->mass.swap(a->spareMass);
-    a->charge.swap(a->spareCharge);
-    a->relmass.swap(a->spareRelmass);
+    permuteAtoms(a, cl, true);
     MRMD_API_END
 }
 

@@ -75,6 +75,14 @@ struct mrmd_cell_list
 
 namespace mrmd_hip
 {
+// Atoms::permute by the cell list's order (full = false: the time loop's form, which leaves
+// the permuted records' forces and the records outside the list's range unset, as the loop
+// recomputes both before reading them)
+void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full);
+}
+
+namespace mrmd_hip
+{
 constexpr int MAX_RANGES = 25;  // (a + 2R)^2 staged columns: a <= 3 with R = 1, a = 1 with R = 2
 constexpr int MAX_SELF = 9;     // a x a columns of rows
 constexpr int STAGE_MAX = 2048; // staged records per tile (double4 -> 64 KB of LDS)
