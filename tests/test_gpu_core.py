@@ -55,6 +55,41 @@ def test_espp_ghosts_bit_exact(M, ctx, espp_positions):
     np.testing.assert_array_equal(gl.corresponding(n), og.corr(n))
 
 
+@pytest.mark.parametrize("edges,ghost", [((10.0, 7.0, 12.0), 2.0), ((3.0, 8.0, 9.0), 2.0), ((4.0, 3.5, 3.0), 2.2)])
+def test_three_axis_ghosts_equal_axis_by_axis(M, ctx, edges, ghost):
+    """mrmd_halo_create makes the three axes' ghosts in one pass over the real atoms; its
+    records (positions, velocities, masses) and correspondingRealAtom equal those of the
+    axis-by-axis exchange (mrmd_halo_create_axis x, y, z) and the oracle's bit for bit,
+    including boxes thinner than two ghost widths (atoms copied both +L and -L along an
+    axis) and atoms exactly on the inner bounds"""
+    rng = np.random.default_rng(7)
+    L = np.array(edges)
+    n = 600
+    pos = rng.uniform(0.0, 1.0, (n, 3)) * L
+    pos[0, 0], pos[1, 1], pos[2, 2] = ghost, L[1] - ghost, 0.0
+    pos[3] = [L[0] - ghost, ghost, L[2] - ghost]
+    vel = rng.normal(0.0, 1.0, (n, 3))
+    mass = rng.uniform(0.5, 2.0, n)
+    box = M.make_box([0, 0, 0], L, ghost)
+    a = M.Atoms.from_arrays(ctx, pos, vel=vel, mass=mass)
+    b = M.Atoms.from_arrays(ctx, pos, vel=vel, mass=mass)
+    ga, gb = M.GhostLayer(ctx), M.GhostLayer(ctx)
+    na = ga.create_ghost_atoms(a, box)
+    for d in range(3):
+        gb.create_ghost_atoms_axis(b, box, d, reset=(d == 0))
+    assert na == b.nghost > 0
+    tot = n + na
+    for f in (M.POS, M.VEL, M.MASS):
+        np.testing.assert_array_equal(a.download(f, tot), b.download(f, tot))
+    np.testing.assert_array_equal(ga.corresponding(tot), gb.corresponding(tot))
+    c = O.Atoms.from_arrays(pos)
+    og = O.Ghost()
+    og.create_xyz(c, O.make_box([0, 0, 0], L, ghost))
+    assert c.nlocal + c.nghost == tot
+    np.testing.assert_array_equal(a.download(M.POS, tot), c.get(O.POS, tot))
+    np.testing.assert_array_equal(ga.corresponding(tot), og.corr(tot))
+
+
 def test_espp_verlet_pairs_sets_energy_forces(M, ctx, espp_positions):
     gbox, cbox, g, c = _espp(M, ctx, espp_positions)
     gl = M.GhostLayer(ctx)
