@@ -190,7 +190,9 @@ __global__ void k_ghostScatter(AtomsView a, int32_t* corr, int64_t total, int ax
 // index order, in the order ghostBlockOrder gives: x: +, -; y: + of (real, x+, x-), then
 // - of the same; z: + of (real, x+, x-, y+ of the three, y- of the three), then - of the
 // same.  Positions take each axis's shift once, as the axis-by-axis copies do.
-constexpr int TILE3 = BLOCK * 4;
+// records per tile of the three-axis passes: rounds x BLOCK, up to four rounds as long as the
+// grid keeps ~2,048 workgroups
+inline int ghostRounds(int64_t n) { return int(std::min<int64_t>(4, std::max<int64_t>(1, n / (int64_t(BLOCK) * 2048)))); }
 constexpr int GHOST_BLOCKS = 26;
 
 __host__ __device__ inline int ghostBlockOrder(int b)
@@ -229,31 +231,26 @@ __device__ inline unsigned waveCodes(unsigned m)
     return m;
 }
 
-// per tile of TILE3 atoms: atoms of each code, cnt[(b - 1) * tiles + tile].  Each lane
-// loads its ROUNDS positions first; a wave then counts only the codes present in it.
-constexpr int ROUNDS = TILE3 / BLOCK;
+// per tile of `rounds` x BLOCK atoms: atoms of each code, cnt[(b - 1) * tiles + tile]; a
+// wave counts only the codes present in it
 __global__ __launch_bounds__(BLOCK) void k_ghostCount3(const double4* pos, int64_t n, SubD s, int32_t* cnt,
-                                                       int64_t tiles)
+                                                       int64_t tiles, int rounds)
 {
     __shared__ int sc[BLOCK / WAVE][GHOST_BLOCKS];
-    const int64_t base = int64_t(blockIdx.x) * TILE3;
+    const int64_t base = int64_t(blockIdx.x) * rounds * BLOCK;
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
-    unsigned codes[ROUNDS], all = 0;
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r)
+    int acc = 0;  // lane b - 1: this wave's atoms of code b
+    for (int r = 0; r < rounds; ++r)
     {
         const int64_t e = base + r * BLOCK + threadIdx.x;
-        codes[r] = e < n ? ghostCodes(ghostFlags(pos[e], s)) : 0u;
-        all |= codes[r];
-    }
-    int acc = 0;  // lane b - 1: this wave's atoms of code b
-    for (unsigned present = __ballot(all != 0) ? waveCodes(all) : 0u; present; present &= present - 1)
-    {
-        const int b = __builtin_ctz(present);
-        int c = 0;
-#pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) c += __popcll(__ballot((codes[r] >> b) & 1u));
-        if (lane == b - 1) acc = c;
+        const unsigned codes = e < n ? ghostCodes(ghostFlags(pos[e], s)) : 0u;
+        if (__ballot(codes != 0) == 0) continue;
+        for (unsigned present = waveCodes(codes); present; present &= present - 1)
+        {
+            const int b = __builtin_ctz(present);
+            const int c = __popcll(__ballot((codes >> b) & 1u));
+            if (lane == b - 1) acc += c;
+        }
     }
     if (lane < GHOST_BLOCKS) sc[w][lane] = acc;
     __syncthreads();
@@ -284,16 +281,18 @@ __global__ void k_ghostBases(const int32_t* tot, int32_t* bases, int32_t* count)
 
 // every real atom writes its ghosts: ghost index n + bases[b] + the tile's offset + its rank
 // in the tile among the atoms of code b (records of the tile in index order: round by round,
-// wave by wave, lane by lane).  Waves without a flagged atom only take part in the counts.
+// wave by wave, lane by lane).  A flagged atom's record is read once, before its codes'
+// copies; waves without a flagged atom only take part in the counts.
 __global__ __launch_bounds__(BLOCK) void k_ghostScatter3(AtomsView a, int32_t* corr, int64_t n, SubD s,
-                                                         const int32_t* off, int64_t tiles, const int32_t* bases)
+                                                         const int32_t* off, int64_t tiles, const int32_t* bases,
+                                                         int rounds)
 {
     __shared__ int wc[BLOCK / WAVE][GHOST_BLOCKS];
-    const int64_t base = int64_t(blockIdx.x) * TILE3;
+    const int64_t base = int64_t(blockIdx.x) * rounds * BLOCK;
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
     // lane b - 1: ghosts of code b of this tile written before this round
     int run = lane < GHOST_BLOCKS ? bases[lane + 1] + off[int64_t(lane) * tiles + blockIdx.x] : 0;
-    for (int r = 0; r < ROUNDS; ++r)
+    for (int r = 0; r < rounds; ++r)
     {
         const int64_t e = base + r * BLOCK + threadIdx.x;
         double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -302,6 +301,18 @@ __global__ __launch_bounds__(BLOCK) void k_ghostScatter3(AtomsView a, int32_t* c
         {
             p = a.pos[e];
             codes = ghostCodes(ghostFlags(p, s));
+        }
+        double v[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0}, mass = 0.0, charge = 0.0, relmass = 0.0;
+        if (codes)
+        {
+            for (int d = 0; d < 3; ++d)
+            {
+                v[d] = a.vel[d * a.cap + e];
+                f[d] = a.force[d * a.cap + e];
+            }
+            mass = a.mass[e];
+            charge = a.charge[e];
+            relmass = a.relmass[e];
         }
         const unsigned present = __ballot(codes != 0) ? waveCodes(codes) : 0u;
         int mine = 0;
@@ -339,12 +350,12 @@ __global__ __launch_bounds__(BLOCK) void k_ghostScatter3(AtomsView a, int32_t* c
                 a.pos[g] = q;
                 for (int d = 0; d < 3; ++d)
                 {
-                    a.vel[d * a.cap + g] = a.vel[d * a.cap + e];
-                    a.force[d * a.cap + g] = a.force[d * a.cap + e];
+                    a.vel[d * a.cap + g] = v[d];
+                    a.force[d * a.cap + g] = f[d];
                 }
-                a.mass[g] = a.mass[e];
-                a.charge[g] = a.charge[e];
-                a.relmass[g] = a.relmass[e];
+                a.mass[g] = mass;
+                a.charge[g] = charge;
+                a.relmass[g] = relmass;
                 corr[g] = int32_t(e);
             }
         }
@@ -561,6 +572,242 @@ __global__ void k_molGhostScatter(MolView m, AtomsView a, int32_t* corr, int64_t
         for (int k = 0; k < 4; ++k) run[k] += t[k];
     }
 }
+// The multi-resolution exchange (MultiResPeriodicGhostExchange.cpp:140-240) of the three
+// axes in one pass over the real molecules, as k_ghostCount3 / k_ghostScatter3 do for
+// atoms: per axis the reference copies the molecules with centre >= maxInner by -L (its
+// first block) and those < minInner by +L, each with its atoms in a run after the atoms made
+// so far; a ghost molecule's later tests are its real molecule's, so the output is the 26
+// code blocks in ghostBlockOrder (code s = 1 here: >= maxInner, -L; 2: < minInner, +L),
+// molecules and their atoms block by block, both in real-molecule order.
+__device__ inline unsigned molGhostFlags(const double4& p, const SubD& s)
+{
+    return unsigned(p.x >= s.maxInner[0]) | (unsigned(p.x < s.minInner[0]) << 1) |
+           (unsigned(p.y >= s.maxInner[1]) << 2) | (unsigned(p.y < s.minInner[1]) << 3) |
+           (unsigned(p.z >= s.maxInner[2]) << 4) | (unsigned(p.z < s.minInner[2]) << 5);
+}
+
+// per tile of `rounds` x BLOCK molecules: molecules of each code, cnt[(b - 1) * tiles + tile], and their
+// atoms, cnt[(26 + b - 1) * tiles + tile]
+__global__ __launch_bounds__(BLOCK) void k_molGhostCount3(MolView m, int64_t n, SubD s, int32_t* cnt,
+                                                          int64_t tiles, int rounds)
+{
+    __shared__ int sc[BLOCK / WAVE][2 * GHOST_BLOCKS];
+    const int64_t base = int64_t(blockIdx.x) * rounds * BLOCK;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    int accM = 0, accA = 0;  // lane b - 1: this wave's molecules / atoms of code b
+    for (int r = 0; r < rounds; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        unsigned codes = 0;
+        int na = 0;
+        if (e < n)
+        {
+            codes = ghostCodes(molGhostFlags(m.pos[e], s));
+            na = int(m.numAtoms[e]);
+        }
+        if (__ballot(codes != 0) == 0) continue;
+        for (unsigned present = waveCodes(codes); present; present &= present - 1)
+        {
+            const int b = __builtin_ctz(present);
+            const bool in = (codes >> b) & 1u;
+            const int cm = __popcll(__ballot(in));
+            int ca = in ? na : 0;
+            for (int off = WAVE / 2; off > 0; off >>= 1) ca += __shfl_xor(ca, off, WAVE);
+            if (lane == b - 1)
+            {
+                accM += cm;
+                accA += ca;
+            }
+        }
+    }
+    if (lane < GHOST_BLOCKS)
+    {
+        sc[w][lane] = accM;
+        sc[w][GHOST_BLOCKS + lane] = accA;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * GHOST_BLOCKS)
+    {
+        int t = 0;
+        for (int q = 0; q < BLOCK / WAVE; ++q) t += sc[q][threadIdx.x];
+        cnt[int64_t(threadIdx.x) * tiles + blockIdx.x] = t;
+    }
+}
+
+// molecule and atom block bases in output order (tot[b - 1] molecules, tot[26 + b - 1] atoms);
+// bases[b] and bases[27 + b] by code, count[0] molecules, count[1] atoms
+__global__ void k_molGhostBases(const int32_t* tot, int32_t* bases, int32_t* count)
+{
+    if (threadIdx.x >= 2) return;
+    const int k = threadIdx.x;  // 0: molecules, 1: atoms
+    int byOrder[GHOST_BLOCKS];
+    for (int b = 1; b <= GHOST_BLOCKS; ++b) byOrder[ghostBlockOrder(b)] = tot[k * GHOST_BLOCKS + b - 1];
+    int start[GHOST_BLOCKS], acc = 0;
+    for (int o = 0; o < GHOST_BLOCKS; ++o)
+    {
+        start[o] = acc;
+        acc += byOrder[o];
+    }
+    bases[k * 27] = 0;
+    for (int b = 1; b <= GHOST_BLOCKS; ++b) bases[k * 27 + b] = start[ghostBlockOrder(b)];
+    count[k] = acc;
+}
+
+// every real molecule writes its ghost molecules and their atoms (molecule record and its
+// first atom read once, before its codes' copies; further atoms of a molecule per copy)
+__global__ __launch_bounds__(BLOCK) void k_molGhostScatter3(MolView m, AtomsView a, int32_t* corr, int64_t nM,
+                                                            int64_t nA, SubD s, const int32_t* off, int64_t tiles,
+                                                            const int32_t* bases, int rounds)
+{
+    __shared__ int wc[BLOCK / WAVE][2 * GHOST_BLOCKS];
+    const int64_t base = int64_t(blockIdx.x) * rounds * BLOCK;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    // lane b - 1: molecules / atoms of code b of this tile written before this round
+    int runM = 0, runA = 0;
+    if (lane < GHOST_BLOCKS)
+    {
+        runM = bases[lane + 1] + off[int64_t(lane) * tiles + blockIdx.x];
+        runA = bases[27 + lane + 1] + off[int64_t(GHOST_BLOCKS + lane) * tiles + blockIdx.x];
+    }
+    for (int r = 0; r < rounds; ++r)
+    {
+        const int64_t e = base + r * BLOCK + threadIdx.x;
+        double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+        unsigned codes = 0;
+        int na = 0;
+        if (e < nM)
+        {
+            p = m.pos[e];
+            codes = ghostCodes(molGhostFlags(p, s));
+            na = int(m.numAtoms[e]);
+        }
+        // the molecule's record and its first atom's, for all its copies
+        double mf[3] = {0.0, 0.0, 0.0}, mg[3] = {0.0, 0.0, 0.0}, lam = 0.0, mod = 0.0;
+        int64_t src = 0;
+        double4 ap = make_double4(0.0, 0.0, 0.0, 0.0);
+        double av[3] = {0.0, 0.0, 0.0}, af[3] = {0.0, 0.0, 0.0}, amass = 0.0, acharge = 0.0, arel = 0.0;
+        if (codes)
+        {
+            for (int d = 0; d < 3; ++d)
+            {
+                mf[d] = m.force[d * m.cap + e];
+                mg[d] = m.grad[d * m.cap + e];
+            }
+            lam = m.lambda[e];
+            mod = m.modLambda[e];
+            src = m.offset[e];
+            if (na > 0)
+            {
+                ap = a.pos[src];
+                for (int d = 0; d < 3; ++d)
+                {
+                    av[d] = a.vel[d * a.cap + src];
+                    af[d] = a.force[d * a.cap + src];
+                }
+                amass = a.mass[src];
+                acharge = a.charge[src];
+                arel = a.relmass[src];
+            }
+        }
+        const unsigned present = __ballot(codes != 0) ? waveCodes(codes) : 0u;
+        int mineM = 0, mineA = 0;
+        for (unsigned pr = present; pr; pr &= pr - 1)
+        {
+            const int b = __builtin_ctz(pr);
+            const bool in = (codes >> b) & 1u;
+            const int cm = __popcll(__ballot(in));
+            int ca = in ? na : 0;
+            for (int o = WAVE / 2; o > 0; o >>= 1) ca += __shfl_xor(ca, o, WAVE);
+            if (lane == b - 1)
+            {
+                mineM = cm;
+                mineA = ca;
+            }
+        }
+        if (lane < GHOST_BLOCKS)
+        {
+            wc[w][lane] = mineM;
+            wc[w][GHOST_BLOCKS + lane] = mineA;
+        }
+        __syncthreads();
+        int beforeM = 0, beforeA = 0, totalM = 0, totalA = 0;
+        if (lane < GHOST_BLOCKS)
+            for (int q = 0; q < BLOCK / WAVE; ++q)
+            {
+                const int cm = wc[q][lane], ca = wc[q][GHOST_BLOCKS + lane];
+                beforeM += q < w ? cm : 0;
+                beforeA += q < w ? ca : 0;
+                totalM += cm;
+                totalA += ca;
+            }
+        __syncthreads();
+        for (unsigned pr = present; pr; pr &= pr - 1)
+        {
+            const int b = __builtin_ctz(pr);
+            const bool in = (codes >> b) & 1u;
+            const unsigned long long mask = __ballot(in);
+            const int stM = __shfl(runM + beforeM, b - 1, WAVE), stA = __shfl(runA + beforeA, b - 1, WAVE);
+            const int inc = waveInclusiveScan(in ? na : 0);
+            if (in)
+            {
+                const int64_t dM = nM + stM + waveRank(mask);
+                const int64_t dA = nA + stA + (inc - na);
+                const int sx = b % 3, sy = (b / 3) % 3, sz = b / 9;
+                // code 1: copied by -L, code 2: by +L
+                const double ox = sx ? (sx == 1 ? -s.diam[0] : s.diam[0]) : 0.0;
+                const double oy = sy ? (sy == 1 ? -s.diam[1] : s.diam[1]) : 0.0;
+                const double oz = sz ? (sz == 1 ? -s.diam[2] : s.diam[2]) : 0.0;
+                double4 q = p;
+                if (sx) q.x += ox;
+                if (sy) q.y += oy;
+                if (sz) q.z += oz;
+                m.pos[dM] = q;
+                for (int d = 0; d < 3; ++d)
+                {
+                    m.force[d * m.cap + dM] = mf[d];
+                    m.grad[d * m.cap + dM] = mg[d];
+                }
+                m.lambda[dM] = lam;
+                m.modLambda[dM] = mod;
+                m.offset[dM] = dA;
+                m.numAtoms[dM] = na;
+                for (int k = 0; k < na; ++k)
+                {
+                    double4 t = ap;
+                    double tv[3] = {av[0], av[1], av[2]}, tf[3] = {af[0], af[1], af[2]};
+                    double tm = amass, tc = acharge, tr = arel;
+                    if (k > 0)
+                    {
+                        t = a.pos[src + k];
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            tv[d] = a.vel[d * a.cap + src + k];
+                            tf[d] = a.force[d * a.cap + src + k];
+                        }
+                        tm = a.mass[src + k];
+                        tc = a.charge[src + k];
+                        tr = a.relmass[src + k];
+                    }
+                    if (sx) t.x += ox;
+                    if (sy) t.y += oy;
+                    if (sz) t.z += oz;
+                    a.pos[dA + k] = t;
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        a.vel[d * a.cap + dA + k] = tv[d];
+                        a.force[d * a.cap + dA + k] = tf[d];
+                    }
+                    a.mass[dA + k] = tm;
+                    a.charge[dA + k] = tc;
+                    a.relmass[dA + k] = tr;
+                    corr[dA + k] = int32_t(src + k);
+                }
+            }
+        }
+        runM += totalM;
+        runA += totalA;
+    }
+}
 }  // namespace
 
 // =========================================================================================
@@ -617,7 +864,8 @@ void createAll(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
     const SubD s = makeSub(box);
     const int64_t n = a->nlocal;
     if (n >= (int64_t(1) << 31) - 1) fail(MRMD_ERR_UNSUP, "more than 2^31 records");
-    const int64_t tiles = std::max<int64_t>(1, (n + TILE3 - 1) / TILE3);
+    const int rounds = ghostRounds(n);
+    const int64_t tiles = std::max<int64_t>(1, (n + rounds * BLOCK - 1) / (rounds * BLOCK));
     gl->tileCnt.ensure(ctx, GHOST_BLOCKS * tiles);
     gl->tileOff.ensure(ctx, GHOST_BLOCKS * tiles);
     gl->blocks.ensure(ctx, 64);
@@ -625,7 +873,7 @@ void createAll(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
     int32_t* bases = tot + 32;
     int32_t* count = bases + 27;
     hipLaunchKernelGGL(k_ghostCount3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->pos.p, n, s,
-                       gl->tileCnt.p, tiles);
+                       gl->tileCnt.p, tiles, rounds);
     checkLaunch();
     scanExclusiveI32Multi(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, GHOST_BLOCKS, tot);
     hipLaunchKernelGGL(k_ghostBases, dim3(1), dim3(64), 0, ctx->stream, tot, bases, count);
@@ -639,7 +887,7 @@ void createAll(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
     if (h[0] > 0)
     {
         hipLaunchKernelGGL(k_ghostScatter3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, a->view(), gl->corr.p,
-                           n, s, gl->tileOff.p, tiles, bases);
+                           n, s, gl->tileOff.p, tiles, bases, rounds);
         checkLaunch();
     }
     gl->corrSize = newTotal;
@@ -679,6 +927,47 @@ void molCreateAxis(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const
     m->nghost += h[0] + h[2];
     ++m->layoutVersion;
     a->nghost += h[1] + h[3];
+}
+
+// the three axes in one pass over the real molecules (no ghosts yet): the same ghost
+// molecules and atoms in the same order as molCreateAxis for x, y and z
+void molCreateAll(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atoms* a, const mrmd_box* box)
+{
+    mrmd_ctx* ctx = gl->ctx;
+    const SubD s = makeSub(box);
+    const int64_t nM = m->nlocal, nA = a->nlocal;
+    const int rounds = ghostRounds(nM);
+    const int64_t tiles = std::max<int64_t>(1, (nM + rounds * BLOCK - 1) / (rounds * BLOCK));
+    gl->tileCnt.ensure(ctx, 2 * GHOST_BLOCKS * tiles);
+    gl->tileOff.ensure(ctx, 2 * GHOST_BLOCKS * tiles);
+    gl->blocks.ensure(ctx, 128);
+    int32_t* tot = gl->blocks.p;
+    int32_t* bases = tot + 56;
+    int32_t* count = bases + 56;
+    hipLaunchKernelGGL(k_molGhostCount3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(), nM, s,
+                       gl->tileCnt.p, tiles, rounds);
+    checkLaunch();
+    scanExclusiveI32Multi(ctx, gl->tileCnt.p, gl->tileOff.p, tiles, 2 * GHOST_BLOCKS, tot);
+    hipLaunchKernelGGL(k_molGhostBases, dim3(1), dim3(64), 0, ctx->stream, tot, bases, count);
+    checkLaunch();
+    int32_t h[2];
+    readbackI32(ctx, count, 2, h);
+    const int64_t newM = nM + h[0], newA = nA + h[1];
+    m->ensureCap(newM);
+    m->size = newM;
+    a->ensureCap(newA);
+    a->size = newA;
+    if (gl->corr.n < newA) gl->corr.reserve(ctx, growCap(newA), gl->corrSize);
+    if (h[0] > 0)
+    {
+        hipLaunchKernelGGL(k_molGhostScatter3, dim3(unsigned(tiles)), dim3(BLOCK), 0, ctx->stream, m->view(),
+                           a->view(), gl->corr.p, nM, nA, s, gl->tileOff.p, tiles, bases, rounds);
+        checkLaunch();
+    }
+    gl->corrSize = newA;
+    m->nghost = h[0];
+    ++m->layoutVersion;
+    a->nghost = h[1];
 }
 
 void checkBox(const mrmd_box* box)
@@ -846,7 +1135,7 @@ int mrmd_multires_halo_create(mrmd_ghost_layer* gl, mrmd_molecules* m, mrmd_atom
     ++m->layoutVersion;
     resetCorr(gl, a->nlocal);
     a->nghost = 0;
-    for (int d = 0; d < 3; ++d) molCreateAxis(gl, m, a, box, d);
+    molCreateAll(gl, m, a, box);
     MRMD_API_END
 }
 

@@ -116,6 +116,64 @@ def test_multires_ghosts_bit_exact(M, ctx):
     np.testing.assert_array_equal(gl.corresponding(na), og.corr(na))
 
 
+@pytest.mark.parametrize("edges,ghost", [((9.0, 6.0, 11.0), 1.8), ((3.0, 7.0, 8.0), 1.8)])
+def test_multires_three_axis_ghosts_equal_axis_by_axis(M, ctx, edges, ghost):
+    """mrmd_multires_halo_create makes the three axes' ghost molecules (with their atoms) in
+    one pass over the real molecules; molecules (positions, lambdas, atom offsets and counts),
+    atoms (positions, velocities, masses) and correspondingRealAtom equal the axis-by-axis
+    exchange's and the oracle's bit for bit: molecules of one to three atoms, a box thinner
+    than two ghost widths along x, molecules exactly on the inner bounds"""
+    rng = np.random.default_rng(5)
+    L = np.array(edges)
+    nm = 300
+    mp = rng.uniform(0.0, 1.0, (nm, 3)) * L
+    mp[0, 0], mp[1, 1], mp[2, 2] = ghost, L[1] - ghost, 0.0
+    napm = rng.integers(1, 4, nm)
+    off = np.concatenate([[0], np.cumsum(napm)[:-1]])
+    ap = np.concatenate([mp[k] + rng.normal(0.0, 0.2, (napm[k], 3)) for k in range(nm)])
+    vel = rng.normal(0.0, 1.0, ap.shape)
+    mass = rng.uniform(0.5, 2.0, ap.shape[0])
+    lam = rng.uniform(0.0, 1.0, nm)
+    box = M.make_box([0, 0, 0], L, ghost)
+
+    def make():
+        gm = M.Molecules(ctx, nm)
+        gm.upload(M.M_POS, mp)
+        gm.upload(M.M_OFFSET, off)
+        gm.upload(M.M_NUMATOMS, napm)
+        gm.upload(M.M_LAMBDA, lam)
+        gm.set_counts(nm, 0)
+        return gm, M.Atoms.from_arrays(ctx, ap, vel=vel, mass=mass)
+
+    ma, aa = make()
+    mb, ab = make()
+    ga, gb = M.GhostLayer(ctx), M.GhostLayer(ctx)
+    ga.multires_create_ghost_atoms(ma, aa, box)
+    for d in range(3):
+        gb.multires_create_ghost_atoms_axis(mb, ab, box, d, reset=(d == 0))
+    assert ma.nghost == mb.nghost > 0 and aa.nghost == ab.nghost
+    tm, ta = nm + ma.nghost, ap.shape[0] + aa.nghost
+    for f in (M.M_POS, M.M_LAMBDA, M.M_OFFSET, M.M_NUMATOMS):
+        np.testing.assert_array_equal(ma.download(f, tm), mb.download(f, tm))
+    for f in (M.POS, M.VEL, M.MASS):
+        np.testing.assert_array_equal(aa.download(f, ta), ab.download(f, ta))
+    np.testing.assert_array_equal(ga.corresponding(ta), gb.corresponding(ta))
+    om = O.Molecules(nm)
+    om.set(O.M_POS, mp)
+    om.set(O.M_OFFSET, off)
+    om.set(O.M_NUMATOMS, napm)
+    om.set_counts(nm, 0)
+    oa = O.Atoms.from_arrays(ap)
+    og = O.Ghost()
+    obox = O.make_box([0, 0, 0], L, ghost)
+    O.lib().or_multires_ghost_xyz(og.h, om.h, oa.h, O.C.byref(obox))
+    assert om.nghost == ma.nghost and oa.nghost == aa.nghost
+    np.testing.assert_array_equal(ma.download(M.M_POS, tm), om.get(O.M_POS, tm))
+    np.testing.assert_array_equal(ma.download(M.M_OFFSET, tm), om.get(O.M_OFFSET, tm))
+    np.testing.assert_array_equal(aa.download(M.POS, ta), oa.get(O.POS, ta))
+    np.testing.assert_array_equal(ga.corresponding(ta), og.corr(ta))
+
+
 def _slab_system(M, ctx, n=6000, Lx=40.0, Lyz=12.0, seed=11):
     pos = uniform(n, [0, 0, 0], [Lx, Lyz, Lyz], seed)
     rng = np.random.default_rng(seed)
