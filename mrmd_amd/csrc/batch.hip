@@ -104,6 +104,10 @@ struct BS
     int32_t* offsets;
     int32_t* fill;
     int32_t* sorted;
+    int32_t* id;       ///< the atom's index in the ensemble as given (atoms move to slot order)
+    int32_t* tmpA;     ///< permutation scratch (given indices, cells)
+    int32_t* tmpB;
+    double* tmpVel;
     double4* sortedPos;
     double* slotLam;
     double* slotGrad;
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
         p.z += dtHalf * vz;
     }
     double xi[3];
-    waveNormals3(valid ? R->seed : 0, valid ? uint64_t(i - R->a0) : 0, uint64_t(*b.step), valid,
+    waveNormals3(valid ? R->seed : 0, valid ? uint64_t(b.id[i] - R->a0) : 0, uint64_t(*b.step), valid,
                  draws[threadIdx.x / WAVE], xi);
     if (valid)
     {
@@ -286,18 +290,26 @@ __global__ void k_bZeroCells(BS b)
     b.fill[c] = 0;
 }
 
+// atoms sit in cell order after their replica's last rebuild, so a wave's atoms share cells
+// over long runs: one atomic per run (laneRun; valid lanes are a prefix of the wave)
 __global__ void k_bCount(BS b)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= b.natoms) return;
-    const int r = b.atomRep[i];
-    if (!b.flag[r]) return;
-    const RepD& R = b.reps[r];
-    const double4 p = b.pos[i];
-    const int c = cellIndex(R, locate(p.x, R.rdxc[0], R.nc[0]), locate(p.y, R.rdxc[1], R.nc[1]),
-                            locate(p.z, R.rdxc[2], R.nc[2]));
-    b.cellOf[i] = c;
-    atomicAdd(&b.counts[c], 1);
+    int c = -1;
+    if (i < b.natoms)
+    {
+        const int r = b.atomRep[i];
+        if (b.flag[r])
+        {
+            const RepD& R = b.reps[r];
+            const double4 p = b.pos[i];
+            c = cellIndex(R, locate(p.x, R.rdxc[0], R.nc[0]), locate(p.y, R.rdxc[1], R.nc[1]),
+                          locate(p.z, R.rdxc[2], R.nc[2]));
+            b.cellOf[i] = c;
+        }
+    }
+    const Run run = laneRun(c, i < b.natoms);
+    if (c >= 0 && run.head == int(threadIdx.x) % WAVE) atomicAdd(&b.counts[c], run.len);
 }
 
 // one workgroup (BLOCK threads) per replica: offsets = a0 + exclusive scan of its counts
@@ -323,20 +335,52 @@ __global__ __launch_bounds__(BLOCK) void k_bScan(BS b)
 __global__ void k_bFill(BS b)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= b.natoms) return;
-    const int r = b.atomRep[i];
-    if (!b.flag[r]) return;
-    const int c = b.cellOf[i];
-    b.sorted[b.offsets[c] + atomicAdd(&b.fill[c], 1)] = int32_t(i);
+    const bool valid = i < b.natoms;
+    const int c = valid && b.flag[b.atomRep[i]] ? b.cellOf[i] : -1;
+    const Run run = laneRun(c, valid);
+    const int lane = int(threadIdx.x) % WAVE;
+    int start = 0;
+    if (c >= 0 && run.head == lane) start = atomicAdd(&b.fill[c], run.len);
+    start = __shfl(start, run.head, WAVE);
+    if (c >= 0) b.sorted[b.offsets[c] + start + (lane - run.head)] = int32_t(i);
 }
 
-// ascending z inside each cell of a flagged replica, the atom index breaking ties (the fill
-// order of the atomics is arbitrary): one group of 32 lanes per cell (sortCellByZ)
+// ascending z inside each cell of a flagged replica, the atom's given index breaking ties
+// (the fill order of the atomics is arbitrary, and atoms move): one group of 32 lanes per
+// cell (sortCellByZ)
 __global__ __launch_bounds__(BLOCK) void k_bSortCells(BS b)
 {
     const int64_t c = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SORT_GROUP;
     if (c >= b.ncells || !b.flag[b.cellRep[c]]) return;
-    sortCellByZ(b.sorted + b.offsets[c], b.counts[c], b.pos, int(threadIdx.x % SORT_GROUP));
+    sortCellByZ<true>(b.sorted + b.offsets[c], b.counts[c], b.pos, int(threadIdx.x % SORT_GROUP), b.id);
+}
+
+// A flagged replica's atoms move to its new slot order (position, velocity, given index and
+// cell; forces are written by the step's force pass before they are read), so that the
+// per-step slot gather, the force stores and the next binning run in atom order.  Pass 1
+// gathers into scratch (the slot positions serve for the positions), pass 2 copies back and
+// makes the slot map the identity.  Results do not depend on where an atom sits: its noise
+// stream follows its given index, and cell-order ties break by it.
+__global__ void k_bPermute1(BS b)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= b.natoms || !b.flag[b.atomRep[s]]) return;
+    const int32_t i = b.sorted[s];
+    b.sortedPos[s] = b.pos[i];
+    for (int d = 0; d < 3; ++d) b.tmpVel[d * b.cap + s] = b.vel[d * b.cap + i];
+    b.tmpA[s] = b.id[i];
+    b.tmpB[s] = b.cellOf[i];
+}
+
+__global__ void k_bPermute2(BS b)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= b.natoms || !b.flag[b.atomRep[s]]) return;
+    b.pos[s] = b.sortedPos[s];
+    for (int d = 0; d < 3; ++d) b.vel[d * b.cap + s] = b.tmpVel[d * b.cap + s];
+    b.id[s] = b.tmpA[s];
+    b.cellOf[s] = b.tmpB[s];
+    b.sorted[s] = int32_t(s);
 }
 
 // slot ranges, image shifts and LDS bases of every tile of a flagged replica: the staged
@@ -995,6 +1039,12 @@ __global__ void k_bFillD(double* p, int64_t n, double v)
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
 }
+
+__global__ void k_bIota(int32_t* p, int64_t n)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = int32_t(k);
+}
 }  // namespace
 
 struct mrmd_batch
@@ -1018,6 +1068,8 @@ struct mrmd_batch
     DBuf<double4> pos, sortedPos;
     DBuf<double> vel, force, slotLam, slotGrad;
     DBuf<int32_t> atomRep, cellRep, cellOf, counts, offsets, fill, sorted, cnt, flag, err;
+    DBuf<int32_t> id, tmpA, tmpB;
+    DBuf<double> tmpVel;
     DBuf<TileG> dtg;
     DBuf<TileB> tiles;
     DBuf<uint16_t> nbr;
@@ -1062,6 +1114,10 @@ BS mrmd_batch::state()
     b.offsets = offsets.p;
     b.fill = fill.p;
     b.sorted = sorted.p;
+    b.id = id.p;
+    b.tmpA = tmpA.p;
+    b.tmpB = tmpB.p;
+    b.tmpVel = tmpVel.p;
     b.sortedPos = sortedPos.p;
     b.slotLam = slotLam.p;
     b.slotGrad = slotGrad.p;
@@ -1172,6 +1228,8 @@ void mrmd_batch::enqueueSteps(int64_t n)
         hipLaunchKernelGGL(k_bScan, dim3(nr), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bFill, dim3(ga), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bPermute1, dim3(ga), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bPermute2, dim3(ga), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,
@@ -1211,7 +1269,7 @@ void mrmd_batch::enqueueSteps(int64_t n)
 namespace
 {
 constexpr int64_t STATE_MAGIC = 0x3154415453444d4dLL;  // bytes "MMDSTAT1"
-constexpr int64_t STATE_VERSION = 1;
+constexpr int64_t STATE_VERSION = 2;
 constexpr int STATE_HEADER = 8;
 
 struct StatePart
@@ -1227,6 +1285,7 @@ std::vector<StatePart> stateParts(mrmd_batch* b)
     p.push_back({b->pos.p, na * sizeof(double4)});
     for (int d = 0; d < 3; ++d) p.push_back({b->vel.p + d * b->cap, na * 8});
     for (int d = 0; d < 3; ++d) p.push_back({b->force.p + d * b->cap, na * 8});
+    p.push_back({b->id.p, na * 4});
     p.push_back({b->step.p, 8});
     p.push_back({b->rebuilds.p, nr * 8});
     p.push_back({b->tfSamples.p, nr * 8});
@@ -1243,6 +1302,17 @@ int64_t stateBytes(mrmd_batch* b)
     int64_t n = STATE_HEADER * 8;
     for (const StatePart& p : stateParts(b)) n += int64_t(p.bytes);
     return n;
+}
+
+// where each atom of the given order sits now
+std::vector<int32_t> placeOf(mrmd_batch* b)
+{
+    const int64_t n = b->natoms;
+    std::vector<int32_t> id(static_cast<size_t>(n)), place(static_cast<size_t>(n));
+    HIPCHECK(hipMemcpyAsync(id.data(), b->id.p, size_t(n) * 4, hipMemcpyDeviceToHost, b->ctx->stream));
+    sync(b->ctx);
+    for (int64_t i = 0; i < n; ++i) place[size_t(id[size_t(i)])] = int32_t(i);
+    return place;
 }
 
 void endListEpoch(mrmd_batch* b)
@@ -1354,6 +1424,10 @@ int mrmd_batch_create(mrmd_ctx* c, const mrmd_replica* reps, int64_t nrep, doubl
         b->atomRep.reserve(x, cp);
         b->cellOf.reserve(x, cp);
         b->sorted.reserve(x, cp);
+        b->id.reserve(x, cp);
+        b->tmpA.reserve(x, cp);
+        b->tmpB.reserve(x, cp);
+        b->tmpVel.reserve(x, 3 * cp);
         b->cnt.reserve(x, cp);
         b->cellRep.reserve(x, b->ncells);
         b->counts.reserve(x, b->ncells);
@@ -1398,6 +1472,8 @@ int mrmd_batch_create(mrmd_ctx* c, const mrmd_replica* reps, int64_t nrep, doubl
         HIPCHECK(hipMemsetAsync(b->compCount.p, 0, size_t(nr) * COMP_BINS * 8, x->stream));
         HIPCHECK(hipMemsetAsync(b->meanComp.p, 0, size_t(nr) * COMP_BINS * 8, x->stream));
         HIPCHECK(hipMemsetAsync(b->cnt.p, 0, size_t(cp) * 4, x->stream));
+        hipLaunchKernelGGL(k_bIota, dim3(gridFor(na)), dim3(BLOCK), 0, x->stream, b->id.p, na);
+        checkLaunch();
         // default table: epsilon = sigma = 1, shifted, uncapped
         const double one = 1.0, zero = 0.0;
         std::vector<Pre> t;
@@ -1464,16 +1540,20 @@ int mrmd_batch_get_tiling(const mrmd_batch* b, int* target, int* block, int32_t*
     return MRMD_OK;
 }
 
+// host arrays are in the order the atoms were given; on the device an atom sits in its
+// replica's slot order of the last rebuild (place)
 int mrmd_batch_upload(mrmd_batch* b, int field, const double* host)
 {
     MRMD_API_BEGIN(b ? b->ctx : nullptr)
     REQUIRE(b && host && (field == MRMD_POS || field == MRMD_VEL), "upload takes POS or VEL");
     const int64_t n = b->natoms;
     mrmd_ctx* c = b->ctx;
+    const std::vector<int32_t> place = placeOf(b);
     if (field == MRMD_POS)
     {
         std::vector<double4> h(static_cast<size_t>(n));
-        for (int64_t i = 0; i < n; ++i) h[size_t(i)] = make_double4(host[3 * i], host[3 * i + 1], host[3 * i + 2], 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            h[size_t(place[size_t(i)])] = make_double4(host[3 * i], host[3 * i + 1], host[3 * i + 2], 0.0);
         HIPCHECK(hipMemcpyAsync(b->pos.p, h.data(), size_t(n) * sizeof(double4), hipMemcpyHostToDevice, c->stream));
         sync(c);
         // positions changed outside the time loop: every replica rebuilds at the next step
@@ -1485,7 +1565,7 @@ int mrmd_batch_upload(mrmd_batch* b, int field, const double* host)
     {
         std::vector<double> h(static_cast<size_t>(3 * b->cap), 0.0);
         for (int64_t i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) h[size_t(d * b->cap + i)] = host[3 * i + d];
+            for (int d = 0; d < 3; ++d) h[size_t(d * b->cap + place[size_t(i)])] = host[3 * i + d];
         HIPCHECK(hipMemcpyAsync(b->vel.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
     }
     sync(c);
@@ -1499,6 +1579,7 @@ int mrmd_batch_download(mrmd_batch* b, int field, double* host)
             "download takes POS, VEL or FORCE");
     const int64_t n = b->natoms;
     mrmd_ctx* c = b->ctx;
+    const std::vector<int32_t> place = placeOf(b);
     if (field == MRMD_POS)
     {
         std::vector<double4> h(static_cast<size_t>(n));
@@ -1506,9 +1587,10 @@ int mrmd_batch_download(mrmd_batch* b, int field, double* host)
         sync(c);
         for (int64_t i = 0; i < n; ++i)
         {
-            host[3 * i] = h[size_t(i)].x;
-            host[3 * i + 1] = h[size_t(i)].y;
-            host[3 * i + 2] = h[size_t(i)].z;
+            const double4& q = h[size_t(place[size_t(i)])];
+            host[3 * i] = q.x;
+            host[3 * i + 1] = q.y;
+            host[3 * i + 2] = q.z;
         }
     }
     else
@@ -1518,7 +1600,7 @@ int mrmd_batch_download(mrmd_batch* b, int field, double* host)
                                 hipMemcpyDeviceToHost, c->stream));
         sync(c);
         for (int64_t i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) host[3 * i + d] = h[size_t(d * b->cap + i)];
+            for (int d = 0; d < 3; ++d) host[3 * i + d] = h[size_t(d * b->cap + place[size_t(i)])];
     }
     MRMD_API_END
 }

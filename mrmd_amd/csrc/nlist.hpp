@@ -171,23 +171,31 @@ __device__ inline void putSentinel(void* st, int32_t maxStaged, double coord, do
 // insertion-sorted by lane 0 of the group alone, in global memory.  `n` is the same for the
 // 32 lanes of a group.
 constexpr int SORT_GROUP = 32;
-template <int E>
-__device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict__ pos, int lane)
+// KEY: ties in z are broken by key[record] instead of the record index (records that move
+// keep their key)
+template <int E, bool KEY = false>
+__device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict__ pos, int lane,
+                                 const int32_t* __restrict__ key = nullptr)
 {
     double z[E];
-    int32_t v[E];
+    int32_t v[E], k[E];
 #pragma unroll
     for (int r = 0; r < E; ++r)
     {
         const int e = r * SORT_GROUP + lane;
         v[r] = e < n ? recs[e] : INT32_MAX;
         z[r] = e < n ? pos[v[r]].z : HUGE_VAL;
+        if constexpr (KEY) k[r] = e < n ? key[v[r]] : INT32_MAX;
     }
     auto less = [](double za, int32_t va, double zb, int32_t vb) { return za < zb || (za == zb && va < vb); };
+    auto tie = [&](int r) -> int32_t {
+        if constexpr (KEY) return k[r];
+        else return v[r];
+    };
 #pragma unroll
-    for (int k = 2; k <= E * SORT_GROUP; k <<= 1)
+    for (int kk = 2; kk <= E * SORT_GROUP; kk <<= 1)
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1)
         {
             if (j >= SORT_GROUP)
             {
@@ -199,8 +207,8 @@ __device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict
                     const int rp = r ^ rj;
                     if (rp < r) continue;
                     const int e = r * SORT_GROUP + lane;  // the lower place of the pair
-                    const bool asc = (e & k) == 0;
-                    if (asc == less(z[rp], v[rp], z[r], v[r]))
+                    const bool asc = (e & kk) == 0;
+                    if (asc == less(z[rp], tie(rp), z[r], tie(r)))
                     {
                         const double tz = z[r];
                         const int32_t tv = v[r];
@@ -208,6 +216,12 @@ __device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict
                         v[r] = v[rp];
                         z[rp] = tz;
                         v[rp] = tv;
+                        if constexpr (KEY)
+                        {
+                            const int32_t tk = k[r];
+                            k[r] = k[rp];
+                            k[rp] = tk;
+                        }
                     }
                 }
             }
@@ -219,13 +233,16 @@ __device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict
                     const int e = r * SORT_GROUP + lane;
                     const double zo = __shfl_xor(z[r], j, SORT_GROUP);
                     const int32_t vo = __shfl_xor(v[r], j, SORT_GROUP);
+                    int32_t ko = vo;
+                    if constexpr (KEY) ko = __shfl_xor(k[r], j, SORT_GROUP);
                     // the lower place of a pair keeps the smaller record in an ascending block
-                    // (e & k == 0) and the larger in a descending one, the upper place the other
-                    const bool wantMin = ((e & k) == 0) != ((e & j) != 0);
-                    if (wantMin ? less(zo, vo, z[r], v[r]) : less(z[r], v[r], zo, vo))
+                    // (e & kk == 0) and the larger in a descending one, the upper place the other
+                    const bool wantMin = ((e & kk) == 0) != ((e & j) != 0);
+                    if (wantMin ? less(zo, ko, z[r], tie(r)) : less(z[r], tie(r), zo, ko))
                     {
                         z[r] = zo;
                         v[r] = vo;
+                        if constexpr (KEY) k[r] = ko;
                     }
                 }
             }
@@ -238,26 +255,33 @@ __device__ inline void sortGroup(int32_t* recs, int n, const double4* __restrict
     }
 }
 
-__device__ inline void sortCellByZ(int32_t* recs, int n, const double4* __restrict__ pos, int lane)
+template <bool KEY = false>
+__device__ inline void sortCellByZ(int32_t* recs, int n, const double4* __restrict__ pos, int lane,
+                                   const int32_t* __restrict__ key = nullptr)
 {
     if (n <= 1) return;
     if (n <= SORT_GROUP)
     {
-        sortGroup<1>(recs, n, pos, lane);
+        sortGroup<1, KEY>(recs, n, pos, lane, key);
         return;
     }
     if (n <= 4 * SORT_GROUP)
     {
-        sortGroup<4>(recs, n, pos, lane);
+        sortGroup<4, KEY>(recs, n, pos, lane, key);
         return;
     }
     if (lane != 0) return;
+    auto tieOf = [&](int32_t v) -> int32_t {
+        if constexpr (KEY) return key[v];
+        else return v;
+    };
     for (int i = 1; i < n; ++i)
     {
         const int32_t v = recs[i];
         const double zv = pos[v].z;
+        const int32_t tv = tieOf(v);
         int j = i - 1;
-        while (j >= 0 && (zv < pos[recs[j]].z || (zv == pos[recs[j]].z && v < recs[j])))
+        while (j >= 0 && (zv < pos[recs[j]].z || (zv == pos[recs[j]].z && tv < tieOf(recs[j]))))
         {
             recs[j + 1] = recs[j];
             --j;
