@@ -526,8 +526,8 @@ int mrmd_batch_get_tiling(const mrmd_batch* b, int* target_staged, int* block, i
  * global memory with identical arithmetic, so results do not depend on it (a tuning and
  * testing knob, no reference counterpart); forces a list rebuild at the next step */
 int mrmd_batch_set_stage_cap(mrmd_batch* b, int limit);
-/* all replicas' atoms concatenated in replica order, n_total x 3 row-major (POS, VEL; FORCE
- * for download) */
+/* all replicas' atoms concatenated in replica order, each replica's atoms in the order
+ * given, n_total x 3 row-major (POS, VEL; FORCE for download) */
 int mrmd_batch_upload(mrmd_batch* b, int field, const double* host);
 int mrmd_batch_download(mrmd_batch* b, int field, double* host);
 /* advance every replica by nsteps; graph_steps > 0 replays a captured graph of that many
@@ -541,7 +541,8 @@ int mrmd_batch_stats(mrmd_batch* b, int64_t* rebuilds, int64_t* steps);
 int mrmd_batch_tf_get(mrmd_batch* b, int64_t replica, int which, double* host, int64_t n);
 /* Checkpoint of the ensemble's dynamic state (no counterpart in the reference, whose
  * checkpoints are H5MD files of one system, io/DumpH5MD.hpp:30-75, io/RestoreH5MD.hpp:36-39):
- * positions, velocities, forces, the step counter that keys the Langevin noise, rebuild
+ * positions, velocities, forces and each atom's given index (atoms sit in cell order on the
+ * device), the step counter that keys the Langevin noise, rebuild
  * and density-sample counts, thermodynamic-force and density bins, compensation-energy
  * sums, counts and means.  state_bytes gives the blob's size; save writes it, load reads a
  * blob of the same replicas into an ensemble created with the same replica descriptions.
