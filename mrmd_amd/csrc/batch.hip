@@ -386,58 +386,87 @@ __global__ void k_bPermute2(BS b)
 // slot ranges, image shifts and LDS bases of every tile of a flagged replica: the staged
 // columns i-major, each the pieces of its z-run [k0 - 1, k1 + 1) (below 0 the image -1,
 // from nz the image +1), and the rows of the self columns (their unshifted pieces)
-__global__ void k_bTiles(BS b)
+// one wave per tile (four per workgroup): lane L takes staged column L of the
+// (ni + 2) x (nj + 2) columns (i-major, the serial order of the layout), its up to three
+// pieces and, for a self column, its rows; wave prefix sums place the pieces, the LDS bases
+// and the rows in that order
+__global__ __launch_bounds__(BLOCK) void k_bTiles(BS b)
 {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = int64_t(blockIdx.x) * (BLOCK / WAVE) + threadIdx.x / WAVE;
     if (t >= b.ntiles) return;
     const TileG G = b.tg[t];
     if (!b.flag[G.rep]) return;
     const RepD& R = b.reps[G.rep];
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     TileB& ti = b.tiles[t];
-    const int spanJ = G.nj + 2;
-    int nr = 0, staged = 0, nself = 0, nrows = 0;
-    for (int sa = 0; sa < G.ni + 2; ++sa)
-        for (int sb = 0; sb < spanJ; ++sb)
+    const int lane = int(threadIdx.x) % WAVE;
+    const int spanJ = G.nj + 2, ncol = (G.ni + 2) * spanJ;
+    const bool col = lane < ncol;
+    const int sa = lane / spanJ, sb = lane % spanJ;
+    int ii = G.i0 - 1 + sa, jj = G.j0 - 1 + sb, sx = 0, sy = 0;
+    if (ii < 0) ii += nx, sx = -1;
+    else if (ii >= nx) ii -= nx, sx = 1;
+    if (jj < 0) jj += ny, sy = -1;
+    else if (jj >= ny) jj -= ny, sy = 1;
+    const bool self = col && sa >= 1 && sa <= G.ni && sb >= 1 && sb <= G.nj;
+    const int zs = G.k0 - 1, ze = G.k1 + 1;
+    const int lo[3] = {zs < 0 ? nz - 1 : 0, zs < 0 ? 0 : zs, 0};
+    const int hi[3] = {zs < 0 ? nz : 0, ze < nz ? ze : nz, ze > nz ? ze - nz : 0};
+    int np = 0, len[3] = {0, 0, 0}, beg[3] = {0, 0, 0}, mainPiece = 0, rows = 0, rowBegin = 0;
+    if (col)
+        for (int pc = 0; pc < 3; ++pc)
         {
-            int ii = G.i0 - 1 + sa, jj = G.j0 - 1 + sb, sx = 0, sy = 0;
-            if (ii < 0) ii += nx, sx = -1;
-            else if (ii >= nx) ii -= nx, sx = 1;
-            if (jj < 0) jj += ny, sy = -1;
-            else if (jj >= ny) jj -= ny, sy = 1;
-            ti.colBase[sa * spanJ + sb] = staged;
-            const bool self = sa >= 1 && sa <= G.ni && sb >= 1 && sb <= G.nj;
-            const int zs = G.k0 - 1, ze = G.k1 + 1;
-            const int lo[3] = {zs < 0 ? nz - 1 : 0, zs < 0 ? 0 : zs, 0};
-            const int hi[3] = {zs < 0 ? nz : 0, ze < nz ? ze : nz, ze > nz ? ze - nz : 0};
-            const int sh[3] = {-1, 0, 1};
-            for (int pc = 0; pc < 3; ++pc)
-            {
-                if (hi[pc] <= lo[pc]) continue;
-                const int cf = cellIndex(R, ii, jj, lo[pc]), cl = cellIndex(R, ii, jj, hi[pc] - 1);
-                ti.begin[nr] = b.offsets[cf];
-                ti.base[nr] = staged;
-                ti.sx[nr] = int8_t(sx);
-                ti.sy[nr] = int8_t(sy);
-                ti.sz[nr] = int8_t(sh[pc]);
-                if (self && sh[pc] == 0)
-                {
-                    const int rf = cellIndex(R, ii, jj, G.k0), rl = cellIndex(R, ii, jj, G.k1 - 1);
-                    ti.selfPiece[nself] = nr;
-                    ti.selfBegin[nself] = b.offsets[rf];
-                    ti.selfPrefix[nself] = nrows;
-                    nrows += b.offsets[rl] + b.counts[rl] - b.offsets[rf];
-                    ++nself;
-                }
-                staged += b.offsets[cl] + b.counts[cl] - b.offsets[cf];
-                ++nr;
-            }
+            if (hi[pc] <= lo[pc]) continue;
+            const int cf = cellIndex(R, ii, jj, lo[pc]), cl = cellIndex(R, ii, jj, hi[pc] - 1);
+            beg[pc] = b.offsets[cf];
+            len[pc] = b.offsets[cl] + b.counts[cl] - beg[pc];
+            if (pc == 1) mainPiece = np;
+            ++np;
         }
-    ti.selfPrefix[nself] = nrows;
-    ti.nr = nr;
-    ti.staged = staged;
-    ti.nself = nself;
-    ti.nrows = nrows;
+    if (self)
+    {
+        const int rf = cellIndex(R, ii, jj, G.k0), rl = cellIndex(R, ii, jj, G.k1 - 1);
+        rowBegin = b.offsets[rf];
+        rows = b.offsets[rl] + b.counts[rl] - rowBegin;
+    }
+    const int staged = len[0] + len[1] + len[2];
+    const int pieceBase = waveInclusiveScan(np) - np;
+    const int stagedBase = waveInclusiveScan(staged) - staged;
+    const int selfIdx = waveInclusiveScan(self ? 1 : 0) - (self ? 1 : 0);
+    const int rowsBase = waveInclusiveScan(rows) - rows;
+    if (col)
+    {
+        ti.colBase[lane] = stagedBase;
+        const int sh[3] = {-1, 0, 1};
+        int nr = pieceBase, base = stagedBase;
+        for (int pc = 0; pc < 3; ++pc)
+        {
+            if (hi[pc] <= lo[pc]) continue;
+            ti.begin[nr] = beg[pc];
+            ti.base[nr] = base;
+            ti.sx[nr] = int8_t(sx);
+            ti.sy[nr] = int8_t(sy);
+            ti.sz[nr] = int8_t(sh[pc]);
+            base += len[pc];
+            ++nr;
+        }
+    }
+    if (self)
+    {
+        ti.selfPiece[selfIdx] = pieceBase + mainPiece;
+        ti.selfBegin[selfIdx] = rowBegin;
+        ti.selfPrefix[selfIdx] = rowsBase;
+    }
+    const int nrTot = __shfl(pieceBase + np, WAVE - 1, WAVE), stTot = __shfl(stagedBase + staged, WAVE - 1, WAVE);
+    const int nself = __shfl(selfIdx + (self ? 1 : 0), WAVE - 1, WAVE), nrows = __shfl(rowsBase + rows, WAVE - 1, WAVE);
+    if (lane == 0)
+    {
+        ti.selfPrefix[nself] = nrows;
+        ti.nr = nrTot;
+        ti.staged = stTot;
+        ti.nself = nself;
+        ti.nrows = nrows;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1215,7 +1244,7 @@ void mrmd_batch::enqueueSteps(int64_t n)
     if (n <= 0) return;
     mrmd_ctx* c = ctx;
     const BS b = state();
-    const unsigned ga = gridFor(natoms), gc = gridFor(ncells), gt = gridFor(ntiles);
+    const unsigned ga = gridFor(natoms), gc = gridFor(ncells);
     const unsigned gs = unsigned((ncells * SORT_GROUP + BLOCK - 1) / BLOCK);
     const unsigned nr = unsigned(reps.size());
     hipLaunchKernelGGL(k_bPre<false>, dim3(ga), dim3(BLOCK), 0, c->stream, b);
@@ -1230,7 +1259,8 @@ void mrmd_batch::enqueueSteps(int64_t n)
         hipLaunchKernelGGL(k_bSortCells, dim3(gs), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bPermute1, dim3(ga), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bPermute2, dim3(ga), dim3(BLOCK), 0, c->stream, b);
-        hipLaunchKernelGGL(k_bTiles, dim3(gt), dim3(BLOCK), 0, c->stream, b);
+        hipLaunchKernelGGL(k_bTiles, dim3(unsigned((ntiles + BLOCK / WAVE - 1) / (BLOCK / WAVE))), dim3(BLOCK), 0,
+                           c->stream, b);
         hipLaunchKernelGGL(k_bGather, dim3(ga), dim3(BLOCK), 0, c->stream, b);
         hipLaunchKernelGGL(k_bBuild, dim3(unsigned(ntiles)), dim3(block), size_t(stageCapBuild + B_PAD) * 24, c->stream,
                            b);
