@@ -108,6 +108,33 @@ __device__ inline void copyAtom(const AtomsView& a, int64_t dst, int64_t src, in
     a.relmass[dst] = a.relmass[src];
 }
 
+// PeriodicMapping::mapIntoDomain (PeriodicMapping.cpp:30-58) of one position on the box
+// [min, max), diam = max - min
+__device__ inline void wrapInto(double4& p, const double* min, const double* max, const double* diam)
+{
+    double x[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (max[d] <= x[d])
+        {
+            x[d] -= diam[d];
+            x[d] = fmax(x[d], min[d]);
+        }
+        if (x[d] < min[d])
+        {
+            x[d] += diam[d];
+            if (max[d] <= x[d]) x[d] = min[d];
+        }
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+}
+struct WrapBox
+{
+    double min[3], max[3], diam[3];
+};
+
 __device__ inline double coord(const double4& p, int axis)
 {
     return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);

@@ -56,23 +56,7 @@ __global__ void k_wrap(double4* pos, int64_t n, SubD s)
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i];
-    double x[3] = {p.x, p.y, p.z};
-    for (int d = 0; d < 3; ++d)
-    {
-        if (s.max[d] <= x[d])
-        {
-            x[d] -= s.diam[d];
-            x[d] = fmax(x[d], s.min[d]);
-        }
-        if (x[d] < s.min[d])
-        {
-            x[d] += s.diam[d];
-            if (s.max[d] <= x[d]) x[d] = s.min[d];
-        }
-    }
-    p.x = x[0];
-    p.y = x[1];
-    p.z = x[2];
+    wrapInto(p, s.min, s.max, s.diam);
     pos[i] = p;
 }
 

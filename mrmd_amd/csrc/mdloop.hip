@@ -59,13 +59,15 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         if (s->max_disp >= s->skin * 0.5)
         {
             s->max_disp = 0.0;
-            MD_CALL(mrmd_halo_wrap(a, box));
             if (cells)
             {
-                MD_CALL(mrmd_cell_list_build(cells, a, 0, a->nlocal, delta, box->min, box->max));
+                // PeriodicMapping and the cell list in one pass over the positions
+                cellListBuildWrapped(cells, a, box, delta);
                 // the ghosts are made again and every force computed again before either is read
                 permuteAtoms(a, cells, false);
             }
+            else
+                MD_CALL(mrmd_halo_wrap(a, box));
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
             MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, s->grid_min, s->grid_max,

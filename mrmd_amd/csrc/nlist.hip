@@ -31,14 +31,22 @@ GridD makeGrid(const double* gmin, const double* gmax, const double* delta)
 
 namespace
 {
-__global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g, int32_t* cellOf, int32_t* counts)
+// WRAP: each position is first mapped into the box (PeriodicMapping) and written back
+template <bool WRAP>
+__global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g, int32_t* cellOf, int32_t* counts,
+                           double4* wrapPos, WrapBox wb)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = k < n;
     int c = -1;
     if (valid)
     {
-        const double4 p = pos[begin + k];
+        double4 p = pos[begin + k];
+        if (WRAP)
+        {
+            wrapInto(p, wb.min, wb.max, wb.diam);
+            wrapPos[begin + k] = p;
+        }
         c = g.cell(p.x, p.y, p.z);
         cellOf[k] = c;
     }
@@ -173,7 +181,7 @@ __global__ void k_zeroI32(int32_t* p, int64_t n)
 }  // namespace
 
 void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
-                    bool zOrder)
+                    bool zOrder, double4* wrapPos, const WrapBox* wrap)
 {
     grid = g;
     begin = b;
@@ -188,8 +196,14 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
     hipLaunchKernelGGL(k_zeroI32, dim3(gridFor(ncell)), dim3(BLOCK), 0, ctx->stream, counts.p, int64_t(ncell));
     hipLaunchKernelGGL(k_zeroI32, dim3(gridFor(ncell)), dim3(BLOCK), 0, ctx->stream, fill.p, int64_t(ncell));
     if (n > 0)
-        hipLaunchKernelGGL(k_binCount, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, b, n, g, cellOf.p,
-                           counts.p);
+    {
+        if (wrap)
+            hipLaunchKernelGGL(k_binCount<true>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, b, n, g, cellOf.p,
+                               counts.p, wrapPos, *wrap);
+        else
+            hipLaunchKernelGGL(k_binCount<false>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, b, n, g,
+                               cellOf.p, counts.p, nullptr, WrapBox{});
+    }
     scanExclusiveI32(ctx, counts.p, offsets.p, ncell, nullptr);
     if (n > 0)
     {
@@ -258,6 +272,21 @@ __global__ void k_copyBack(T* dst, const T* src, int64_t n)
 
 namespace mrmd_hip
 {
+// PeriodicMapping of the real atoms fused with the cell list build over them (the LJ time
+// loop's rebuild: the wrap and the binning's pass over the positions are one pass)
+void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta)
+{
+    KTimer t(cl->ctx, "cell_list_build");
+    WrapBox wb;
+    for (int d = 0; d < 3; ++d)
+    {
+        wb.min[d] = box->min[d];
+        wb.max[d] = box->max[d];
+        wb.diam[d] = box->max[d] - box->min[d];
+    }
+    cl->bin.build(cl->ctx, a->pos.p, 0, a->nlocal, makeGrid(box->min, box->max, delta), false, false, a->pos.p, &wb);
+}
+
 // Atoms::permute by a cell list's order; full = false (the time loop, right before the ghosts
 // are made again and the forces computed again) moves neither the forces of the permuted
 // records nor the records outside the list's range, whose values are rewritten before use

@@ -62,8 +62,10 @@ struct Binning
     DBuf<int32_t> sorted;   ///< slot -> record index
     DBuf<double4> sortedPos;
 
+    /// wrap (if given): every position is first mapped into that box and written to
+    /// wrapPos (the same array as pos for an in-place wrap)
     void build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
-               bool zOrder = false);
+               bool zOrder = false, double4* wrapPos = nullptr, const WrapBox* wrap = nullptr);
 };
 }  // namespace mrmd_hip
 
@@ -79,6 +81,8 @@ namespace mrmd_hip
 // the permuted records' forces and the records outside the list's range unset, as the loop
 // recomputes both before reading them)
 void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full);
+// PeriodicMapping of the real atoms and the cell list build over them in one pass
+void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta);
 }
 
 namespace mrmd_hip
