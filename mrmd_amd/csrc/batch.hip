@@ -490,13 +490,26 @@ __global__ void k_bGather(BS b)
             if (bin != -1) key = R.h0 + bin;
         }
         b.sortedPos[s] = make_double4(p.x, p.y, p.z, mod);
-        b.slotLam[s] = lam;
-        b.slotGrad[s] = g;
+        // lambda and its gradient are read for hybrid records only (0 < lambda_mod < 1)
+        if (mod > 0.0 && mod < 1.0)
+        {
+            b.slotLam[s] = lam;
+            b.slotGrad[s] = g;
+        }
     }
-    // atoms in cell order share their x-bin over long runs: one atomic per run of a wave
-    // (counts are integers, so the sums are exact in any order)
-    const Run r = laneRun(int(key), valid);
-    if (valid && key >= 0 && r.head == int(threadIdx.x) % WAVE) atomicAddD(&b.tfDensity[key], double(r.len));
+    // a wave's atoms (a few cells of one column, so one x-cell) fall into a few x-bins: one
+    // atomic per distinct bin of the wave (counts are integers, so the sums are exact in any
+    // order)
+    const int lane = int(threadIdx.x) % WAVE;
+    const int k32 = int(key);
+    for (unsigned long long pending = __ballot(k32 >= 0); pending;)
+    {
+        const int src = __builtin_ctzll(pending);
+        const int k0 = __shfl(k32, src, WAVE);
+        const unsigned long long m = __ballot(k32 == k0);
+        if (lane == src) atomicAddD(&b.tfDensity[k0], double(__popcll(m)));
+        pending &= ~m;
+    }
 }
 
 // copy a tile's neighbourhood into LDS, shifting each piece by its periodic image: {x, y, z}
