@@ -196,9 +196,20 @@ __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
         p.y += dtHalf * vy;
         p.z += dtHalf * vz;
     }
-    double xi[3];
-    waveNormals3(valid ? R->seed : 0, valid ? uint64_t(b.id[i] - R->a0) : 0, uint64_t(*b.step), valid,
-                 draws[threadIdx.x / WAVE], xi);
+    // the noise, one replica of the wave at a time (nearly always one): its seed is then
+    // wave-uniform, so the Philox key stays in scalar registers
+    double xi[3] = {0.0, 0.0, 0.0};
+    const uint64_t id = valid ? uint64_t(b.id[i] - R->a0) : 0;
+    for (unsigned long long pending = __ballot(valid); pending;)
+    {
+        const int rr = __builtin_amdgcn_readfirstlane(__shfl(r, __builtin_ctzll(pending), WAVE));
+        const bool mine = valid && r == rr;
+        double xt[3];
+        waveNormals3(b.reps[rr].seed, mine ? id : 0, uint64_t(*b.step), mine, draws[threadIdx.x / WAVE], xt);
+        if (mine)
+            for (int d = 0; d < 3; ++d) xi[d] = xt[d];
+        pending &= ~__ballot(mine);
+    }
     if (valid)
     {
         const double damping = R->damping, sigma = R->sigma;

@@ -577,6 +577,10 @@ void rowRegion(mrmd_verlet_list* vl)
     readbackI32(ctx, box, 6, h);
     for (int d = 0; d < 3; ++d)
     {
+        // rows exist, so every axis holds at least one cell of them
+        if (h[d] < 0 || h[3 + d] < h[d] || h[3 + d] >= vl->bin.grid.n[d])
+            fail(MRMD_ERR_STATE, "cell box of the list rows out of the grid: axis %d cells [%d, %d] of %d (%lld rows)",
+                 d, h[d], h[3 + d], vl->bin.grid.n[d], (long long)rows);
         vl->rowOrigin[d] = h[d];
         vl->rowExtent[d] = h[3 + d] - h[d] + 1;
     }
