@@ -71,8 +71,8 @@ struct mrmd_ctx
     std::string profileOnly;  ///< non-empty: only timers of this name record
     std::map<std::string, mrmd_hip::KStat> stats;
     std::vector<hipEvent_t> eventPool;
-    double* pinned = nullptr;  // small pinned host staging area (64 doubles, coherent)
-    double hostSeq = 0.0;      // last sequence number a kernel was asked to post to pinned
+    double* pinned = nullptr;  // small pinned host staging area (64 words, coherent)
+    uint32_t hostTag = 0;      // tag of the last call whose result kernels post to pinned
     int64_t bytes = 0;         // device bytes held by objects of this context
     void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};  // per-purpose device scratch
     size_t scratchBytes[4] = {0, 0, 0, 0};
@@ -254,18 +254,29 @@ inline void flushStats(mrmd_ctx* ctx)
 
 inline void sync(mrmd_ctx* ctx) { HIPCHECK(hipStreamSynchronize(ctx->stream)); }
 
-// Kernels post small results to the context's coherent pinned buffer followed by a sequence
-// number in a slot of their own; the host polls that slot (waitHostSeq), which returns as
-// soon as the number lands, and synchronises the stream only as a fallback after 20 ms.
-// Slots (doubles): 0 the displacement maximum, 1 its sequence number, 2 the sequence number
-// of readbackI32, 8.. its integers.
-constexpr int DISP_SEQ_SLOT = 1;
-constexpr int READBACK_SEQ_SLOT = 2;
-constexpr int READBACK_INT_SLOT = 8;
+// Kernels post small results to the context's coherent pinned buffer as tagged 64-bit words:
+// the high half is the call's tag (nextTag: a per-context counter, never 0), the low half a
+// 32-bit value, each word written by one atomic 64-bit store.  The host polls until every
+// word of the call carries the call's tag (waitTags), so a value is taken only from a word
+// that this call wrote, whatever order the words reach host memory in; the stream is
+// synchronised only as a fallback after 20 ms.  Words (uint64 at ctx->pinned, 64 of them):
+// 0..3 untagged host copies (energies, after a synchronisation), 4..5 the displacement
+// maximum (low, high half of the double), 8..23 the integers of readbackI32.
+constexpr int DISP_WORD = 4;
+constexpr int READBACK_WORD = 8;
 constexpr int READBACK_INTS = 16;
-void waitHostSeq(mrmd_ctx* ctx, int slot, double want);
+constexpr int PINNED_WORDS = 64;
+uint32_t nextTag(mrmd_ctx* ctx);
+void waitTags(mrmd_ctx* ctx, int word, int n, uint32_t tag);
+// the low halves of n tagged words (after waitTags)
+inline uint32_t tagValue(const mrmd_ctx* ctx, int word)
+{
+    return uint32_t(__atomic_load_n(reinterpret_cast<const uint64_t*>(ctx->pinned) + word, __ATOMIC_ACQUIRE));
+}
 // n (<= READBACK_INTS) int32 device values to the host through the pinned buffer
 void readbackI32(mrmd_ctx* ctx, const int32_t* dev, int n, int32_t* out);
+
+__host__ __device__ inline uint64_t tagWord(uint32_t tag, uint32_t v) { return (uint64_t(tag) << 32) | v; }
 
 inline void checkLaunch() { HIPCHECK(hipGetLastError()); }
 

@@ -145,11 +145,28 @@ void scanExclusiveI32Multi(mrmd_ctx* ctx, const int32_t* in, int32_t* out, int64
     for (int k = 0; k < count; ++k) scanImpl<int32_t>(ctx, in + k * n, out + k * n, n, totalsDev + k);
 }
 
-void waitHostSeq(mrmd_ctx* ctx, int slot, double want)
+uint32_t nextTag(mrmd_ctx* ctx)
 {
-    volatile const double* h = ctx->pinned;
+    if (++ctx->hostTag == 0)
+    {
+        // the counter wrapped: no word may still hold a tag that is about to be reused
+        sync(ctx);
+        std::memset(static_cast<void*>(ctx->pinned), 0, PINNED_WORDS * sizeof(uint64_t));
+        ctx->hostTag = 1;
+    }
+    return ctx->hostTag;
+}
+
+void waitTags(mrmd_ctx* ctx, int word, int n, uint32_t tag)
+{
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(ctx->pinned) + word;
+    auto tagged = [&] {
+        for (int k = 0; k < n; ++k)
+            if (uint32_t(__atomic_load_n(w + k, __ATOMIC_ACQUIRE) >> 32) != tag) return false;
+        return true;
+    };
     const auto t0 = std::chrono::steady_clock::now();
-    while (h[slot] != want)
+    while (!tagged())
     {
         if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
         {
@@ -157,22 +174,20 @@ void waitHostSeq(mrmd_ctx* ctx, int slot, double want)
             break;
         }
     }
-    std::atomic_thread_fence(std::memory_order_acquire);
-    if (h[slot] != want) fail(MRMD_ERR_STATE, "a device result did not reach the host");
+    if (!tagged()) fail(MRMD_ERR_STATE, "a device result did not reach the host");
 }
 }  // namespace mrmd_hip
 
 namespace
 {
-// copies n int32 values into the coherent pinned buffer, then (after a system-scope fence)
-// the sequence number
-__global__ void k_postI32(const int32_t* src, int n, int32_t* hostInts, double* hostSeq, double seq)
+// n int32 values as tagged words into the coherent pinned buffer: agent-scope loads (the
+// values of the previous kernels, from any XCD), one system-scope 64-bit store per word
+__global__ void k_postI32(const int32_t* src, int n, uint64_t* host, uint32_t tag)
 {
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < n; ++k) hostInts[k] = src[k];
-    __threadfence_system();
-    *hostSeq = seq;
-    __threadfence_system();
+    const int k = threadIdx.x;
+    if (k >= n) return;
+    const int32_t v = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(host + k, tagWord(tag, uint32_t(v)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 }  // namespace
 
@@ -181,14 +196,13 @@ namespace mrmd_hip
 void readbackI32(mrmd_ctx* ctx, const int32_t* dev, int n, int32_t* out)
 {
     if (n < 0 || n > READBACK_INTS) fail(MRMD_ERR_ARG, "readback of %d values (at most %d)", n, READBACK_INTS);
-    ctx->hostSeq += 1.0;
-    ctx->pinned[READBACK_SEQ_SLOT] = -1.0;
-    int32_t* ints = reinterpret_cast<int32_t*>(ctx->pinned + READBACK_INT_SLOT);
-    hipLaunchKernelGGL(k_postI32, dim3(1), dim3(64), 0, ctx->stream, dev, n, ints, ctx->pinned + READBACK_SEQ_SLOT,
-                       ctx->hostSeq);
+    if (n == 0) return;
+    const uint32_t tag = nextTag(ctx);
+    hipLaunchKernelGGL(k_postI32, dim3(1), dim3(64), 0, ctx->stream, dev, n,
+                       reinterpret_cast<uint64_t*>(ctx->pinned) + READBACK_WORD, tag);
     checkLaunch();
-    waitHostSeq(ctx, READBACK_SEQ_SLOT, ctx->hostSeq);
-    for (int k = 0; k < n; ++k) out[k] = ints[k];
+    waitTags(ctx, READBACK_WORD, n, tag);
+    for (int k = 0; k < n; ++k) out[k] = int32_t(tagValue(ctx, READBACK_WORD + k));
 }
 }  // namespace mrmd_hip
 
@@ -401,6 +415,7 @@ int mrmd_ctx_create(int device, mrmd_ctx** out)
     ljNoAllocSetup(device);
     // coherent: kernels write scalars here for the host to read while the stream runs on
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), 64 * sizeof(double), hipHostMallocCoherent));
+    std::memset(static_cast<void*>(c->pinned), 0, PINNED_WORDS * sizeof(uint64_t));
     *out = c;
     return MRMD_OK;
     }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstring>
 
 #include "containers.hpp"
 #include "detmath.hpp"
@@ -292,11 +293,11 @@ __global__ void k_scaleVel(AtomsView a, int64_t n, double beta)
 
 // per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
 // eight independent loads per lane in flight)
-// host (if not null) is coherent pinned host memory that receives the result in host[0]
-// and then, after a system-scope fence, the sequence number seq in host[1]: the host can
-// take the value as soon as the number appears, without waiting for the stream
-__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, double* host,
-                                                       double seq)
+// host (if not null) is coherent pinned host memory that receives the result as two tagged
+// words (low and high half of the double, each with the call's tag in its high half): the
+// host takes the value as soon as both words carry the tag, without waiting for the stream
+__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, uint64_t* host,
+                                                       uint32_t tag)
 {
     __shared__ double red[BLOCK / WAVE];
     double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -314,10 +315,10 @@ __global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, i
         *out = r;
         if (host)
         {
-            host[0] = r;
-            __threadfence_system();
-            host[1] = seq;
-            __threadfence_system();
+            const uint64_t bits = uint64_t(__double_as_longlong(r));
+            __hip_atomic_store(host, tagWord(tag, uint32_t(bits)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(host + 1, tagWord(tag, uint32_t(bits >> 32)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -329,26 +330,35 @@ double* dispPartials(mrmd_atoms* a, unsigned grid)
     return static_cast<double*>(ctxScratch(a->ctx, 3, (size_t(grid) + 1) * sizeof(double))) + 1;
 }
 
-// The reduction kernel writes the maximum to ctx->pinned[0] and then the call's sequence
-// number to pinned[DISP_SEQ_SLOT] (dispHost stores both itself when no kernel runs); the
-// host polls for the number (waitHostSeq).
-void finishDisp(mrmd_atoms* a, double* maxDisp)
+// The reduction kernel posts the maximum as the two tagged words at DISP_WORD of the pinned
+// buffer (dispHost writes them itself when no kernel runs); the host polls for the call's tag
+// (waitTags).
+void finishDisp(mrmd_atoms* a, double* maxDisp, uint32_t tag)
 {
     if (!maxDisp) return;
-    waitHostSeq(a->ctx, DISP_SEQ_SLOT, a->ctx->hostSeq);
-    *maxDisp = sqrt(static_cast<volatile const double*>(a->ctx->pinned)[0]);
+    mrmd_ctx* c = a->ctx;
+    waitTags(c, DISP_WORD, 2, tag);
+    const uint64_t bits = uint64_t(tagValue(c, DISP_WORD)) | (uint64_t(tagValue(c, DISP_WORD + 1)) << 32);
+    double d2;
+    std::memcpy(&d2, &bits, sizeof(d2));
+    *maxDisp = sqrt(d2);
 }
 
-// target of the reduction's host copy (the context's pinned slots) when the caller wants
-// the value; nlocal = 0: no kernel writes them, so they are set here
-double* dispHost(mrmd_atoms* a, const double* maxDisp)
+// target of the reduction's host copy (the context's pinned words) and the call's tag when
+// the caller wants the value; nlocal = 0: no kernel writes them, so they are set here
+uint64_t* dispHost(mrmd_atoms* a, const double* maxDisp, uint32_t* tag)
 {
+    *tag = 0;
     if (!maxDisp) return nullptr;
     mrmd_ctx* c = a->ctx;
-    c->hostSeq += 1.0;
-    c->pinned[0] = 0.0;
-    c->pinned[DISP_SEQ_SLOT] = a->nlocal > 0 ? -1.0 : c->hostSeq;
-    return c->pinned;
+    *tag = nextTag(c);
+    uint64_t* w = reinterpret_cast<uint64_t*>(c->pinned) + DISP_WORD;
+    if (a->nlocal == 0)
+    {
+        __atomic_store_n(w, tagWord(*tag, 0u), __ATOMIC_RELEASE);
+        __atomic_store_n(w + 1, tagWord(*tag, 0u), __ATOMIC_RELEASE);
+    }
+    return w;
 }
 }  // namespace
 
@@ -363,7 +373,8 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
     const unsigned grid = gridFor(a->nlocal);
     double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
-    double* host = dispHost(a, maxDisp);
+    uint32_t tag;
+    uint64_t* host = dispHost(a, maxDisp, &tag);
     if (a->nlocal == 0) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
     {
@@ -388,10 +399,10 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
         else
             uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->hostSeq);
+                           tag);
     }
     checkLaunch();
-    finishDisp(a, maxDisp);
+    finishDisp(a, maxDisp, tag);
     MRMD_API_END
 }
 
@@ -405,18 +416,19 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     const unsigned grid = gridFor(a->nlocal);
     double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
-    double* host = dispHost(a, maxDisp);
+    uint32_t tag;
+    uint64_t* host = dispHost(a, maxDisp, &tag);
     if (a->nlocal > 0)
     {
         KTimer t(ctx, "vv_pre");
         hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->hostSeq);
+                           tag);
     }
     else
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     checkLaunch();
-    finishDisp(a, maxDisp);
+    finishDisp(a, maxDisp, tag);
     MRMD_API_END
 }
 
@@ -441,7 +453,8 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
     const unsigned grid = gridFor(a->nlocal);
     double* part = dispPartials(a, grid);
     double* ms = dispScalar(a);
-    double* host = dispHost(a, maxDisp);
+    uint32_t tag;
+    uint64_t* host = dispHost(a, maxDisp, &tag);
     if (a->nlocal == 0) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, ms);
     if (a->nlocal > 0)
     {
@@ -459,10 +472,10 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
         else
             uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
         hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
-                           ctx->hostSeq);
+                           tag);
     }
     checkLaunch();
-    finishDisp(a, maxDisp);
+    finishDisp(a, maxDisp, tag);
     MRMD_API_END
 }
 
