@@ -133,6 +133,8 @@ __device__ inline void wrapInto(double4& p, const double* min, const double* max
 struct WrapBox
 {
     double min[3], max[3], diam[3];
+    const double4* from = nullptr;   ///< if not null: the current positions are from[slotOf[i]]
+    const int32_t* slotOf = nullptr;
 };
 
 __device__ inline double coord(const double4& p, int axis)
@@ -147,11 +149,13 @@ __device__ inline double coord(const double4& p, int axis)
 struct HistD;
 // the fused post/pre step of the time loops; with adrMol (the AdResS loop, molecule k = atom
 // k) each local atom's molecule force (ContributeMoleculeForceToAtoms) and the thermodynamic
-// force of its x-bin (ThermodynamicForce::apply) are added to its force first
+// force of its x-bin (ThermodynamicForce::apply) are added to its force first.  With
+// sortedPos / slotOf each new position is stored at the atom's slot too; slotsOnly: the
+// positions are read from the slots and stored there alone (the positions array goes stale)
 int fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
                  uint64_t call, double* maxDisp, double4* sortedPos = nullptr, const int32_t* slotOf = nullptr,
                  const mrmd_molecules* adrMol = nullptr, const double* tfForce = nullptr,
-                 const HistD* tfHd = nullptr);
+                 const HistD* tfHd = nullptr, bool slotsOnly = false);
 // whether mrmd_lj_apply without a predicate uses the full-row kernel (no ghost forces)
 bool ljFullRows(const mrmd_lj* lj);
 // device address of the allocation-free apply's scratch array on `device` (the current
@@ -175,7 +179,11 @@ int ljApplyAssign(mrmd_lj* lj, mrmd_atoms* atoms, mrmd_verlet_list* vl);
 // zero the forces of records [lo, hi)
 int fillForceRange(mrmd_atoms* atoms, int64_t lo, int64_t hi);
 // UpdateGhostAtoms::updateOnlyPos that also writes each ghost's new position to
-// sortedPos[slotOf[ghost]] (the slot-order copy of a Verlet list)
+// sortedPos[slotOf[ghost]] (the slot-order copy of a Verlet list); slotsOnly: every record's
+// position is read from its slot and the ghost's stored there alone
 int ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,
-                      const int32_t* slotOf);
+                      const int32_t* slotOf, bool slotsOnly = false);
+// PeriodicMapping of the real atoms whose current positions are in a list's slot-order copy:
+// pos[i] = wrap(from[slotOf[i]]) for the local atoms
+int haloWrapFromSlots(mrmd_atoms* a, const mrmd_box* box, const double4* from, const int32_t* slotOf);
 }  // namespace mrmd_hip

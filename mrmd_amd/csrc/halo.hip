@@ -50,12 +50,12 @@ SubD makeSub(const mrmd_box* b)
     return s;
 }
 
-// PeriodicMapping.cpp:30-58
-__global__ void k_wrap(double4* pos, int64_t n, SubD s)
+// PeriodicMapping.cpp:30-58; from (if not null): the current positions are from[slotOf[i]]
+__global__ void k_wrap(double4* pos, int64_t n, SubD s, const double4* from, const int32_t* slotOf)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double4 p = pos[i];
+    double4 p = from ? from[slotOf[i]] : pos[i];
     wrapInto(p, s.min, s.max, s.diam);
     pos[i] = p;
 }
@@ -349,14 +349,18 @@ __global__ __launch_bounds__(BLOCK) void k_ghostScatter3(AtomsView a, int32_t* c
 
 // UpdateGhostAtoms.cpp:31-68; sortedPos (if not null): the new position is also written to
 // the record's sorted slot slotOf[i] of a Verlet list (the time loop's gather-free path)
+// SLOTS: every record's current position is at its slot of sortedPos, and the ghost's new
+// one is stored there alone
+template <bool SLOTS>
 __global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, int64_t end, SubD s,
                               double4* sortedPos, const int32_t* slotOf)
 {
     const int64_t i = begin + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= end) return;
     const int32_t real = corr[i];
-    double4 p = pos[i];
-    const double4 q = pos[real];
+    const int32_t slot = SLOTS ? slotOf[i] : 0;
+    double4 p = SLOTS ? sortedPos[slot] : pos[i];
+    const double4 q = SLOTS ? sortedPos[slotOf[real]] : pos[real];
     const double dx[3] = {p.x - q.x, p.y - q.y, p.z - q.z};
     double x[3] = {q.x, q.y, q.z};
     for (int d = 0; d < 3; ++d)
@@ -366,6 +370,11 @@ __global__ void k_ghostUpdate(double4* pos, const int32_t* corr, int64_t begin, 
     p.x = x[0];
     p.y = x[1];
     p.z = x[2];
+    if (SLOTS)
+    {
+        sortedPos[slot] = p;
+        return;
+    }
     pos[i] = p;
     if (sortedPos) sortedPos[slotOf[i]] = p;
 }
@@ -969,7 +978,7 @@ void checkBox(const mrmd_box* box)
 // UpdateGhostAtoms::updateOnlyPos with each new ghost position also written to its sorted
 // slot of a Verlet list (time loops that keep the list's slot-order copy current)
 int mrmd_hip::ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box, double4* sortedPos,
-                                const int32_t* slotOf)
+                                const int32_t* slotOf, bool slotsOnly)
 {
     MRMD_API_BEGIN(gl ? gl->ctx : nullptr)
     REQUIRE(gl && a && sortedPos && slotOf, "invalid arguments");
@@ -977,8 +986,23 @@ int mrmd_hip::ghostUpdateSorted(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_
     REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "correspondingRealAtom shorter than the atoms");
     if (a->nghost == 0) return MRMD_OK;
     KTimer t(gl->ctx, "halo_update");
-    hipLaunchKernelGGL(k_ghostUpdate, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
-                       gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box), sortedPos, slotOf);
+    auto kernel = slotsOnly ? k_ghostUpdate<true> : k_ghostUpdate<false>;
+    hipLaunchKernelGGL(kernel, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p, gl->corr.p,
+                       a->nlocal, a->nlocal + a->nghost, makeSub(box), sortedPos, slotOf);
+    checkLaunch();
+    MRMD_API_END
+}
+
+// PeriodicMapping of the local atoms from their slots of a list's slot-order copy
+int mrmd_hip::haloWrapFromSlots(mrmd_atoms* a, const mrmd_box* box, const double4* from, const int32_t* slotOf)
+{
+    MRMD_API_BEGIN(a ? a->ctx : nullptr)
+    REQUIRE(a && from && slotOf, "invalid arguments");
+    checkBox(box);
+    if (a->nlocal == 0) return MRMD_OK;
+    KTimer t(a->ctx, "halo_wrap");
+    hipLaunchKernelGGL(k_wrap, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, a->ctx->stream, a->pos.p, a->nlocal,
+                       makeSub(box), from, slotOf);
     checkLaunch();
     MRMD_API_END
 }
@@ -1011,7 +1035,7 @@ int mrmd_halo_wrap(mrmd_atoms* a, const mrmd_box* box)
     if (a->nlocal == 0) return MRMD_OK;
     KTimer t(a->ctx, "halo_wrap");
     hipLaunchKernelGGL(k_wrap, dim3(gridFor(a->nlocal)), dim3(BLOCK), 0, a->ctx->stream, a->pos.p, a->nlocal,
-                       makeSub(box));
+                       makeSub(box), nullptr, nullptr);
     checkLaunch();
     MRMD_API_END
 }
@@ -1048,7 +1072,7 @@ int mrmd_halo_update(mrmd_ghost_layer* gl, mrmd_atoms* a, const mrmd_box* box)
     REQUIRE(gl->corrSize >= a->nlocal + a->nghost, "correspondingRealAtom shorter than the atoms");
     if (a->nghost == 0) return MRMD_OK;
     KTimer t(gl->ctx, "halo_update");
-    hipLaunchKernelGGL(k_ghostUpdate, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
+    hipLaunchKernelGGL(k_ghostUpdate<false>, dim3(gridFor(a->nghost)), dim3(BLOCK), 0, gl->ctx->stream, a->pos.p,
                        gl->corr.p, a->nlocal, a->nlocal + a->nghost, makeSub(box), nullptr, nullptr);
     checkLaunch();
     MRMD_API_END

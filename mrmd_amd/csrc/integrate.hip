@@ -169,7 +169,10 @@ struct AdrFold
     HistD hd;
 };
 
-template <bool LANGEVIN, bool UNI, bool ADR = false>
+// SLOTS: the positions live in sortedPos alone (the time loop's slot-order copy, current for
+// every record): each atom's position is read from and stored to its slot only, and the
+// positions array is left as it was
+template <bool LANGEVIN, bool UNI, bool ADR = false, bool SLOTS = false>
 __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, double dt, double zeta, double temperature,
                                                    uint64_t seed, uint64_t call, LangevinFactors uf, double* partials,
                                                    double4* sortedPos, const int32_t* slotOf, AdrFold adr)
@@ -182,6 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, doubl
     const double dtHalf = 0.5 * dt;
     double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
     double vx = 0.0, vy = 0.0, vz = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+    int32_t slot = 0;
     LangevinFactors f = uf;
     if (live)
     {
@@ -206,7 +210,13 @@ __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, doubl
         vy += dtfm * fy;
         vz += dtfm * fz;
         // pre-force kick and drift of the next step
-        p = a.pos[i];
+        if (SLOTS)
+        {
+            slot = slotOf[i];
+            p = sortedPos[slot];
+        }
+        else
+            p = a.pos[i];
         vx += dtfm * fx;
         vy += dtfm * fy;
         vz += dtfm * fz;
@@ -245,8 +255,13 @@ __global__ __launch_bounds__(BLOCK) void k_postPre(AtomsView a, int64_t n, doubl
     }
     if (live)
     {
-        a.pos[i] = p;
-        if (sortedPos) sortedPos[slotOf[i]] = p;
+        if (SLOTS)
+            sortedPos[slot] = p;
+        else
+        {
+            a.pos[i] = p;
+            if (sortedPos) sortedPos[slotOf[i]] = p;
+        }
         a.vel[i] = vx;
         a.vel[a.cap + i] = vy;
         a.vel[2 * a.cap + i] = vz;
@@ -365,7 +380,7 @@ uint64_t* dispHost(mrmd_atoms* a, const double* maxDisp, uint32_t* tag)
 // the fused post/pre step of time loops (k_postPre); synchronises for the displacement
 int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta, double temperature, uint64_t seed,
                            uint64_t call, double* maxDisp, double4* sortedPos, const int32_t* slotOf,
-                           const mrmd_molecules* adrMol, const double* tfForce, const HistD* tfHd)
+                           const mrmd_molecules* adrMol, const double* tfForce, const HistD* tfHd, bool slotsOnly)
 {
     MRMD_API_BEGIN(a ? a->ctx : nullptr)
     REQUIRE(a && maxDisp, "invalid arguments");
@@ -393,6 +408,14 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
                     "the fused AdResS step needs Langevin steps, a molecule per local atom and a type column per "
                     "atom type");
             uni ? launch(k_postPre<true, true, true>) : launch(k_postPre<true, false, true>);
+        }
+        else if (slotsOnly)
+        {
+            REQUIRE(sortedPos && slotOf, "positions in slot order need the slot copy and map");
+            if (langevin)
+                uni ? launch(k_postPre<true, true, false, true>) : launch(k_postPre<true, false, false, true>);
+            else
+                uni ? launch(k_postPre<false, true, false, true>) : launch(k_postPre<false, false, false, true>);
         }
         else if (langevin)
             uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);

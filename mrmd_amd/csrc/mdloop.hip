@@ -28,11 +28,18 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     const double rn = s->neighbor_radius;
     double delta[3];
     for (int d = 0; d < 3; ++d) delta[d] = rn;
-#define MD_CALL(expr)                         \
-    do                                        \
-    {                                         \
-        const int rc_ = (expr);               \
-        if (rc_ != MRMD_OK) return rc_;       \
+    // posInSlots: the current positions of every record are in the list's slot-order copy
+    // alone (the positions array is stale); it is brought back before the call returns
+    bool posInSlots = false;
+#define MD_CALL(expr)                                             \
+    do                                                            \
+    {                                                             \
+        const int rc_ = (expr);                                   \
+        if (rc_ != MRMD_OK)                                       \
+        {                                                         \
+            if (posInSlots) vl->positionsFromSlots(a->pos.p);     \
+            return rc_;                                           \
+        }                                                         \
     } while (0)
     // a step's post-force integration and the next step's pre-force integration run as one
     // fused pass (fusedPostPre, bit-identical to the two calls); the first pre-force and the
@@ -49,7 +56,11 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     // the start of every force call.  Here the passes that move the atoms write that copy
     // themselves: the fused post/pre pass stores each local atom's new position at its slot,
     // the ghost update each ghost's, and a rebuild's binning gathers it; the force call then
-    // skips its gather (same values, so the same bits).
+    // skips its gather (same values, so the same bits).  Between rebuilds the slot-order copy
+    // is the only one kept: the post/pre pass and the ghost update read and store positions
+    // at the slots alone, the rebuild's PeriodicMapping reads them from there into the
+    // positions array, and the call's end copies them back (same values, one pass fewer of
+    // position stores per step).
     bool localsScattered = false;  // the last pre-force pass wrote every local atom's slot
     for (int64_t k = 0; k < nsteps; ++k)
     {
@@ -59,15 +70,21 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         if (s->max_disp >= s->skin * 0.5)
         {
             s->max_disp = 0.0;
+            // the positions, from the slots when they are kept there, back in the positions array
+            const double4* from = posInSlots ? vl->bin.sortedPos.p : nullptr;
+            const int32_t* slotOf = posInSlots ? vl->slotOf.p : nullptr;
             if (cells)
             {
                 // PeriodicMapping and the cell list in one pass over the positions
-                cellListBuildWrapped(cells, a, box, delta);
+                cellListBuildWrapped(cells, a, box, delta, from, slotOf);
                 // the ghosts are made again and every force computed again before either is read
                 permuteAtoms(a, cells, false);
             }
+            else if (from)
+                MD_CALL(haloWrapFromSlots(a, box, from, slotOf));
             else
                 MD_CALL(mrmd_halo_wrap(a, box));
+            posInSlots = false;  // the ghosts are made again from the real atoms
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
             MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, s->grid_min, s->grid_max,
@@ -77,11 +94,18 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         }
         else if (localsScattered && vl->nrows == a->nlocal + a->nghost)
         {
-            MD_CALL(ghostUpdateSorted(gl, a, box, vl->bin.sortedPos.p, vl->ensureSlotOf()));
+            MD_CALL(ghostUpdateSorted(gl, a, box, vl->bin.sortedPos.p, vl->ensureSlotOf(), posInSlots));
             fresh = true;
         }
         else
+        {
+            if (posInSlots)
+            {
+                vl->positionsFromSlots(a->pos.p);
+                posInSlots = false;
+            }
             MD_CALL(mrmd_halo_update(gl, a, box));
+        }
         vl->sortedFresh = fresh;
         if (ljAssignable(lj) && vl->begin == 0 && vl->end == a->nlocal)
         {
@@ -104,13 +128,28 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         if (k + 1 < nsteps)
         {
             const bool scatter = vl->ntiles > 0 && vl->nrows == a->nlocal + a->nghost;
+            // the full-row force call has just read every record's position at its slot (its
+            // own gather or the copies above), so the slots hold the current positions
+            const bool slotsOnly = scatter && ljFullRows(lj);
+            if (posInSlots && !slotsOnly)
+            {
+                vl->positionsFromSlots(a->pos.p);
+                posInSlots = false;
+            }
             MD_CALL(fusedPostPre(a, s->dt, s->langevin != 0, s->zeta, s->temperature, s->seed, s->call, &d,
-                                 scatter ? vl->bin.sortedPos.p : nullptr, scatter ? vl->ensureSlotOf() : nullptr));
+                                 scatter ? vl->bin.sortedPos.p : nullptr, scatter ? vl->ensureSlotOf() : nullptr,
+                                 nullptr, nullptr, nullptr, slotsOnly));
             localsScattered = scatter;
+            if (slotsOnly) posInSlots = true;
         }
         else
             MD_CALL(mrmd_vv_post(a, s->dt));
         ++s->steps;
+    }
+    if (posInSlots)
+    {
+        vl->positionsFromSlots(a->pos.p);
+        posInSlots = false;
     }
 #undef MD_CALL
     MRMD_API_END

@@ -31,7 +31,8 @@ GridD makeGrid(const double* gmin, const double* gmax, const double* delta)
 
 namespace
 {
-// WRAP: each position is first mapped into the box (PeriodicMapping) and written back
+// WRAP: each position is first mapped into the box (PeriodicMapping) and written back; its
+// current value is wb.from[wb.slotOf[i]] when wb.from is set
 template <bool WRAP>
 __global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g, int32_t* cellOf, int32_t* counts,
                            double4* wrapPos, WrapBox wb)
@@ -41,7 +42,7 @@ __global__ void k_binCount(const double4* pos, int64_t begin, int64_t n, GridD g
     int c = -1;
     if (valid)
     {
-        double4 p = pos[begin + k];
+        double4 p = (WRAP && wb.from) ? wb.from[wb.slotOf[begin + k]] : pos[begin + k];
         if (WRAP)
         {
             wrapInto(p, wb.min, wb.max, wb.diam);
@@ -274,7 +275,8 @@ namespace mrmd_hip
 {
 // PeriodicMapping of the real atoms fused with the cell list build over them (the LJ time
 // loop's rebuild: the wrap and the binning's pass over the positions are one pass)
-void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta)
+void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta,
+                          const double4* from, const int32_t* slotOf)
 {
     KTimer t(cl->ctx, "cell_list_build");
     WrapBox wb;
@@ -284,6 +286,8 @@ void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box
         wb.max[d] = box->max[d];
         wb.diam[d] = box->max[d] - box->min[d];
     }
+    wb.from = from;
+    wb.slotOf = slotOf;
     cl->bin.build(cl->ctx, a->pos.p, 0, a->nlocal, makeGrid(box->min, box->max, delta), false, false, a->pos.p, &wb);
 }
 

@@ -82,7 +82,8 @@ namespace mrmd_hip
 // recomputes both before reading them)
 void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full);
 // PeriodicMapping of the real atoms and the cell list build over them in one pass
-void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta);
+void cellListBuildWrapped(mrmd_cell_list* cl, mrmd_atoms* a, const mrmd_box* box, const double* delta,
+                          const double4* from = nullptr, const int32_t* slotOf = nullptr);
 }
 
 namespace mrmd_hip
@@ -393,6 +394,9 @@ struct mrmd_verlet_list
 
     /// copy the current positions into sorted-slot order (bin.sortedPos) for the tiled kernels
     void refreshSorted(const double4* pos);
+    /// the inverse: pos[record] = bin.sortedPos[slot] for every binned record (time loops that
+    /// kept the positions in slot order alone)
+    void positionsFromSlots(double4* pos);
     /// slotOf, inverted from bin.sorted when the list was rebuilt since the last call
     const int32_t* ensureSlotOf();
     /// materialise the global-index full rows from the tiled list (no-op when current)

@@ -473,6 +473,12 @@ __global__ void k_gatherSorted(const double4* pos, const int32_t* sorted, int64_
     if (s < n) out[s] = pos[sorted[s]];
 }
 
+__global__ void k_scatterSorted(const double4* sortedPos, const int32_t* sorted, int64_t n, double4* pos)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) pos[sorted[k]] = sortedPos[k];
+}
+
 __global__ void k_invertSorted(const int32_t* sorted, int64_t n, int32_t* slotOf)
 {
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -746,6 +752,14 @@ void mrmd_verlet_list::refreshSorted(const double4* pos)
     if (nrows > 0)
         hipLaunchKernelGGL(k_gatherSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, pos, bin.sorted.p, nrows,
                            bin.sortedPos.p);
+    checkLaunch();
+}
+
+void mrmd_verlet_list::positionsFromSlots(double4* pos)
+{
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_scatterSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, bin.sortedPos.p,
+                           bin.sorted.p, nrows, pos);
     checkLaunch();
 }
 

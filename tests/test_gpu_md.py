@@ -110,3 +110,7 @@ def test_native_loop_equals_step_calls(M, ctx, langevin, permute, masses):
     assert a.atoms.nghost == b.atoms.nghost
     for f in (M.POS, M.VEL, M.FORCE):
         assert np.array_equal(a.local(f), b.local(f)), f
+    # the ghost records' positions too (the native loop keeps positions in the list's slot
+    # order between rebuilds and copies them back when the call returns)
+    n = a.atoms.nlocal + a.atoms.nghost
+    assert np.array_equal(a.atoms.download(M.POS, n), b.atoms.download(M.POS, n))
