@@ -62,6 +62,9 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     // positions array, and the call's end copies them back (same values, one pass fewer of
     // position stores per step).
     bool localsScattered = false;  // the last pre-force pass wrote every local atom's slot
+    // the ghost records' forces are zero: nothing between two rebuilds writes them on the
+    // store-form path, so they are filled once after each ghost creation
+    bool ghostForcesZero = false;
     for (int64_t k = 0; k < nsteps; ++k)
     {
         if (s->langevin) ++s->call;
@@ -87,6 +90,7 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
             posInSlots = false;  // the ghosts are made again from the real atoms
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
+            ghostForcesZero = false;  // the new ghosts carry their atoms' forces
             MD_CALL(mrmd_verlet_build(vl, a, 0, a->nlocal, rn, s->cell_ratio, s->grid_min, s->grid_max,
                                       s->max_neigh));
             ++s->rebuilds;
@@ -110,16 +114,19 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         if (ljAssignable(lj) && vl->begin == 0 && vl->end == a->nlocal)
         {
             // the force kernel stores the rows' forces (0.0 + F, what zeroing and adding
-            // leave); only the ghost records need their zero.  A caller's list whose rows are
-            // not all local atoms (possible before the loop's first rebuild) takes the
-            // fill-and-add path, so that no local atom keeps an old force
-            MD_CALL(fillForceRange(a, a->nlocal, a->nlocal + a->nghost));
+            // leave) and writes no ghost record; only the ghost records need their zero,
+            // once per ghost creation.  A caller's list whose rows are not all local atoms
+            // (possible before the loop's first rebuild) takes the fill-and-add path, so that
+            // no local atom keeps an old force
+            if (!ghostForcesZero) MD_CALL(fillForceRange(a, a->nlocal, a->nlocal + a->nghost));
+            ghostForcesZero = true;
             MD_CALL(ljApplyAssign(lj, a, vl));
         }
         else
         {
             MD_CALL(mrmd_atoms_fill(a, MRMD_FORCE, 0.0));
             MD_CALL(mrmd_lj_apply(lj, a, vl, nullptr));
+            ghostForcesZero = false;
         }
         // the full-row kernel leaves every ghost force at the zero just filled in, so the
         // fold (which adds only non-zero ghost forces) would change nothing

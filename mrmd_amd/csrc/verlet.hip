@@ -99,16 +99,38 @@ __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets,
 // wave and then a workgroup reduction, so that only a few hundred workgroups post atomics
 // to the six words.
 constexpr int BOX_BLOCKS = 512;
+// q = c / n and r = c % n for 0 <= c < 2^31, 1 <= n < 2^31 (rn = 1.0 / n): the quotient from
+// a double product, off by at most one, corrected by the remainder's range
+__device__ inline int divCell(int c, int n, double rn, int& r)
+{
+    int q = int(double(c) * rn);
+    r = c - q * n;
+    if (r < 0)
+    {
+        --q;
+        r += n;
+    }
+    else if (r >= n)
+    {
+        ++q;
+        r -= n;
+    }
+    return q;
+}
 __global__ __launch_bounds__(BLOCK) void k_rowCellBox(const int32_t* cellOf, int64_t rowLo, int64_t rowHi, int n1,
                                                        int n2, int32_t* box)
 {
     __shared__ int red[6][BLOCK / WAVE];
     int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {-1, -1, -1};
+    const double r1 = 1.0 / double(n1), r2 = 1.0 / double(n2);
     for (int64_t k = rowLo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < rowHi;
          k += int64_t(gridDim.x) * blockDim.x)
     {
         const int c = cellOf[k];
-        const int v[3] = {(c / n2) / n1, (c / n2) % n1, c % n2};
+        int kc, jc;
+        const int ij = divCell(c, n2, r2, kc);
+        const int ic = divCell(ij, n1, r1, jc);
+        const int v[3] = {ic, jc, kc};
         for (int d = 0; d < 3; ++d)
         {
             lo[d] = min(lo[d], v[d]);

@@ -114,3 +114,5 @@ def test_native_loop_equals_step_calls(M, ctx, langevin, permute, masses):
     # order between rebuilds and copies them back when the call returns)
     n = a.atoms.nlocal + a.atoms.nghost
     assert np.array_equal(a.atoms.download(M.POS, n), b.atoms.download(M.POS, n))
+    # and their forces (zero on both sides; the native loop fills them once per ghost creation)
+    assert np.array_equal(a.atoms.download(M.FORCE, n), b.atoms.download(M.FORCE, n))
