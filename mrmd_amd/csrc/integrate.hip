@@ -306,23 +306,25 @@ __global__ void k_scaleVel(AtomsView a, int64_t n, double beta)
     a.vel[2 * a.cap + i] *= beta;
 }
 
-// per-workgroup maxima of the squared displacement, then their maximum (one workgroup;
-// eight independent loads per lane in flight)
+// per-workgroup maxima of the squared displacement, then their maximum (one workgroup of
+// 1,024 lanes, eight independent loads per lane in flight: a few rounds of loads even at
+// millions of atoms, whose latency, not bandwidth, bounds this kernel)
 // host (if not null) is coherent pinned host memory that receives the result as two tagged
 // words (low and high half of the double, each with the call's tag in its high half): the
 // host takes the value as soon as both words carry the tag, without waiting for the stream
-__global__ __launch_bounds__(BLOCK) void k_maxPartials(const double* partials, int64_t n, double* out, uint64_t* host,
-                                                       uint32_t tag)
+constexpr int MAXP_THREADS = 1024;
+__global__ __launch_bounds__(MAXP_THREADS) void k_maxPartials(const double* partials, int64_t n, double* out,
+                                                              uint64_t* host, uint32_t tag)
 {
-    __shared__ double red[BLOCK / WAVE];
+    __shared__ double red[MAXP_THREADS / WAVE];
     double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int64_t k = threadIdx.x;
-    for (; k + 7 * BLOCK < n; k += 8 * BLOCK)
+    for (; k + 7 * MAXP_THREADS < n; k += 8 * MAXP_THREADS)
     {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], partials[k + u * BLOCK]);
+        for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], partials[k + u * MAXP_THREADS]);
     }
-    for (; k < n; k += BLOCK) m[0] = fmax(m[0], partials[k]);
+    for (; k < n; k += MAXP_THREADS) m[0] = fmax(m[0], partials[k]);
     double r = fmax(fmax(fmax(m[0], m[1]), fmax(m[2], m[3])), fmax(fmax(m[4], m[5]), fmax(m[6], m[7])));
     r = blockMax(r, red);
     if (threadIdx.x == 0)
@@ -421,7 +423,7 @@ int mrmd_hip::fusedPostPre(mrmd_atoms* a, double dt, bool langevin, double zeta,
             uni ? launch(k_postPre<true, true>) : launch(k_postPre<true, false>);
         else
             uni ? launch(k_postPre<false, true>) : launch(k_postPre<false, false>);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(MAXP_THREADS), 0, ctx->stream, part, int64_t(grid), ms, host,
                            tag);
     }
     checkLaunch();
@@ -445,7 +447,7 @@ int mrmd_vv_pre(mrmd_atoms* a, double dt, double* maxDisp)
     {
         KTimer t(ctx, "vv_pre");
         hipLaunchKernelGGL(k_vvPre, dim3(grid), dim3(BLOCK), 0, ctx->stream, a->view(), a->nlocal, dt, part);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(MAXP_THREADS), 0, ctx->stream, part, int64_t(grid), ms, host,
                            tag);
     }
     else
@@ -494,7 +496,7 @@ int mrmd_langevin_pre(mrmd_atoms* a, double dt, double zeta, double temperature,
             uni ? launch(k_langevinPre<true, true>) : launch(k_langevinPre<true, false>);
         else
             uni ? launch(k_langevinPre<false, true>) : launch(k_langevinPre<false, false>);
-        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(BLOCK), 0, ctx->stream, part, int64_t(grid), ms, host,
+        hipLaunchKernelGGL(k_maxPartials, dim3(1), dim3(MAXP_THREADS), 0, ctx->stream, part, int64_t(grid), ms, host,
                            tag);
     }
     checkLaunch();
