@@ -362,6 +362,8 @@ void mrmd_atoms::setSize(int64_t n)
         hipLaunchKernelGGL(k_zeroAtoms, dim3(gridFor(n - size)), dim3(BLOCK), 0, ctx->stream, view(), size, n);
         checkLaunch();
         if (massValue != 0.0) massUniform = false;
+        if (chargeValue != 0.0) chargeUniform = false;
+        if (relmassValue != 0.0) relmassUniform = false;
     }
     size = n;
 }
@@ -626,26 +628,29 @@ int mrmd_atoms_upload(mrmd_atoms* a, int field, const void* host, int64_t off, i
                                reinterpret_cast<const int64_t*>(a->stage.p), off, n);
             break;
         case MRMD_MASS:
+        case MRMD_CHARGE:
+        case MRMD_RELMASS:
         {
-            HIPCHECK(hipMemcpyAsync(a->mass.p + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            double* dst = field == MRMD_MASS ? a->mass.p : (field == MRMD_CHARGE ? a->charge.p : a->relmass.p);
+            bool& uniform = field == MRMD_MASS ? a->massUniform
+                                               : (field == MRMD_CHARGE ? a->chargeUniform : a->relmassUniform);
+            double& value = field == MRMD_MASS ? a->massValue
+                                               : (field == MRMD_CHARGE ? a->chargeValue : a->relmassValue);
+            HIPCHECK(hipMemcpyAsync(dst + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
             const double* m = static_cast<const double*>(host);
             bool same = true;
             for (int64_t k = 1; k < n && same; ++k) same = m[k] == m[0];
+            if (n == 0)
+                break;
             if (off == 0 && n >= a->size)
             {
-                a->massUniform = same;
-                a->massValue = m[0];
+                uniform = same;
+                value = m[0];
             }
             else
-                a->massUniform = a->massUniform && same && m[0] == a->massValue;
+                uniform = uniform && same && m[0] == value;
             break;
         }
-        case MRMD_CHARGE:
-            HIPCHECK(hipMemcpyAsync(a->charge.p + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            break;
-        case MRMD_RELMASS:
-            HIPCHECK(hipMemcpyAsync(a->relmass.p + off, a->stage.p, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            break;
     }
     checkLaunch();
     sync(ctx);
@@ -730,9 +735,13 @@ int mrmd_atoms_fill(mrmd_atoms* a, int field, double value)
             break;
         case MRMD_CHARGE:
             hipLaunchKernelGGL(k_fill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->charge.p, n, value);
+            a->chargeUniform = true;
+            a->chargeValue = value;
             break;
         case MRMD_RELMASS:
             hipLaunchKernelGGL(k_fill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->relmass.p, n, value);
+            a->relmassUniform = true;
+            a->relmassValue = value;
             break;
     }
     checkLaunch();
@@ -764,9 +773,11 @@ int mrmd_atoms_device_ptr(mrmd_atoms* a, int field, void** ptr, int64_t* stride)
             break;
         case MRMD_CHARGE:
             p = a->charge.p;
+            a->chargeUniform = false;
             break;
         case MRMD_RELMASS:
             p = a->relmass.p;
+            a->relmassUniform = false;
             break;
         default:
             fail(MRMD_ERR_ARG, "field %d has no separate device array", field);

@@ -47,6 +47,9 @@ struct mrmd_atoms
     /// Langevin step compute its mass-dependent factors once)
     bool massUniform = true;
     double massValue = 0.0;
+    /// the same for charge and relmass (a permutation leaves a uniform field as it is)
+    bool chargeUniform = true, relmassUniform = true;
+    double chargeValue = 0.0, relmassValue = 0.0;
     mrmd_hip::DBuf<double4> pos;
     mrmd_hip::DBuf<double> vel, force, mass, charge, relmass;
     mrmd_hip::DBuf<double> stage;  ///< transfer staging

@@ -230,7 +230,9 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
 namespace
 {
 // every field of records [lo, hi) copied unchanged (the records a permutation leaves alone)
-__global__ void k_copyRecords(AtomsView src, AtomsView dst, int64_t lo, int64_t hi)
+// scalars: bit 0 mass, bit 1 charge, bit 2 relmass are moved (a field of one value over all
+// records stays where it is)
+__global__ void k_copyRecords(AtomsView src, AtomsView dst, int64_t lo, int64_t hi, unsigned scalars)
 {
     const int64_t i = lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= hi) return;
@@ -240,14 +242,16 @@ __global__ void k_copyRecords(AtomsView src, AtomsView dst, int64_t lo, int64_t 
         dst.vel[c * dst.cap + i] = src.vel[c * src.cap + i];
         dst.force[c * dst.cap + i] = src.force[c * src.cap + i];
     }
-    dst.mass[i] = src.mass[i];
-    dst.charge[i] = src.charge[i];
-    dst.relmass[i] = src.relmass[i];
+    if (scalars & 1u) dst.mass[i] = src.mass[i];
+    if (scalars & 2u) dst.charge[i] = src.charge[i];
+    if (scalars & 4u) dst.relmass[i] = src.relmass[i];
 }
 
 // Cabana::permute: new[begin + k] = old[perm[k]] for every field (FORCE: forces too)
+// (scalars as for k_copyRecords)
 template <bool FORCE>
-__global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* perm, int64_t n, int64_t begin)
+__global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* perm, int64_t n, int64_t begin,
+                                unsigned scalars)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -258,9 +262,9 @@ __global__ void k_permuteGather(AtomsView src, AtomsView dst, const int32_t* per
         dst.vel[c * dst.cap + d] = src.vel[c * src.cap + s];
         if (FORCE) dst.force[c * dst.cap + d] = src.force[c * src.cap + s];
     }
-    dst.mass[d] = src.mass[s];
-    dst.charge[d] = src.charge[s];
-    dst.relmass[d] = src.relmass[s];
+    if (scalars & 1u) dst.mass[d] = src.mass[s];
+    if (scalars & 2u) dst.charge[d] = src.charge[s];
+    if (scalars & 4u) dst.relmass[d] = src.relmass[s];
 }
 
 template <class T>
@@ -302,26 +306,29 @@ void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full)
     mrmd_ctx* ctx = a->ctx;
     KTimer t(ctx, "atoms_permute");
     // gather into the spare buffers (same capacity), then swap them in; the old buffers
-    // become the spares of the next permutation
+    // become the spares of the next permutation.  A mass, charge or relmass field of one
+    // value over all records is the same after any permutation: it is neither moved nor
+    // swapped
+    const unsigned scalars = (a->massUniform ? 0u : 1u) | (a->chargeUniform ? 0u : 2u) | (a->relmassUniform ? 0u : 4u);
     a->sparePos.reserve(ctx, a->cap);
     a->spareVel.reserve(ctx, 3 * a->cap);
     a->spareForce.reserve(ctx, 3 * a->cap);
-    a->spareMass.reserve(ctx, a->cap);
-    a->spareCharge.reserve(ctx, a->cap);
-    a->spareRelmass.reserve(ctx, a->cap);
+    if (scalars & 1u) a->spareMass.reserve(ctx, a->cap);
+    if (scalars & 2u) a->spareCharge.reserve(ctx, a->cap);
+    if (scalars & 4u) a->spareRelmass.reserve(ctx, a->cap);
     if (full)
         hipLaunchKernelGGL(k_permuteGather<true>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin);
+                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin, scalars);
     else
         hipLaunchKernelGGL(k_permuteGather<false>, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin);
+                           a->spareView(), cl->bin.sorted.p, n, cl->bin.begin, scalars);
     // records outside [begin, end) are unchanged: copy them across
     const int64_t b = cl->bin.begin, e = cl->bin.end, s = a->size;
     auto copyRange = [&](int64_t lo, int64_t hi)
     {
         if (hi <= lo) return;
         hipLaunchKernelGGL(k_copyRecords, dim3(gridFor(hi - lo)), dim3(BLOCK), 0, ctx->stream, a->view(),
-                           a->spareView(), lo, hi);
+                           a->spareView(), lo, hi, scalars);
     };
     if (full)
     {
@@ -332,9 +339,9 @@ void permuteAtoms(mrmd_atoms* a, const mrmd_cell_list* cl, bool full)
     a->pos.swap(a->sparePos);
     a->vel.swap(a->spareVel);
     a->force.swap(a->spareForce);
-    a->mass.swap(a->spareMass);
-    a->charge.swap(a->spareCharge);
-    a->relmass.swap(a->spareRelmass);
+    if (scalars & 1u) a->mass.swap(a->spareMass);
+    if (scalars & 2u) a->charge.swap(a->spareCharge);
+    if (scalars & 4u) a->relmass.swap(a->spareRelmass);
 }
 }  // namespace mrmd_hip
 

@@ -258,10 +258,26 @@ def test_cell_list_and_permute_bit_exact(M, ctx, espp_positions):
     occ = O.Cells(c, 0, c.nlocal, delta, [0, 0, 0], [L_ESPP] * 3)
     for a, b in zip(gcl.download(), occ.get()):
         np.testing.assert_array_equal(a, b)
+    n = g.nlocal
+    idx = np.arange(n, dtype=float)
+    g.upload(M.MASS, 1.0 + 1e-6 * idx)    # per-record values: moved with the records
+    g.upload(M.CHARGE, -idx)
+    g.fill(M.RELMASS, 0.5)                # one value for all: left where it is
     g.permute(gcl)
     occ.permute(c)
     np.testing.assert_array_equal(g.download(M.POS), c.get(O.POS))
     np.testing.assert_array_equal(g.download(M.VEL), c.get(O.VEL))
+    src = g.download(M.VEL)[:, 0] / 3.0   # record k came from record src[k] (vel = arange)
+    np.testing.assert_array_equal(g.download(M.MASS), 1.0 + 1e-6 * src)
+    np.testing.assert_array_equal(g.download(M.CHARGE), -src)
+    np.testing.assert_array_equal(g.download(M.RELMASS), np.full(n, 0.5))
+    # a field that stops being uniform moves again on the next permutation (the same order
+    # again: the records are already in cell order, so the permutation is the identity)
+    g.upload(M.RELMASS, 2.0 + idx)
+    gcl.build(g, 0, n, delta, [0, 0, 0], [L_ESPP] * 3)
+    g.permute(gcl)
+    np.testing.assert_array_equal(g.download(M.RELMASS), 2.0 + idx)
+    np.testing.assert_array_equal(g.download(M.CHARGE), -src)
 
 
 def test_ghost_update_and_fold(M, ctx, espp_positions):
