@@ -160,12 +160,29 @@ __global__ __launch_bounds__(SORT_CELLS) void k_binSortCells(const int32_t* offs
 }
 
 // (z, index) order inside each cell: one group of 32 lanes per cell (sortCellByZ)
+// outPos (if not null): each slot's record copied there once the cell is in order (the
+// sorted-slot positions), and slotOf (if not null) the slot of each record
 __global__ __launch_bounds__(BLOCK) void k_sortCellsZ(const int32_t* offsets, const int32_t* counts, int32_t* sorted,
-                                                      int ncell, const double4* __restrict__ pos)
+                                                      int ncell, const double4* __restrict__ pos, double4* outPos,
+                                                      int32_t* slotOf)
 {
     const int64_t c = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SORT_GROUP;
     if (c >= ncell) return;
-    sortCellByZ(sorted + offsets[c], counts[c], pos, int(threadIdx.x % SORT_GROUP));
+    const int lane = int(threadIdx.x % SORT_GROUP);
+    const int base = offsets[c], n = counts[c];
+    int32_t* recs = sorted + base;
+    sortCellByZ(recs, n, pos, lane);
+    if (!outPos) return;
+    // every lane reads back only the places it wrote itself; a crowded cell, sorted by lane 0
+    // alone in memory, is copied by lane 0 alone
+    const bool serial = n > 4 * SORT_GROUP;
+    if (serial && lane != 0) return;
+    for (int e = serial ? 0 : lane; e < n; e += serial ? 1 : SORT_GROUP)
+    {
+        const int32_t v = recs[e];
+        outPos[base + e] = pos[v];
+        if (slotOf) slotOf[v] = base + e;
+    }
 }
 
 __global__ void k_gatherPos(const double4* pos, const int32_t* sorted, int64_t n, double4* out)
@@ -182,7 +199,7 @@ __global__ void k_zeroI32(int32_t* p, int64_t n)
 }  // namespace
 
 void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
-                    bool zOrder, double4* wrapPos, const WrapBox* wrap)
+                    bool zOrder, double4* wrapPos, const WrapBox* wrap, int32_t* slotOfOut)
 {
     grid = g;
     begin = b;
@@ -210,18 +227,18 @@ void Binning::build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, con
     {
         hipLaunchKernelGGL(k_binFill, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, cellOf.p, b, n, offsets.p,
                            fill.p, sorted.p);
+        if (withPos) sortedPos.ensure(ctx, n);
         if (zOrder)
+            // the z sort also writes the sorted-slot positions (and the slot of each record)
             hipLaunchKernelGGL(k_sortCellsZ, dim3(unsigned((int64_t(ncell) * SORT_GROUP + BLOCK - 1) / BLOCK)),
-                               dim3(BLOCK), 0, ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos);
+                               dim3(BLOCK), 0, ctx->stream, offsets.p, counts.p, sorted.p, ncell, pos,
+                               withPos ? sortedPos.p : nullptr, withPos ? slotOfOut : nullptr);
         else
             hipLaunchKernelGGL(k_binSortCells, dim3((ncell + SORT_CELLS - 1) / SORT_CELLS), dim3(SORT_CELLS), 0,
                                ctx->stream, offsets.p, counts.p, sorted.p, ncell);
-    }
-    if (withPos && n > 0)
-    {
-        sortedPos.ensure(ctx, n);
-        hipLaunchKernelGGL(k_gatherPos, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, sorted.p, n,
-                           sortedPos.p);
+        if (withPos && !zOrder)
+            hipLaunchKernelGGL(k_gatherPos, dim3(gridFor(n)), dim3(BLOCK), 0, ctx->stream, pos, sorted.p, n,
+                               sortedPos.p);
     }
     checkLaunch();
 }

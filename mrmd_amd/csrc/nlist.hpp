@@ -63,9 +63,11 @@ struct Binning
     DBuf<double4> sortedPos;
 
     /// wrap (if given): every position is first mapped into that box and written to
-    /// wrapPos (the same array as pos for an in-place wrap)
+    /// wrapPos (the same array as pos for an in-place wrap); slotOfOut (z order with
+    /// positions, begin 0): the slot of each record, written by the z sort
     void build(mrmd_ctx* ctx, const double4* pos, int64_t b, int64_t e, const GridD& g, bool withPos,
-               bool zOrder = false, double4* wrapPos = nullptr, const WrapBox* wrap = nullptr);
+               bool zOrder = false, double4* wrapPos = nullptr, const WrapBox* wrap = nullptr,
+               int32_t* slotOfOut = nullptr);
 };
 }  // namespace mrmd_hip
 

@@ -662,12 +662,14 @@ bool verletBuild(mrmd_verlet_list* vl, const double4* pos, int64_t nall, int64_t
     const double delta[3] = {gs, gs, gs};
     const GridD g = makeGrid(gmin, gmax, delta);
     vl->sentinel = std::max({gmax[0], gmax[1], gmax[2]}) + SENTINEL_GAP + 4.0 * rn;
-    vl->bin.build(ctx, pos, 0, nall, g, true, true);
+    // the binning's z sort also writes each record's slot (slotOf)
+    vl->slotOf.ensure(ctx, std::max<int64_t>(nall, 1));
+    vl->bin.build(ctx, pos, 0, nall, g, true, true, nullptr, nullptr, vl->slotOf.p);
     rowRegion(vl);
     vl->globalValid = false;
     vl->halfValid = false;
     vl->banded = vl->bandLo <= vl->bandHi;
-    vl->slotOfValid = false;
+    vl->slotOfValid = true;
     vl->sortedFresh = false;
     // tile shape: a x a columns (a <= 3; 1 when the stencil reaches two cells), height K
     // chosen to stage ~1650 records; shrink until every tile fits in LDS
