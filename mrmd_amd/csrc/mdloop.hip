@@ -16,6 +16,27 @@
 
 using namespace mrmd_hip;
 
+namespace
+{
+// positions kept in a Verlet list's slot-order copy alone (held) are copied back to the
+// positions array by release() or, when a call leaves early, by the destructor
+struct SlotPositions
+{
+    mrmd_verlet_list* vl;
+    mrmd_atoms* a;
+    bool held = false;
+    void release()
+    {
+        if (held) vl->positionsFromSlots(a->pos.p);
+        held = false;
+    }
+    ~SlotPositions()
+    {
+        if (held) vl->positionsFromSlotsNoThrow(a->pos.p);
+    }
+};
+}  // namespace
+
 extern "C" {
 
 int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_lj* lj, mrmd_cell_list* cells,
@@ -28,18 +49,15 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
     const double rn = s->neighbor_radius;
     double delta[3];
     for (int d = 0; d < 3; ++d) delta[d] = rn;
-    // posInSlots: the current positions of every record are in the list's slot-order copy
-    // alone (the positions array is stale); it is brought back before the call returns
-    bool posInSlots = false;
-#define MD_CALL(expr)                                             \
-    do                                                            \
-    {                                                             \
-        const int rc_ = (expr);                                   \
-        if (rc_ != MRMD_OK)                                       \
-        {                                                         \
-            if (posInSlots) vl->positionsFromSlots(a->pos.p);     \
-            return rc_;                                           \
-        }                                                         \
+    // slots.held: the current positions of every record are in the list's slot-order copy
+    // alone (the positions array is stale); they are copied back before the call returns,
+    // also when it returns with an error
+    SlotPositions slots{vl, a};
+#define MD_CALL(expr)                         \
+    do                                        \
+    {                                         \
+        const int rc_ = (expr);               \
+        if (rc_ != MRMD_OK) return rc_;       \
     } while (0)
     // a step's post-force integration and the next step's pre-force integration run as one
     // fused pass (fusedPostPre, bit-identical to the two calls); the first pre-force and the
@@ -74,20 +92,25 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         {
             s->max_disp = 0.0;
             // the positions, from the slots when they are kept there, back in the positions array
-            const double4* from = posInSlots ? vl->bin.sortedPos.p : nullptr;
-            const int32_t* slotOf = posInSlots ? vl->slotOf.p : nullptr;
+            const double4* from = slots.held ? vl->bin.sortedPos.p : nullptr;
+            const int32_t* slotOf = slots.held ? vl->slotOf.p : nullptr;
+            // (the real atoms' positions are current in the positions array after the wrap; the
+            // ghosts are made again from them)
             if (cells)
             {
                 // PeriodicMapping and the cell list in one pass over the positions
                 cellListBuildWrapped(cells, a, box, delta, from, slotOf);
+                slots.held = false;
                 // the ghosts are made again and every force computed again before either is read
                 permuteAtoms(a, cells, false);
             }
             else if (from)
+            {
                 MD_CALL(haloWrapFromSlots(a, box, from, slotOf));
+                slots.held = false;
+            }
             else
                 MD_CALL(mrmd_halo_wrap(a, box));
-            posInSlots = false;  // the ghosts are made again from the real atoms
             int64_t ng = 0;
             MD_CALL(mrmd_halo_create(gl, a, box, &ng));
             ghostForcesZero = false;  // the new ghosts carry their atoms' forces
@@ -98,16 +121,12 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
         }
         else if (localsScattered && vl->nrows == a->nlocal + a->nghost)
         {
-            MD_CALL(ghostUpdateSorted(gl, a, box, vl->bin.sortedPos.p, vl->ensureSlotOf(), posInSlots));
+            MD_CALL(ghostUpdateSorted(gl, a, box, vl->bin.sortedPos.p, vl->ensureSlotOf(), slots.held));
             fresh = true;
         }
         else
         {
-            if (posInSlots)
-            {
-                vl->positionsFromSlots(a->pos.p);
-                posInSlots = false;
-            }
+            slots.release();
             MD_CALL(mrmd_halo_update(gl, a, box));
         }
         vl->sortedFresh = fresh;
@@ -138,26 +157,18 @@ int mrmd_md_run(mrmd_atoms* a, mrmd_ghost_layer* gl, mrmd_verlet_list* vl, mrmd_
             // the full-row force call has just read every record's position at its slot (its
             // own gather or the copies above), so the slots hold the current positions
             const bool slotsOnly = scatter && ljFullRows(lj);
-            if (posInSlots && !slotsOnly)
-            {
-                vl->positionsFromSlots(a->pos.p);
-                posInSlots = false;
-            }
+            if (!slotsOnly) slots.release();
             MD_CALL(fusedPostPre(a, s->dt, s->langevin != 0, s->zeta, s->temperature, s->seed, s->call, &d,
                                  scatter ? vl->bin.sortedPos.p : nullptr, scatter ? vl->ensureSlotOf() : nullptr,
                                  nullptr, nullptr, nullptr, slotsOnly));
             localsScattered = scatter;
-            if (slotsOnly) posInSlots = true;
+            if (slotsOnly) slots.held = true;
         }
         else
             MD_CALL(mrmd_vv_post(a, s->dt));
         ++s->steps;
     }
-    if (posInSlots)
-    {
-        vl->positionsFromSlots(a->pos.p);
-        posInSlots = false;
-    }
+    slots.release();
 #undef MD_CALL
     MRMD_API_END
 }

@@ -399,6 +399,7 @@ struct mrmd_verlet_list
     /// the inverse: pos[record] = bin.sortedPos[slot] for every binned record (time loops that
     /// kept the positions in slot order alone)
     void positionsFromSlots(double4* pos);
+    void positionsFromSlotsNoThrow(double4* pos) noexcept;
     /// slotOf, inverted from bin.sorted when the list was rebuilt since the last call
     const int32_t* ensureSlotOf();
     /// materialise the global-index full rows from the tiled list (no-op when current)

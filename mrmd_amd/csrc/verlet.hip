@@ -787,6 +787,14 @@ void mrmd_verlet_list::positionsFromSlots(double4* pos)
     checkLaunch();
 }
 
+// (the error paths of the time loop: a launch error is left for the caller's next call)
+void mrmd_verlet_list::positionsFromSlotsNoThrow(double4* pos) noexcept
+{
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_scatterSorted, dim3(gridFor(nrows)), dim3(BLOCK), 0, ctx->stream, bin.sortedPos.p,
+                           bin.sorted.p, nrows, pos);
+}
+
 const int32_t* mrmd_verlet_list::ensureSlotOf()
 {
     if (!slotOfValid)
