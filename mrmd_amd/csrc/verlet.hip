@@ -52,11 +52,31 @@ __device__ inline int64_t tileOfCell(const TileGeom& tg, int cell)
 }
 
 // one thread per tile: row ranges, neighbourhood column ranges and their LDS bases
+// the maximum of v over the 64 lanes of the wave (every lane active)
+__device__ inline int waveMaxInt(int v)
+{
+    for (int off = WAVE / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+
+__device__ void tileSetup(const TileGeom& tg, int64_t t, const int32_t* offsets, const int32_t* counts,
+                          TileInfo* tiles, int& stagedOut);
+
+// one lane per tile; the largest staged neighbourhood goes to *maxStaged with one atomic per
+// wave
 __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets, const int32_t* counts,
                             TileInfo* tiles, int32_t* maxStaged)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
+    int staged = 0;
+    if (t < ntiles) tileSetup(tg, t, offsets, counts, tiles, staged);
+    staged = waveMaxInt(staged);
+    if (threadIdx.x % WAVE == 0 && staged > 0) atomicMax(maxStaged, staged);
+}
+
+__device__ void tileSetup(const TileGeom& tg, int64_t t, const int32_t* offsets, const int32_t* counts,
+                          TileInfo* tiles, int& stagedOut)
+{
     int i0, j0, k0, k1;
     tileCoords(tg, t, i0, j0, k0, k1);
     TileInfo& ti = tiles[t];
@@ -91,7 +111,7 @@ __global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets,
     ti.staged = staged;
     ti.nself = nself;
     ti.nrows = nrows;
-    atomicMax(maxStaged, staged);
+    stagedOut = staged;
 }
 
 // bounding box of the cells of the list rows (records [rowLo, rowHi) of the binning):
@@ -123,18 +143,27 @@ __global__ __launch_bounds__(BLOCK) void k_rowCellBox(const int32_t* cellOf, int
     __shared__ int red[6][BLOCK / WAVE];
     int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {-1, -1, -1};
     const double r1 = 1.0 / double(n1), r2 = 1.0 / double(n2);
-    for (int64_t k = rowLo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < rowHi;
-         k += int64_t(gridDim.x) * blockDim.x)
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    // eight independent loads per lane in flight, then their cells
+    constexpr int U = 8;
+    for (int64_t k0 = rowLo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k0 < rowHi; k0 += U * stride)
     {
-        const int c = cellOf[k];
-        int kc, jc;
-        const int ij = divCell(c, n2, r2, kc);
-        const int ic = divCell(ij, n1, r1, jc);
-        const int v[3] = {ic, jc, kc};
-        for (int d = 0; d < 3; ++d)
+        int c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = k0 + u * stride < rowHi ? cellOf[k0 + u * stride] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
         {
-            lo[d] = min(lo[d], v[d]);
-            hi[d] = max(hi[d], v[d]);
+            if (c[u] < 0) continue;
+            int kc, jc;
+            const int ij = divCell(c[u], n2, r2, kc);
+            const int ic = divCell(ij, n1, r1, jc);
+            const int v[3] = {ic, jc, kc};
+            for (int d = 0; d < 3; ++d)
+            {
+                lo[d] = min(lo[d], v[d]);
+                hi[d] = max(hi[d], v[d]);
+            }
         }
     }
     for (int d = 0; d < 3; ++d)
@@ -462,7 +491,9 @@ __global__ __launch_bounds__(BLOCK) void k_verletTiled(const double4* __restrict
         cnt[s] = min(width, cap);  // the true width goes to maxCnt; reads stay inside the list
         localMax = max(localMax, width);
     }
-    if (localMax > 0) atomicMax(maxCnt, localMax);
+    // one atomic per wave (every lane of the workgroup reaches this point)
+    localMax = waveMaxInt(localMax);
+    if (threadIdx.x % WAVE == 0 && localMax > 0) atomicMax(maxCnt, localMax);
 }
 
 // tiled rows (per sorted slot, LDS slots) -> global rows (per record begin+r, record indices)
