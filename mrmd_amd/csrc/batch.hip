@@ -229,15 +229,38 @@ __global__ __launch_bounds__(BLOCK) void k_bPre(BS b)
         const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
         d2 = dx * dx + dy * dy + dz * dz;
     }
-    // wave-aggregated maximum when the whole wave belongs to one replica
+    // the maximum aggregated over the workgroup's waves of one replica each (one atomic per
+    // run of equal replicas among them); a wave of several replicas posts per lane
+    __shared__ double wmax[BLOCK / WAVE];
+    __shared__ int wrep[BLOCK / WAVE];
     const int r0 = __shfl(r, 0, WAVE);
-    if (__all(r == r0 || r < 0) && r0 >= 0)
+    const bool single = __all(r == r0 || r < 0) && r0 >= 0;
+    const double mx = waveMax(d2);
+    if (threadIdx.x % WAVE == 0)
     {
-        const double mx = waveMax(d2);
-        if (threadIdx.x % WAVE == 0) atomicMaxNonNeg(&b.dispStep[r0], mx);
+        wmax[threadIdx.x / WAVE] = mx;
+        wrep[threadIdx.x / WAVE] = single ? r0 : -1;
     }
-    else if (valid)
-        atomicMaxNonNeg(&b.dispStep[r], d2);
+    if (!single && valid) atomicMaxNonNeg(&b.dispStep[r], d2);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int cur = -1;
+        double m = 0.0;
+        for (int w = 0; w < BLOCK / WAVE; ++w)
+        {
+            if (wrep[w] < 0) continue;
+            if (wrep[w] != cur)
+            {
+                if (cur >= 0) atomicMaxNonNeg(&b.dispStep[cur], m);
+                cur = wrep[w];
+                m = wmax[w];
+            }
+            else
+                m = fmax(m, wmax[w]);
+        }
+        if (cur >= 0) atomicMaxNonNeg(&b.dispStep[cur], m);
+    }
 }
 
 // rebuild criterion per replica (examples/02_LennardJones_NVE/02_LennardJones_NVE.cpp:138-165)
