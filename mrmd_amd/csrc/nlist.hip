@@ -91,7 +91,17 @@ __global__ __launch_bounds__(SORT_CELLS) void k_binSortCells(const int32_t* offs
     const int c = c0 + int(threadIdx.x);
     if (r1 - r0 <= SORT_RUN)
     {
-        for (int k = threadIdx.x; k < r1 - r0; k += SORT_CELLS) idx[k] = sorted[r0 + k];
+        // eight independent loads per lane in flight
+        const int len = r1 - r0;
+        for (int k0 = threadIdx.x; k0 < len; k0 += 8 * SORT_CELLS)
+        {
+            int32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = k0 + u * SORT_CELLS < len ? sorted[r0 + k0 + u * SORT_CELLS] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + u * SORT_CELLS < len) idx[k0 + u * SORT_CELLS] = v[u];
+        }
         if (c < c1)
         {
             const int o = offsets[c] - r0, n = counts[c];

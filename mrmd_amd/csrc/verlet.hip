@@ -64,7 +64,7 @@ __device__ void tileSetup(const TileGeom& tg, int64_t t, const int32_t* offsets,
 
 // one lane per tile; the largest staged neighbourhood goes to *maxStaged with one atomic per
 // wave
-__global__ void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets, const int32_t* counts,
+__global__ __launch_bounds__(BLOCK) void k_tileSetup(TileGeom tg, int64_t ntiles, const int32_t* offsets, const int32_t* counts,
                             TileInfo* tiles, int32_t* maxStaged)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -83,29 +83,48 @@ __device__ void tileSetup(const TileGeom& tg, int64_t t, const int32_t* offsets,
     const int span = tg.cols + 2 * tg.R;
     const int kl = max(k0 - tg.R, 0), kh = min(k1 + tg.R, tg.n2);
     const int iEnd = min(i0 + tg.cols, tg.o0 + tg.e0), jEnd = min(j0 + tg.cols, tg.o1 + tg.e1);
+    // every range's table entries are loaded first (independent loads in flight together),
+    // then the ranges are laid out
+    int first[MAX_RANGES], last[MAX_RANGES], lastCount[MAX_RANGES], selfFirst[MAX_RANGES], selfLast[MAX_RANGES],
+        selfCount[MAX_RANGES];
+#pragma unroll
+    for (int q = 0; q < MAX_RANGES; ++q)
+    {
+        const int a = q / 5, b = q % 5;  // (a, b) of a span x span neighbourhood, span <= 5
+        const int ii = i0 - tg.R + a, jj = j0 - tg.R + b;
+        const bool in = a < span && b < span && ii >= 0 && ii < tg.n0 && jj >= 0 && jj < tg.n1;
+        const bool self = in && ii >= i0 && ii < iEnd && jj >= j0 && jj < jEnd;
+        const int col = (ii * tg.n1 + jj) * tg.n2;
+        first[q] = in ? offsets[col + kl] : 0;
+        last[q] = in ? offsets[col + kh - 1] : 0;
+        lastCount[q] = in ? counts[col + kh - 1] : 0;
+        selfFirst[q] = self ? offsets[col + k0] : 0;
+        selfLast[q] = self ? offsets[col + k1 - 1] : 0;
+        selfCount[q] = self ? counts[col + k1 - 1] : -1;
+    }
     int nr = 0, staged = 0, nself = 0, nrows = 0;
-    for (int a = 0; a < span; ++a)
-        for (int b = 0; b < span; ++b, ++nr)
+#pragma unroll
+    for (int q = 0; q < MAX_RANGES; ++q)
+    {
+        const int a = q / 5, b = q % 5;
+        if (a >= span || b >= span) continue;
+        const int ii = i0 - tg.R + a, jj = j0 - tg.R + b;
+        ti.base[nr] = staged;
+        ti.begin[nr] = first[q];
+        if (ii >= 0 && ii < tg.n0 && jj >= 0 && jj < tg.n1)
         {
-            const int ii = i0 - tg.R + a, jj = j0 - tg.R + b;
-            ti.base[nr] = staged;
-            ti.begin[nr] = 0;
-            if (ii < 0 || ii >= tg.n0 || jj < 0 || jj >= tg.n1) continue;
-            const int c0 = (ii * tg.n1 + jj) * tg.n2 + kl;
-            const int c1 = (ii * tg.n1 + jj) * tg.n2 + (kh - 1);
-            ti.begin[nr] = offsets[c0];
-            staged += offsets[c1] + counts[c1] - offsets[c0];
-            if (ii >= i0 && ii < iEnd && jj >= j0 && jj < jEnd)
+            staged += last[q] + lastCount[q] - first[q];
+            if (selfCount[q] >= 0)
             {
-                const int r0 = (ii * tg.n1 + jj) * tg.n2 + k0;
-                const int r1 = (ii * tg.n1 + jj) * tg.n2 + (k1 - 1);
                 ti.selfRange[nself] = nr;
-                ti.selfBegin[nself] = offsets[r0];
+                ti.selfBegin[nself] = selfFirst[q];
                 ti.selfPrefix[nself] = nrows;
-                nrows += offsets[r1] + counts[r1] - offsets[r0];
+                nrows += selfLast[q] + selfCount[q] - selfFirst[q];
                 ++nself;
             }
         }
+        ++nr;
+    }
     ti.selfPrefix[nself] = nrows;
     ti.nr = nr;
     ti.staged = staged;
